@@ -1,0 +1,497 @@
+"""Pins for the CPU oracle: each test checks it against something the oracle's own
+code does not restate -- the reference's shipped wavelet file, closed forms,
+invariants of the stencil, exact schedules derived independently, a numpy FFT,
+brute force, or physics (analytic Green's function, finite-difference
+gradient check)."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from fwiedu import synth
+from fwiedu.records import Array2D, fptr
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+C = np.array([0.77793115, -0.17388691, 0.02338713], np.float32)
+D = np.array([-2.8215452, 1.57663238, -0.18347238, 0.01761260], np.float32)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+# --------------------------------------------------------------------------- stencil
+def _unit_coefs(Nx, Nz, irx=0.0, irz=0.0, ir=1.0):
+    one = np.ones((Nx, Nz), np.float32)
+    z = np.zeros((Nx, Nz), np.float32)
+    # p2 = c1*p1 + c2*p0 + c3*K*(irx*dpdx + irz*dpdz + ir*(pxx + pzz))
+    return dict(ir=one * ir, irx=one * irx, irz=one * irz, K=one, c1=z, c2=z, c3=one)
+
+
+def _sweep(o, p1, p0, cf, dx, dz, zs=3):
+    Nx, Nz = p1.shape
+    p2 = np.zeros_like(p1)
+    o.fd_step(fptr(p2), fptr(p1), fptr(p0), fptr(cf["ir"]), fptr(cf["irx"]), fptr(cf["irz"]),
+              fptr(cf["K"]), fptr(cf["c1"]), fptr(cf["c2"]), fptr(cf["c3"]), Nx, Nz,
+              1.0 / dx, 1.0 / dz, 1.0 / (dx * dx), 1.0 / (dz * dz), 3, Nx - 4, zs, Nz - 4)
+    return p2
+
+
+def test_stencil_coefficient_sums():
+    # consistency of the optimised (non-Taylor) coefficients, propagation.c:47-53
+    assert abs(float(D[0]) + 2 * float(D[1:].sum())) < 2e-7
+    assert abs(2 * sum((k + 1) * float(C[k]) for k in range(3)) - 1.00063744) < 1e-7
+    assert abs(2 * sum((k + 1) ** 2 * float(D[k + 1]) for k in range(3)) - 2.00251252) < 1e-7
+
+
+@pytest.mark.parametrize("axis", ["x", "z"])
+def test_stencil_polynomials(oracles, axis):
+    """Applied to z, z^2, x, x^2 the stencil returns the coefficient moments."""
+    o = oracles["f64"]
+    Nx, Nz, h = 16, 18, 2.0
+    ix, iz = np.meshgrid(np.arange(Nx), np.arange(Nz), indexing="ij")
+    s = (ix if axis == "x" else iz).astype(np.float64)
+    Cd, Dd = C.astype(np.float64), D.astype(np.float64)  # the FP32 constants, exactly
+    m0 = Dd[0] + 2 * Dd[1:].sum()                        # ~1e-8, not 0, in FP32 constants
+    first = 2 * sum((k + 1) * Cd[k] for k in range(3)) / h
+    second = 2 * sum((k + 1) ** 2 * Dd[k + 1] for k in range(3)) / (h * h)
+    # Laplacian of s^2 (ir = 1, no first-derivative terms); both axes see m0 * p
+    p1 = (s * s).astype(np.float32)
+    out = _sweep(o, p1, np.zeros_like(p1), _unit_coefs(Nx, Nz), h, h)
+    want = second + 2 * m0 * s * s / (h * h)
+    assert np.allclose(out[3:-3, 3:-3], want[3:-3, 3:-3], rtol=1.2e-7, atol=0)  # FP32 output
+    assert abs(second * h * h - 2.00251252) < 1e-7 and abs(first * h - 1.00063744) < 1e-7
+    # first derivative of s through invrhox / invrhoz (ir = 0)
+    p1 = s.astype(np.float32)
+    cf = _unit_coefs(Nx, Nz, irx=1.0 if axis == "x" else 0.0, irz=1.0 if axis == "z" else 0.0, ir=0.0)
+    out = _sweep(o, p1, np.zeros_like(p1), cf, h, h)
+    assert np.allclose(out[3:-3, 3:-3], first, rtol=1.2e-7, atol=0)
+
+
+def test_stencil_plane_wave_symbol(oracles):
+    """cos(k z): the second-derivative stencil multiplies it by its symbol
+    (D0 + 2 sum D_m cos(k m h)) / h^2."""
+    o = oracles["f64"]
+    Nx, Nz, h, k = 10, 64, 1.0, 0.7
+    iz = np.arange(Nz)[None, :].repeat(Nx, 0)
+    p1 = np.cos(k * iz).astype(np.float32)
+    out = _sweep(o, p1, np.zeros_like(p1), _unit_coefs(Nx, Nz), h, h)
+    Dd = D.astype(np.float64)
+    sym = Dd[0] + 2 * sum(Dd[m] * np.cos(k * m) for m in (1, 2, 3))
+    m0 = Dd[0] + 2 * Dd[1:].sum()  # the x part of the Laplacian of a z-only field
+    want = (sym + m0) * p1.astype(np.float64)
+    # inputs are cos rounded to FP32: allow sum|D| * 2^-24 on top of the output rounding
+    assert np.abs(out[3:-3, 3:-3] - want[3:-3, 3:-3]).max() < 5e-7
+
+
+def test_stencil_time_terms_and_bounds(oracles):
+    """c1*p1 + c2*p0 with a zero Laplacian; cells outside the loop bounds stay put."""
+    o = oracles["f32s"]
+    Nx, Nz = 12, 14
+    rng = np.random.default_rng(0)
+    p1 = np.full((Nx, Nz), 3.0, np.float32)  # constant: all derivatives vanish
+    p0 = rng.standard_normal((Nx, Nz)).astype(np.float32)
+    cf = _unit_coefs(Nx, Nz)
+    cf["c1"] = np.full((Nx, Nz), 2.0, np.float32)
+    cf["c2"] = np.full((Nx, Nz), -1.0, np.float32)
+    out = _sweep(o, p1, p0, cf, 1.0, 1.0, zs=5)
+    inner = (slice(3, Nx - 3), slice(5, Nz - 3))
+    assert np.allclose(out[inner], 6.0 - p0[inner], atol=2e-6)
+    mask = np.ones((Nx, Nz), bool)
+    mask[inner] = False
+    assert np.all(out[mask] == 0)
+
+
+def test_freesurface_mirror(oracles):
+    o = oracles["f32"]
+    Nx, Nz, pml = 5, 20, 6
+    p = np.random.default_rng(1).standard_normal((Nx, Nz)).astype(np.float32)
+    q = p.copy()
+    o.freesurface(fptr(q), Nx, Nz, pml)
+    assert np.all(q[:, pml] == 0)
+    for k in range(1, pml + 1):
+        assert np.array_equal(q[:, pml - k], -p[:, pml + k])
+    assert np.array_equal(q[:, pml + 1:], p[:, pml + 1:])
+
+
+# --------------------------------------------------------------------------- set-up
+def test_pad_model_edges(oracles):
+    o = oracles["f32"]
+    nz, nx, nl = 4, 3, 2
+    m = np.arange(nx * nz, dtype=np.float32).reshape(nx, nz)
+    out = np.zeros((nx + 2 * nl, nz + 2 * nl), np.float32)
+    o.pad_model(fptr(m), nz, nx, fptr(out), nl)
+    want = np.pad(m, ((nl, nl), (nl, nl)), mode="edge")
+    assert np.array_equal(out, want)
+
+
+@pytest.mark.parametrize("nlayer,R", [(30, 2.6004e-6), (40, 1.0e-6)])
+def test_meal_closed_form(oracles, nlayer, R):
+    """Homogeneous v0: D in the outermost layer is -(1+pd) v0 ln R / (2 n h), the
+    weight profile is 2^((l/n)^2.5) - 1 and D = 0, Beta = 1 inside."""
+    o = oracles["f64"]
+    v0, h, fp = 2000.0, 10.0, 10.0
+    NZ, NX = 2 * nlayer + 8, 2 * nlayer + 10
+    vp = np.full((NX, NZ), v0, np.float32)
+    Dm = np.zeros_like(vp)
+    Bm = np.zeros_like(vp)
+    o.meal_coefs(fptr(Dm), fptr(Bm), fptr(vp), fp, h, h, NZ, NX, nlayer)
+    Rc = 10.0 ** (-(np.log10(nlayer) - 1) / np.log10(2) - 3 - 1)
+    assert abs(Rc - R) / R < 1e-3
+    d0 = -3.0 * v0 * np.log(Rc) / (2 * nlayer * h)
+    beta0 = 2.0 * (v0 / np.sqrt(3.0)) / (h * fp) / 10.0
+    mid = NX // 2
+    for l in range(1, nlayer + 1):
+        w = 2.0 ** ((l / nlayer) ** 2.5) - 1.0
+        for iz in (nlayer - l, NZ - nlayer - 1 + l):  # top and bottom rows
+            assert abs(Dm[mid, iz] - d0 * w) <= 1e-6 * d0
+            assert abs(Bm[mid, iz] - (1 + (beta0 - 1) * w)) <= 1e-6 * beta0
+    assert np.all(Dm[nlayer:NX - nlayer, nlayer:NZ - nlayer] == 0)
+    assert np.all(Bm[nlayer:NX - nlayer, nlayer:NZ - nlayer] == 1)
+    assert abs(Dm[0, 0] - np.sqrt(2) * d0) <= 1e-6 * d0  # corner: both edges at l = n
+
+
+def test_setup_coefficients(oracles):
+    """Interior of a homogeneous model: D = 0 so coef1 = 2, coef2 = -1, coef3 = dt^2;
+    K = rho v^2; grad(1/rho) = 0.  A density ramp has grad(1/rho) from the
+    12-point optimised stencil: its coefficient moment is 2*sum(k C_k)."""
+    o = oracles["f64"]
+    nz, nx, pml, h, dt = 20, 24, 8, 10.0, 0.001
+    vp = np.full((nx, nz), 2500.0, np.float32)
+    ramp = (1000.0 + 5.0 * np.arange(nx))[:, None].repeat(nz, 1).astype(np.float32)
+    Nz, Nx = nz + 2 * pml, nx + 2 * pml
+    out = {k: np.zeros((Nx, Nz), np.float32) for k in ("D", "B", "c1", "c2", "c3", "K", "ir", "irx", "irz")}
+    o.setup_coefs(fptr(vp), fptr(ramp), nx, nz, pml, h, h, dt, 10.0,
+                  *[fptr(out[k]) for k in ("D", "B", "c1", "c2", "c3", "K", "ir", "irx", "irz")])
+    inner = (slice(pml, pml + nx), slice(pml, pml + nz))
+    deep = (slice(pml + 6, pml + nx - 6), slice(pml + 6, pml + nz - 6))  # 12-point reach
+    assert np.allclose(out["c1"][inner], 2.0) and np.allclose(out["c2"][inner], -1.0)
+    assert np.allclose(out["c3"][inner], dt * dt, rtol=1e-6)
+    rp = np.pad(ramp, pml, mode="edge")
+    assert np.allclose(out["K"][inner], (rp * 2500.0 ** 2)[inner], rtol=1e-6)
+    assert np.all(out["irz"][deep] == 0.0)
+    C12 = [0.91067892, -0.34187892, 0.13833962, -0.04880710, 0.01302148, -0.00199047]
+    ir = 1.0 / rp.astype(np.float64)
+    xs = slice(pml + 6, pml + nx - 6)
+    want = sum(C12[k] * (ir[pml + 6 + k + 1:pml + nx - 6 + k + 1] - ir[pml + 6 - k - 1:pml + nx - 6 - k - 1])
+               for k in range(6)) / h
+    assert np.allclose(out["irx"][xs, pml:pml + nz], want[:, pml:pml + nz], rtol=1e-5)
+
+
+# --------------------------------------------------------------------------- checkpoints
+# (d, nCheckPoint, ntPoint, recompute sweeps) derived independently by simulating
+# checkpoint.c:47-97 and the reverse loop propagation.c:638-673
+PLANS = {1500: (55, 55, 108, 1445), 4001: (89, 89, 176, 3912),
+         5000: (100, 100, 198, 4901), 8000: (126, 127, 251, 7873)}
+
+
+@pytest.mark.parametrize("nt", sorted(PLANS))
+def test_checkpoint_plan(oracles, nt):
+    assert oracles["f32"].checkpoint_plan(nt) == PLANS[nt]
+    d, n, ntp = synth.checkpoint_plan(nt)
+    assert (d, n, ntp, synth.recompute_steps(nt)) == PLANS[nt]
+
+
+def test_checkpoint_plan_brute_force(oracles):
+    """Every step is stored once by the forward pass or reconstructed once."""
+    for nt in (1, 2, 3, 7, 20, 101, 240, 999):
+        d, nc, ntp, r = oracles["f32"].checkpoint_plan(nt)
+        stored_fwd = {it for k in range(0, nt, d) for it in (k - 1, k) if 0 <= it < nt}
+        assert r == nt - len(stored_fwd)
+        assert nc == sum(1 for it in range(nt) if it % d == 0 or (it + 1) % d == 0)
+        assert ntp == nc + d - 2
+
+
+def test_memory_cost(oracles):
+    d, nc, ntp, _ = oracles["f32"].checkpoint_plan(4001)
+    assert abs(oracles["f32"].memory_cost_checkpoint(4001, 681, 301) - ntp * 681 * 301 * 4 / 1024 ** 3) < 1e-6
+
+
+def _run(o, w, vp, rho, problem, obs, pre=None, grad=None, fullstore=False, precondition=1):
+    vpa, rha = Array2D(vp), Array2D(rho)
+    g = grad.pp if grad is not None else None
+    args = (vpa.pp, rha.pp, obs.recs, w.nshot, w.nz, w.nx, w.dx, w.dz, w.dt, w.nt, w.pml,
+            w.freesurface, w.freq, float(w.filter), precondition)
+    if fullstore:
+        o.gradient_fullstore(*args, pre.recs, g)
+    else:
+        o.propagation(*args, problem, (pre or obs).recs, g)
+
+
+def _observed(o, w):
+    obs = w.shots()
+    _run(o, w, w.vp_true, w.rho_true, 0, obs)
+    return obs
+
+
+@pytest.mark.parametrize("fs", [0, 1])
+def test_checkpoint_gradient_equals_store_everything(oracles, fs):
+    """FP64, no free surface: reconstruction repeats the forward pass exactly, so the
+    checkpointed gradient equals the store-every-field gradient bit for bit.  With
+    the free surface the reconstruction order (FD, mirror, source) differs from
+    the forward one (FD, source, mirror), which the oracle keeps."""
+    o = oracles["f64"]
+    w = synth.workload("tiny", freesurface=fs, shot_depth_cells=1)
+    obs = _observed(o, w)
+    g = {}
+    for full in (False, True):
+        pre = obs.header_copy()
+        grad = Array2D(np.zeros((w.nx, w.nz), np.float32))
+        _run(o, w, w.vp_start, w.rho_start, 1, obs, pre, grad, fullstore=full, precondition=0)
+        g[full] = grad.a.copy()
+    if fs == 0:
+        assert np.array_equal(g[False], g[True])
+    else:
+        assert 1e-5 < rel(g[False], g[True]) < 0.2
+
+
+def test_record_first_sample_zero_and_reciprocity(oracles):
+    """Records sample p1 at level it, so sample 0 is zero; with equal source and
+    receiver depth, swapping source and receiver columns gives the same trace
+    in a homogeneous medium (acoustic reciprocity, scaled by K at each end)."""
+    o = oracles["f64"]
+    w = synth.workload("tiny", nshot=2)
+    w.vp_true[:] = 2000.0
+    w.rho_true[:] = 1000.0
+    obs = _observed(o, w)
+    cols = w.shot_columns()
+    t0 = obs.trace(0)[cols[1]]
+    t1 = obs.trace(1)[cols[0]]
+    assert np.all(obs.trace(0)[:, 0] == 0)
+    assert rel(t0, t1) < 1e-6
+
+
+def test_point_source_matches_2d_greens_function(oracles):
+    """Homogeneous medium: the modelled trace at offset r equals K * (G2D * s),
+    G2D = H(t - r/c) / (2 pi c sqrt(c^2 t^2 - r^2)), before any boundary echo."""
+    o = oracles["f64"]
+    c, h, dt = 1500.0, 5.0, 0.0005
+    w = synth.workload("tiny", nz=90, nx=90, dx=h, dz=h, dt=dt, nt=700, pml=30, nshot=1,
+                       shot_depth_cells=45, rcv_depth_cells=45, f0=25.0, t0=0.05)
+    w.vp_true[:] = c
+    w.rho_true[:] = 1000.0
+    obs = _observed(o, w)
+    sc = w.shot_columns()[0]
+    off = 30
+    trace = obs.trace(0)[sc + off].astype(np.float64)
+    r = off * h
+    # the source added at step it appears in p1 at level it+1 and is sampled from there
+    tt = (np.arange(w.nt)) * dt
+    K = 1000.0 * c * c
+    n_sub = 8
+    tau = (np.arange(w.nt * n_sub) + 0.5) * dt / n_sub
+    G = np.where(c * tau > r, 1.0 / (2 * np.pi * c * np.sqrt(np.maximum(c * c * tau * tau - r * r, 1e-30))), 0.0)
+    s_fine = np.interp(tau, tt + dt, w.src.astype(np.float64), left=0.0, right=0.0)
+    pred = K * np.convolve(s_fine, G)[: w.nt * n_sub] * (dt / n_sub)
+    pred = pred[n_sub - 1::n_sub][: w.nt]
+    win = slice(0, int((r + 2 * 30 * h) / c / dt))  # before the first boundary echo
+    assert rel(trace[win], pred[win]) < 0.08
+
+
+def test_gradient_matches_finite_difference_directions(oracles):
+    """FP64, no preconditioning, no taper effects at the test scale: the
+    directional derivative of the misfit along random slowness perturbations is
+    proportional (positive factor) to <grad, ds>."""
+    o = oracles["f64"]
+    w = synth.workload("tiny", nt=300)
+    obs = _observed(o, w)
+    pre = obs.header_copy()
+    grad = Array2D(np.zeros((w.nx, w.nz), np.float32))
+    _run(o, w, w.vp_start, w.rho_start, 1, obs, pre, grad, precondition=0)
+    s0 = 1.0 / w.vp_start.astype(np.float64)
+    rng = np.random.default_rng(3)
+    fd, an = [], []
+    for _ in range(5):
+        ds = rng.standard_normal(s0.shape)
+        ds[:, :w.waterdepth] = 0
+        from scipy.ndimage import gaussian_filter
+        ds = gaussian_filter(ds, 2.0) * 1e-3 * s0.mean()
+        vals = []
+        for sgn in (1, -1):
+            vp = (1.0 / (s0 + sgn * ds)).astype(np.float32)
+            p = obs.header_copy()
+            _run(o, w, vp, w.rho_start, 2, obs, p)
+            vals.append(o.objective(p.recs, w.nshot))
+        fd.append((vals[0] - vals[1]) / 2.0)
+        an.append(float((grad.a.astype(np.float64) * ds).sum()))
+    fd, an = np.array(fd), np.array(an)
+    corr = np.corrcoef(fd, an)[0, 1]
+    assert corr > 0.95, (fd, an)
+    assert np.all(np.sign(fd) == np.sign(an))
+
+
+# --------------------------------------------------------------------------- wavelet
+def test_ricker_fixture_and_peak_frequency(oracles):
+    ref = np.fromfile(os.path.join(GOLDEN, "ricker_6001_1ms_10Hz_delay0.15ms.bin"), np.float32)
+    assert np.array_equal(synth.ricker(6001, 0.001, 10.0, 0.15), ref)
+    fp = oracles["f32"].peak_frequency(fptr(ref), ref.size, 0.001)
+    assert fp == np.float32(60) * (np.float32(1) / (np.float32(6001) * np.float32(0.001)))
+    assert abs(fp - 9.99833) < 1e-4
+    assert fp == np.float32(synth.peak_frequency(ref, 0.001))
+
+
+# --------------------------------------------------------------------------- optimisation
+def _rec_set(arrs):
+    from fwiedu.records import Record2D
+    recs = (Record2D * len(arrs))()
+    keep = []
+    for i, a in enumerate(arrs):
+        t = Array2D(a)
+        keep.append(t)
+        recs[i].ntr, recs[i].ns = a.shape
+        recs[i].traces_p = t.pp
+    return recs, keep
+
+
+def test_objective_and_step_closed_forms(oracles):
+    o = oracles["f32"]
+    r0 = [np.full((3, 5), 2.0, np.float32), np.full((2, 4), -1.0, np.float32)]
+    recs, k0 = _rec_set(r0)
+    assert o.objective(recs, 2) == pytest.approx(0.5 * (15 * 4 + 8 * 1))
+    r1 = [a * np.float32(0.5) for a in r0]  # r0 - r1 = r0/2  ->  step = 2
+    recs1, k1 = _rec_set(r1)
+    assert o.step_length(recs, recs1, 2) == 2.0
+    assert o.step_length(recs, recs, 2) == 0.0  # zero denominator returns 0
+
+
+def test_preconditioning_closed_forms(oracles):
+    o = oracles["f32"]
+    g = np.linspace(-1, 1, 50).astype(np.float32)
+    p = np.full(50, 3.0, np.float32)
+    g1, p1 = g.copy(), p.copy()
+    o.shot_precondition(fptr(g1), fptr(p1), 50)
+    assert np.allclose(g1, g / (8.0 / 7.0), rtol=1e-6) and np.allclose(p1, 3.0 * 7.0 / 8.0, rtol=1e-6)
+    g2 = g.copy()
+    o.global_precondition(fptr(g2), fptr(p), 50, 0.005)
+    assert np.allclose(g2, g / (3.0 * 1.005), rtol=1e-6)
+
+
+def test_model_perturbation_brute_force(oracles):
+    """Histogram rule of optimization.c:178-298 re-derived with numpy: bins of
+    width (max(0,g)-min(0,g))/1000, grow from the modal bin until 99% of the mass,
+    scale = mean(slowness)*fraction / max |bin edge|, out = -scale*g."""
+    o = oracles["f32"]
+    rng = np.random.default_rng(5)
+    g = (rng.standard_normal(4000) * 3.0 + 0.5).astype(np.float32)
+    slow = rng.uniform(2e-4, 6e-4, 4000).astype(np.float32)
+    out = np.zeros_like(g)
+    sc = o.model_perturbation(0.01, fptr(slow), fptr(g), fptr(out), g.size)
+    # the rule is stated in FP32 (bin width, bin index, mass accumulator), so is this
+    f32 = np.float32
+    lo_v, hi_v = min(f32(0), g.min()), max(f32(0), g.max())
+    width = f32((hi_v - lo_v) / 1000.0)
+    idx = (((g - lo_v) / width).astype(np.float64) + 0.5).astype(int)
+    hist = np.bincount(idx, minlength=1001)
+    hi = lo = int(np.argmax(hist))
+    n = f32(g.size)
+    acc = f32(hist[hi]) / n
+    while acc < f32(0.99):
+        if hi < 1000:
+            hi += 1
+            acc = f32(acc + f32(hist[hi]) / n)
+        if lo > 0:
+            lo -= 1
+            acc = f32(acc + f32(hist[lo]) / n)
+    edge = max(abs(f32(lo_v + f32(lo) * width)), abs(f32(lo_v + f32(hi) * width)))
+    want = slow.astype(np.float64).mean() * 0.01 / float(edge)
+    assert sc == pytest.approx(want, rel=1e-5)
+    assert np.allclose(out, -sc * g, rtol=1e-6)
+
+
+def test_taper_shape(oracles):
+    o = oracles["f32"]
+    nt, ntr = 30, 12
+    tr = Array2D(np.ones((ntr, nt), np.float32))
+    o.taper_traces(tr.pp, nt, ntr, 6, 0)
+    L = 6
+    cosw = 0.5 * (1 + np.cos(np.pi * np.arange(L + 1) / L))
+    assert np.allclose(tr.a[0, :L + 1], cosw[::-1], atol=1e-7)
+    assert np.allclose(tr.a[0, nt - L - 1:], cosw, atol=1e-7)
+    assert np.all(tr.a[:, L + 1:nt - L - 1] == 1.0)
+    tr2 = Array2D(np.ones((ntr, nt), np.float32))
+    o.taper_traces(tr2.pp, nt, ntr, 0, 100)  # clamped to ntr/3 = 4
+    cw = 0.5 * (1 + np.cos(np.pi * np.arange(5) / 4))
+    assert np.allclose(tr2.a[:5, 0], cw[::-1], atol=1e-7)
+    assert np.allclose(tr2.a[ntr - 5:, 0], cw, atol=1e-7)
+
+
+def test_smooth2d_properties(oracles):
+    """Constant fields are kept (weights are normalised); an impulse spreads into
+    the normalised cos^2 window along the chosen axis only."""
+    o = oracles["f32"]
+    nx, nz, N = 9, 25, 4
+    a = Array2D(np.full((nx, nz), 7.0, np.float32))
+    b = Array2D(np.zeros((nx, nz), np.float32))
+    o.smooth2d(a.pp, b.pp, nx, nz, N, 1)
+    assert np.allclose(b.a, 7.0, rtol=1e-6)
+    imp = np.zeros((nx, nz), np.float32)
+    imp[4, 12] = 1.0
+    a = Array2D(imp)
+    o.smooth2d(a.pp, b.pp, nx, nz, N, 1)
+    wts = np.cos(np.pi * (np.arange(2 * N + 1) - N) / (2.0 * N)) ** 2
+    assert np.allclose(b.a[4, 12 - N:12 + N + 1], wts[::-1] / wts.sum(), rtol=1e-5)
+    assert np.all(b.a[[0, 1, 2, 3, 5, 6, 7, 8]] == 0)
+
+
+def test_vp_helpers(oracles):
+    o = oracles["f32s"]  # the -ffast-math build divides through an approximate reciprocal
+    v = np.array([1500, 2000, 0.0, 4000], np.float32)
+    s = np.zeros_like(v)
+    o.v2slow(fptr(v), fptr(s), 4)
+    assert s[2] == 0 and s[0] == np.float32(1) / np.float32(1500)
+    rho = np.zeros(3, np.float32)
+    o.vp2density(fptr(v[[0, 1, 3]].copy()), fptr(rho), 3)
+    assert np.allclose(rho, 310 * np.array([1500, 2000, 4000.0]) ** 0.25, rtol=1e-6)
+    m = np.array([1, 5, 9], np.float32)
+    o.clip(fptr(m), 3, 2.0, 8.0)
+    assert list(m) == [2, 5, 8]
+
+
+# --------------------------------------------------------------------------- trace filters
+def _numpy_window_filter(x, win):
+    X = np.fft.rfft(x.astype(np.float64), axis=1)
+    X *= win[: X.shape[1]]
+    return np.fft.irfft(X, n=x.shape[1], axis=1)
+
+
+@pytest.mark.parametrize("kind", ["low", "band"])
+@pytest.mark.parametrize("nt", [64, 75])
+def test_trace_filters_vs_numpy_fft(oracles, kind, nt):
+    """FFTW r2c/c2r window filters (common.c:584-794), parity unpinned against
+    FFTW itself (absent); checked against numpy's FFT with the same window."""
+    o = oracles["f64"]
+    ntr, dt, fp, fc = 5, 0.004, 20.0, 40.0
+    x = np.random.default_rng(2).standard_normal((ntr, nt)).astype(np.float32)
+    a, b = Array2D(x.copy()), Array2D(np.zeros_like(x))
+    (o.lowpass_traces if kind == "low" else o.bandpass_traces)(a.pp, b.pp, nt, ntr, fp, fc, dt)
+    f = np.arange(nt) * (1.0 / (nt * dt))
+    if kind == "low":
+        win = np.where(f < fp, 1.0, np.where(f <= fc, 0.5 * (1 + np.cos(np.pi * (f - fp) / (fc - fp))), 0.0))
+    else:
+        win = np.where(f < fp, 0.5 * (1 + np.cos(np.pi * (fp - f) / fp)),
+                       np.where(f <= fc, 0.5 * (1 + np.cos(np.pi * (f - fp) / (fc - fp))), 0.0))
+    want = _numpy_window_filter(x, win)
+    assert np.abs(b.a - want).max() < 1e-5
+
+
+def test_interpolation_band_limited(oracles):
+    """Zero-stuffing interpolation by r reproduces a slow periodic signal."""
+    o = oracles["f64"]
+    ntr, nt, r = 2, 41, 2
+    ns = (nt - 1) * r + 1
+    t_out = np.arange(ns)
+    x_full = np.cos(2 * np.pi * 2 * t_out / ns)
+    x = x_full[::r].astype(np.float32)[None, :].repeat(ntr, 0)
+    a = Array2D(x)
+    b = Array2D(np.zeros((ntr, ns), np.float32))
+    o.interpolate_traces(a.pp, ntr, nt, r, b.pp)
+    X = np.zeros(ns, complex)
+    X[::1] = np.fft.fft(np.where(t_out % r == 0, x_full, 0.0))
+    keep = ns // (2 * r)
+    w = np.zeros(ns)
+    w[:keep] = r
+    w[ns - keep + 1:] = r
+    want = np.real(np.fft.ifft(X * w))
+    assert np.abs(b.a[0] - want).max() < 1e-5
