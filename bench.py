@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Benchmark of the FWI gradient hot path on one MI355X.
+
+One step = one FWI gradient (iso_acoustic2d_propagation problem 1 semantics:
+forward with checkpoints and pseudo-Hessian, adjoint source, reconstruction,
+adjoint and imaging, per-shot and global preconditioning) over all shots of the
+221 x 601 demo-shape workload (nt 4001, 120 shots x 601 receivers, pml 40, free
+surface), with the model and observed records resident in HBM (a
+fwiedu_session).  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--plain-shots 4] [--cpu-shots 4]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "fwi-edu_amd"), os.path.join(ROOT, "oracle"), ROOT]
+
+import numpy as np  # noqa: E402
+
+from fwiedu import lib as flib  # noqa: E402
+from fwiedu import synth  # noqa: E402
+from fwiedu.records import Array2D  # noqa: E402
+
+METRIC = "gradient shots/s at 221x601 (plain vs batched), % HBM roofline, p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def interior_cells(w):
+    Nz, Nx = w.nz + 2 * w.pml, w.nx + 2 * w.pml
+    zs = w.pml if w.freesurface else 3
+    return (Nx - 6) * (Nz - 3 - zs)
+
+
+def observed(L, w):
+    obs = w.shots()
+    flib.propagate(L, w, Array2D(w.vp_true), Array2D(w.rho_true), obs, 0)
+    return obs
+
+
+def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
+    L.fwiedu_set_batch(batch)
+    vp, rho = Array2D(w.vp_start), Array2D(w.rho_start)
+    sess = L.fwiedu_session_create(vp.pp, rho.pp, obs.recs, w.nshot, w.nz, w.nx, w.dx, w.dz, w.dt,
+                                   w.nt, w.pml, w.freesurface, w.freq, float(w.filter),
+                                   w.precondition)
+    for i in range(warmup):
+        L.fwiedu_session_gradient(sess, None)
+        log(f"warmup {i + 1}/{warmup} done")
+    L.fwiedu_reset_stats()
+    L.fwiedu_set_kernel_timing(ktime_every)
+    times = []
+    misfit = None
+    for i in range(steps):
+        t0 = time.perf_counter()
+        misfit = L.fwiedu_session_gradient(sess, None)  # returns after a stream sync
+        times.append(time.perf_counter() - t0)
+        log(f"step {i + 1}/{steps}: {times[-1] * 1e3:.1f} ms, misfit {misfit:.6g}")
+    L.fwiedu_set_kernel_timing(0)
+    stats = {k: flib.stat(k) for k in ("fd_launches", "fd_timed", "fd_ms_avg", "fd_cells_per_launch",
+                                       "fd_shots_per_launch", "batch", "device_bytes")}
+    L.fwiedu_session_destroy(sess)
+    L.fwiedu_set_batch(0)
+    return times, stats, misfit
+
+
+def cpu_baseline(w, nshots):
+    """The FP32 oracle (reference algorithm and compiler flags, OpenMP over all
+    host threads) on `nshots` gradient shots of the same workload."""
+    from oracle import Oracle
+    o = Oracle("f32")
+    sub = synth.workload(w.name, nshot=nshots)
+    obs = sub.shots()
+    vpt, rht = Array2D(sub.vp_true), Array2D(sub.rho_true)
+    o.propagation(vpt.pp, rht.pp, obs.recs, nshots, sub.nz, sub.nx, sub.dx, sub.dz, sub.dt, sub.nt,
+                  sub.pml, sub.freesurface, 0.0, 2.0, 1, 0, obs.recs, None)
+    vps, rhs = Array2D(sub.vp_start), Array2D(sub.rho_start)
+    pre = obs.header_copy()
+    g = Array2D(np.zeros((sub.nx, sub.nz), np.float32))
+    t0 = time.perf_counter()
+    o.propagation(vps.pp, rhs.pp, obs.recs, nshots, sub.nz, sub.nx, sub.dx, sub.dz, sub.dt, sub.nt,
+                  sub.pml, sub.freesurface, 0.0, 2.0, 1, 1, pre.recs, g.pp)
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": nshots / dt, "unit": "gradient shots/s", "cores": cores, "kind": "port",
+            "sample": f"{nshots} of the 120 gradient shots of the same workload "
+                      f"({dt:.1f} s), oracle/liboracle_f32.so (-O3 -ffast-math, OpenMP)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c1")
+    ap.add_argument("--nshot", type=int, default=0, help="override the shot count")
+    ap.add_argument("--batch", type=int, default=0, help="shots per launch (0 = all that fit)")
+    ap.add_argument("--plain-shots", type=int, default=4)
+    ap.add_argument("--cpu-shots", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ktime-every", type=int, default=11)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(shots would shard across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    L = flib.load()
+    if not L.fwiedu_device_ok():
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+    over = {"nshot": args.nshot} if args.nshot else {}
+    w = synth.workload(args.workload, **over)
+    log(f"workload {w.name}: {w.nz}x{w.nx} nt={w.nt} shots={w.nshot}")
+    t0 = time.perf_counter()
+    obs = observed(L, w)
+    log(f"observed data modelled on the GPU in {time.perf_counter() - t0:.2f} s")
+
+    times, st, misfit = timed_gradients(L, w, obs, args.steps, args.warmup, args.batch,
+                                        args.ktime_every)
+    step_s = float(np.mean(times))
+    shots_per_s = w.nshot / step_s
+    cu = w.cell_updates_per_gradient_shot()
+    S = int(st["fd_shots_per_launch"])
+    cells = interior_cells(w)
+    bytes_per_launch = cells * (12.0 * S + 28.0)   # 12 B/shot-cell + 7 shared coefficient streams
+    avg_ms = st["fd_ms_avg"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
+
+    # plain path: one shot per launch
+    plain = None
+    if args.plain_shots > 0:
+        pw = synth.workload(args.workload, nshot=args.plain_shots)
+        pobs = observed(L, pw)
+        ptimes, pst, _ = timed_gradients(L, pw, pobs, 1, 0, 1, args.ktime_every)
+        ps = pw.nshot / float(np.mean(ptimes))
+        plain = {"shots_per_launch": 1, "gradient_shots_per_s": ps,
+                 "gcell_updates_per_s": ps * cu / 1e9,
+                 "fd_avg_launch_us": pst["fd_ms_avg"] * 1e3,
+                 "sample": f"{args.plain_shots} shots, 1 step"}
+        log(f"plain: {ps:.2f} shots/s")
+
+    cpu = None if args.no_cpu else cpu_baseline(w, args.cpu_shots)
+    if cpu:
+        log(f"cpu baseline: {cpu['value']:.3f} shots/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": shots_per_s,
+        "unit": "gradient shots/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded layered model with faults; observed data modelled on the GPU)",
+        "config": {"workload": f"{w.name}: {w.nz}x{w.nx} @ {w.dx} m, nt {w.nt}, {w.nshot} shots x "
+                               f"{len(w.receiver_columns())} receivers, pml {w.pml}, free surface "
+                               f"{w.freesurface}, checkpointed adjoint gradient, precondition "
+                               f"{w.precondition}, model and records resident in HBM",
+                   "shots_per_launch": S, "batch": int(st["batch"])},
+        "gcell_updates_per_s": shots_per_s * cu / 1e9,
+        "cell_updates_per_gradient_shot": cu,
+        "misfit": misfit,
+        "plain": plain,
+        "batched_over_plain": (shots_per_s / plain["gradient_shots_per_s"]) if plain else None,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "kernel": "fd_kernel<FD_RECON> (reconstruction sweep)",
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_ms * 1e3,
+                     "launches_timed": int(st["fd_timed"])},
+        "cpu_baseline": cpu,
+        "device_bytes": st["device_bytes"],
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,495 @@
+// C-ABI entry points besides the propagation engine: optimisation helpers
+// (optimization.c), host checkpoint bookkeeping (checkpoint.c), single-sweep
+// helpers (propagation.c:13-101, 192-354) and the fwiedu_* controls.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "engine.hpp"
+
+using namespace fwiedu;
+
+namespace {
+
+// Upload the trace blocks [ntr][ns] of Nshot records back to back.
+float *upload_records(DevBuf &buf, Record2D *r, int Nshot, long &n)
+{
+    n = 0;
+    for (int s = 0; s < Nshot; ++s) n += (long)r[s].ntr * r[s].ns;
+    float *d = buf.as<float>(n + 64);
+    long off = 0;
+    for (int s = 0; s < Nshot; ++s) {
+        const long k = (long)r[s].ntr * r[s].ns;
+        h2d(d + off, r[s].traces_p[0], k * sizeof(float));
+        off += k;
+    }
+    return d;
+}
+
+double *scratch_res()
+{
+    Ctx &c = ctx();
+    return c.scratch.as<double>(reduce_scratch_doubles() + 64);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// optimization.c
+
+// compute_adjoint_source2d, optimization.c:300-419
+extern "C" void compute_adjoint_source2d(Record2D *record2d_pre, Record2D *record2d_obs,
+                                         float **vp, float freq, int filter,
+                                         Record2D *record2d_adjsource)
+{
+    ensure_ready();
+    Ctx &c = ctx();
+    Record2D *a = record2d_adjsource;
+    const long n = (long)a->ntr * a->ns;
+    float *pre = c.rec.as<float>(n + 64), *obs = c.obs.as<float>(n + 64);
+    h2d(pre, record2d_pre->traces_p[0], n * sizeof(float));
+    h2d(obs, record2d_obs->traces_p[0], n * sizeof(float));
+    adjoint_source_device(a, vp, freq, filter, pre, obs, pre);
+    d2h(a->traces_p[0], pre, n * sizeof(float));
+    sync();
+}
+
+// calculate_objective_value, optimization.c:99-131
+extern "C" float calculate_objective_value(Record2D *record2d, int Nshot)
+{
+    ensure_ready();
+    Ctx &c = ctx();
+    long n;
+    float *d = upload_records(c.rec, record2d, Nshot, n);
+    double *scr = scratch_res();
+    double *res = scr + reduce_scratch_doubles();
+    launch_sum_sq(d, n, scr, res, c.st);
+    double h[2];
+    d2h(h, res, 2 * sizeof(double));
+    sync();
+    return (float)(h[0] * 0.5);
+}
+
+// calculate_step_length, optimization.c:134-175
+extern "C" float calculate_step_length(Record2D *record2d_pre0, Record2D *record2d_pre1, int Nshot)
+{
+    ensure_ready();
+    Ctx &c = ctx();
+    long n0, n1;
+    float *a = upload_records(c.rec, record2d_pre0, Nshot, n0);
+    float *b = upload_records(c.obs, record2d_pre1, Nshot, n1);
+    if (n0 != n1) fail("<error>: calculate_step_length: record sizes differ");
+    double *scr = scratch_res();
+    double *res = scr + reduce_scratch_doubles();
+    launch_sum_step(a, b, n0, scr, res, c.st);
+    double h[2];
+    d2h(h, res, 2 * sizeof(double));
+    sync();
+    if (h[1] == 0.0) {
+        fprintf(stderr, "<warning>: denominator is zero in step length calculation, returning 0.\n");
+        return 0.0f;
+    }
+    return (float)(h[0] / h[1]);
+}
+
+// compute_model_perturbation, optimization.c:178-298: min/max, histogram and
+// the slowness mean on the device; the 1001-bin growth loop on the host.
+extern "C" float compute_model_perturbation(float fraction, float *slowness, float *gradient,
+                                            float *out, int len)
+{
+    ensure_ready();
+    Ctx &c = ctx();
+    float *g = c.misc[0].as<float>((size_t)len + 64);
+    float *sl = c.misc[1].as<float>((size_t)len + 64);
+    h2d(g, gradient, (size_t)len * sizeof(float));
+    h2d(sl, slowness, (size_t)len * sizeof(float));
+    float *fs = c.misc[2].as<float>(2 * 512 + 64);
+    launch_minmax(g, len, fs, fs + 2 * 512, c.st);
+    float mm[2];
+    d2h(mm, fs + 2 * 512, 2 * sizeof(float));
+    sync();
+    const float Min = mm[0], Max = mm[1];
+    const float interval = (float)((Max - Min) / 1000.0);
+    int *hist = c.misc[3].as<int>(1001 + 64);
+    HIPCHECK(hipMemsetAsync(hist, 0, 1001 * sizeof(int), c.st));
+    launch_histogram(g, len, Min, interval, hist, c.st);
+    int h[1001];
+    d2h(h, hist, sizeof(h));
+    double *scr = scratch_res();
+    double *res = scr + reduce_scratch_doubles();
+    launch_sum(sl, len, scr, res, c.st);
+    double avg[2];
+    d2h(avg, res, 2 * sizeof(double));
+    sync();
+    int peak = 0, pos = 0;
+    for (int i = 0; i < 1001; ++i)
+        if (h[i] > peak) {
+            peak = h[i];
+            pos = i;
+        }
+    int hi = pos, lo = pos;
+    const float cap = 0.99f;
+    float acc = (float)h[hi] / (float)len;
+    while (acc < cap) {
+        bool moved = false;
+        if (hi < 1000) {
+            hi++;
+            acc += (float)h[hi] / (float)len;
+            moved = true;
+        }
+        if (lo > 0) {
+            lo--;
+            acc += (float)h[lo] / (float)len;
+            moved = true;
+        }
+        if (!moved) break;
+    }
+    float a = Min + (float)lo * interval, b = Min + (float)hi * interval;
+    a = a < 0 ? -a : a;
+    b = b < 0 ? -b : b;
+    float scalar = (a > b) ? a : b;
+    scalar = (float)((avg[0] / len * (double)fraction) / (double)scalar);
+    launch_scale(g, sl, len, scalar, c.st);
+    d2h(out, sl, (size_t)len * sizeof(float));
+    sync();
+    return scalar;
+}
+
+// apply_gradient_shot_precondition, optimization.c:58-97
+extern "C" void apply_gradient_shot_precondition(float *grad, float *precond, int n)
+{
+    ensure_ready();
+    Ctx &c = ctx();
+    float *g = c.misc[0].as<float>((size_t)n + 64), *p = c.misc[1].as<float>((size_t)n + 64);
+    h2d(g, grad, (size_t)n * sizeof(float));
+    h2d(p, precond, (size_t)n * sizeof(float));
+    double *scr = c.scratch.as<double>(reduce_scratch_doubles() + 1024);
+    launch_shot_precondition(g, p, n, 1, n, scr, c.st);
+    d2h(grad, g, (size_t)n * sizeof(float));
+    d2h(precond, p, (size_t)n * sizeof(float));
+    sync();
+}
+
+// apply_gradient_precondition, optimization.c:13-56
+extern "C" void apply_gradient_precondition(float *grad, const float *precond, int n, float alpha)
+{
+    ensure_ready();
+    Ctx &c = ctx();
+    float *g = c.misc[0].as<float>((size_t)n + 64), *p = c.misc[1].as<float>((size_t)n + 64);
+    h2d(g, grad, (size_t)n * sizeof(float));
+    h2d(p, precond, (size_t)n * sizeof(float));
+    launch_global_precondition(g, p, n, alpha, scratch_res(), c.st);
+    d2h(grad, g, (size_t)n * sizeof(float));
+    sync();
+}
+
+// ---------------------------------------------------------------------------
+// propagation.c single-sweep helpers on host arrays
+
+// acoustic_iso2d_fd_it_6order, propagation.c:13-74.  The box the sweep reads
+// is gathered into a local padded layout, swept on the device, and the written
+// box is scattered back.
+extern "C" void acoustic_iso2d_fd_it_6order(float **p2, float **p1, float **p0, float **invrho,
+                                            float **invrhox, float **invrhoz, float **K,
+                                            float **coef1, float **coef2, float **coef3,
+                                            float invdx, float invdz, float invdxdx,
+                                            float invdzdz, int ixStart, int ixEnd, int izStart,
+                                            int izEnd)
+{
+    if (ixEnd < ixStart || izEnd < izStart) return;
+    ensure_ready();
+    Ctx &c = ctx();
+    const int nxi = ixEnd - ixStart + 1, nzi = izEnd - izStart + 1;
+    const int Nx = nxi + 6, Nz = nzi + 6, ld = (Nz + 63) / 64 * 64;
+    const long fsz = (long)Nx * ld;
+    std::vector<float> h((size_t)(CO_N + 2) * fsz, 0.0f);
+    float **co[CO_N];
+    co[CO_C1] = coef1;
+    co[CO_C2] = coef2;
+    co[CO_C3] = coef3;
+    co[CO_K] = K;
+    co[CO_IR] = invrho;
+    co[CO_IRX] = invrhox;
+    co[CO_IRZ] = invrhoz;
+    for (int lx = 0; lx < Nx; ++lx) {
+        const int ix = ixStart - 3 + lx;
+        const bool inner = lx >= 3 && lx < 3 + nxi;
+        for (int lz = 0; lz < Nz; ++lz) {
+            const int iz = izStart - 3 + lz;
+            const long i = (long)lx * ld + lz;
+            h[(size_t)(CO_N)*fsz + i] = p1[ix][iz];  // p1 with its halo
+            if (inner && lz >= 3 && lz < 3 + nzi) {
+                h[(size_t)(CO_N + 1) * fsz + i] = p0[ix][iz];
+                for (int k = 0; k < CO_N; ++k) h[(size_t)k * fsz + i] = co[k][ix][iz];
+            }
+        }
+    }
+    float *d = c.misc[0].as<float>((size_t)(CO_N + 3) * fsz);
+    h2d(d, h.data(), h.size() * sizeof(float));
+    HIPCHECK(hipMemsetAsync(d + (CO_N + 2) * fsz, 0, fsz * sizeof(float), c.st));
+    FdArgs a;
+    memset(&a, 0, sizeof(a));
+    a.coef = d;
+    a.cur = d + CO_N * fsz;
+    a.prev = d + (CO_N + 1) * fsz;
+    a.out = d + (CO_N + 2) * fsz;
+    a.nshot = 1;
+    a.Nx = Nx;
+    a.ld = ld;
+    a.ixs = 3;
+    a.ixe = 3 + nxi - 1;
+    a.izs = 3;
+    a.ize = 3 + nzi - 1;
+    a.invdx = invdx;
+    a.invdz = invdz;
+    a.invdxdx = invdxdx;
+    a.invdzdz = invdzdz;
+    launch_fd(FD_ADJ, a, c.st);
+    std::vector<float> o((size_t)fsz);
+    d2h(o.data(), d + (CO_N + 2) * fsz, fsz * sizeof(float));
+    sync();
+    for (int lx = 3; lx < 3 + nxi; ++lx)
+        for (int lz = 3; lz < 3 + nzi; ++lz)
+            p2[ixStart - 3 + lx][izStart - 3 + lz] = o[(size_t)lx * ld + lz];
+}
+
+// acoustic_iso2d_freesurface, propagation.c:76-101
+extern "C" void acoustic_iso2d_freesurface(float **p2, int Nx, int Nz, int pmlThick)
+{
+    ensure_ready();
+    Ctx &c = ctx();
+    const int rows = 2 * pmlThick + 1;
+    if (rows > Nz) fail("<error>: acoustic_iso2d_freesurface: 2*pmlThick+1 > Nz");
+    const int ld = (rows + 63) / 64 * 64;
+    std::vector<float> h((size_t)Nx * ld, 0.0f);
+    for (int ix = 0; ix < Nx; ++ix) memcpy(&h[(size_t)ix * ld], p2[ix], rows * sizeof(float));
+    float *d = c.misc[0].as<float>(h.size());
+    h2d(d, h.data(), h.size() * sizeof(float));
+    launch_freesurface(d, Nx, ld, pmlThick, c.st);
+    d2h(h.data(), d, h.size() * sizeof(float));
+    sync();
+    for (int ix = 0; ix < Nx; ++ix) memcpy(p2[ix], &h[(size_t)ix * ld], rows * sizeof(float));
+}
+
+// mealCoefs2d, propagation.c:192-354 (note the dz, dx argument order)
+extern "C" void mealCoefs2d(float **D, float **Beta, float **vp, float **vs, float fpeak,
+                           float eal_alpha, float dz, float dx, int NZ, int NX, int nlayer)
+{
+    ensure_ready();
+    Ctx &c = ctx();
+    if (eal_alpha == -1.0f) eal_alpha = 2.5f;
+    Geom g;
+    memset(&g, 0, sizeof(g));
+    g.Nx = NX;
+    g.Nz = NZ;
+    g.ld = (NZ + 63) / 64 * 64;
+    g.dx = dx;
+    g.dz = dz;
+    g.fsz = (long)NX * g.ld;
+    const int nvs = vs ? 1 : 0;
+    std::vector<float> h((size_t)(1 + nvs) * g.fsz, 0.0f);
+    for (int ix = 0; ix < NX; ++ix) {
+        memcpy(&h[(size_t)ix * g.ld], vp[ix], NZ * sizeof(float));
+        if (vs) memcpy(&h[(size_t)(g.fsz + ix * g.ld)], vs[ix], NZ * sizeof(float));
+    }
+    std::vector<float> w = meal_weights(eal_alpha, nlayer);
+    float *d = c.misc[0].as<float>((size_t)(3 + nvs) * g.fsz + w.size() + 64);
+    float *dv = d, *dvs = vs ? d + g.fsz : nullptr;
+    float *dD = d + (1 + nvs) * g.fsz, *dB = dD + g.fsz, *dw = dB + g.fsz;
+    h2d(d, h.data(), h.size() * sizeof(float));
+    h2d(dw, w.data(), w.size() * sizeof(float));
+    launch_meal(dv, dvs, dD, dB, g, nlayer, dw, meal_lnR(nlayer, 2.0f), 2.0f, sqrtf(3.0f), fpeak,
+                c.st);
+    std::vector<float> o((size_t)2 * g.fsz);
+    d2h(o.data(), dD, o.size() * sizeof(float));
+    sync();
+    for (int ix = 0; ix < NX; ++ix) {
+        memcpy(D[ix], &o[(size_t)ix * g.ld], NZ * sizeof(float));
+        memcpy(Beta[ix], &o[(size_t)(g.fsz + ix * g.ld)], NZ * sizeof(float));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// checkpoint.c host bookkeeping (same schedule and slot map)
+
+extern "C" void acoustic2dCheckPoint_init(Acoustic2dCheckPoint *chkpt)
+{
+    chkpt->status = nullptr;
+    chkpt->p = nullptr;
+    chkpt->storage_p = nullptr;
+}
+
+static void free_storage(float ***s)
+{
+    if (!s) return;
+    free(s[0][0]);
+    free(s[0]);
+    free(s);
+}
+
+extern "C" void alloc_acoustic2dCheckPoint(Acoustic2dCheckPoint *chkpt, int NZ, int NX, int nt)
+{
+    chkpt->nt = nt;
+    chkpt->dCheckPoint = std::max((int)roundf(sqrtf(2.0f * (nt - 1))), 3);
+    const int d = chkpt->dCheckPoint;
+    free(chkpt->status);
+    chkpt->status = (int *)calloc(nt, sizeof(int));
+    free(chkpt->p);
+    chkpt->p = (float ***)calloc(nt, sizeof(float **));
+    int nc = 0;
+    for (int it = 0; it < nt; ++it)
+        if (it % d == 0 || (it + 1) % d == 0) nc++;
+    chkpt->nCheckPoint = nc;
+    chkpt->ntPoint = nc + d - 2;
+    chkpt->memCost = 1.0f * (float)(chkpt->ntPoint * NZ * NX * (int)sizeof(float)) / powf(1024.0f, 3);
+    free_storage(chkpt->storage_p);
+    const int np = chkpt->ntPoint;
+    float ***s = (float ***)malloc((size_t)np * sizeof(float **));
+    float **rows = (float **)malloc((size_t)np * NX * sizeof(float *));
+    float *blk = (float *)calloc((size_t)np * NX * NZ, sizeof(float));
+    if (!s || !rows || !blk || !chkpt->status || !chkpt->p)
+        fail("<error>: alloc_acoustic2dCheckPoint: out of host memory");
+    for (int k = 0; k < np; ++k) {
+        s[k] = rows + (size_t)k * NX;
+        for (int ix = 0; ix < NX; ++ix) s[k][ix] = blk + ((size_t)k * NX + ix) * NZ;
+    }
+    chkpt->storage_p = s;
+    int ichk = 0, icur = nc;
+    for (int it = 0; it < nt; ++it) {
+        if (it % d == 0 || (it + 1) % d == 0) {
+            chkpt->p[it] = s[ichk++];
+            icur = nc;
+        } else {
+            chkpt->p[it] = s[icur++];
+        }
+    }
+}
+
+extern "C" void free_acoustic2dCheckPoint(Acoustic2dCheckPoint *chkpt)
+{
+    free(chkpt->status);
+    chkpt->status = nullptr;
+    free(chkpt->p);
+    chkpt->p = nullptr;
+    free_storage(chkpt->storage_p);
+    chkpt->storage_p = nullptr;
+}
+
+extern "C" void save_acoustic2dCheckPoint(Acoustic2dCheckPoint *chkpt, float **p0, float **p1,
+                                          int NP, int it)
+{
+    if (it < 0 || it >= chkpt->nt || it % chkpt->dCheckPoint) return;
+    if (it - 1 >= 0) {
+        memcpy(chkpt->p[it - 1][0], p0[0], (size_t)NP * sizeof(float));
+        chkpt->status[it - 1] = 1;
+    }
+    memcpy(chkpt->p[it][0], p1[0], (size_t)NP * sizeof(float));
+    chkpt->status[it] = 1;
+}
+
+extern "C" void get_acoustic2dCheckPoint(Acoustic2dCheckPoint *chkpt, float **p0, float **p1, int NP,
+                                         int it)
+{
+    if (it < 0 || it >= chkpt->nt || it % chkpt->dCheckPoint) return;
+    if (it - 1 >= 0)
+        memcpy(p0[0], chkpt->p[it - 1][0], (size_t)NP * sizeof(float));
+    else
+        memset(p0[0], 0, (size_t)NP * sizeof(float));
+    memcpy(p1[0], chkpt->p[it][0], (size_t)NP * sizeof(float));
+}
+
+extern "C" void save_acoustic2dWavefield(Acoustic2dCheckPoint *chkpt, float **p, int NP, int it)
+{
+    if (it < 0 || it >= chkpt->nt) return;
+    memcpy(chkpt->p[it][0], p[0], (size_t)NP * sizeof(float));
+    chkpt->status[it] = 1;
+}
+
+extern "C" void get_acoustic2dWavefield(Acoustic2dCheckPoint *chkpt, float **p, int NP, int it)
+{
+    if (it < 0 || it >= chkpt->nt) return;
+    memcpy(p[0], chkpt->p[it][0], (size_t)NP * sizeof(float));
+}
+
+extern "C" int get_acoustic2dCheckPointStatus(Acoustic2dCheckPoint *chkpt, int it)
+{
+    if (chkpt->status[it] == 1) return -1;
+    int s = it;
+    while (s >= 0 && chkpt->status[s] == 0) s--;
+    return s;
+}
+
+extern "C" float calculate_memory_cost_checkpoint(int nt, int NX, int NZ)
+{
+    const int d = std::max((int)roundf(sqrtf(2.0f * (nt - 1))), 3);
+    int nc = 0;
+    for (int it = 0; it < nt; ++it)
+        if (it % d == 0 || (it + 1) % d == 0) nc++;
+    const int np = nc + d - 2;
+    return 1.0f * (float)(np * NZ * NX * (int)sizeof(float)) / powf(1024.0f, 3);
+}
+
+// ---------------------------------------------------------------------------
+// fwiedu_* controls
+
+extern "C" void fwiedu_set_batch(int s) { ctx().batch_req = s < 0 ? 0 : s; }
+extern "C" void fwiedu_set_memory_fraction(double f)
+{
+    ctx().memfrac = (f > 0.0 && f <= 0.98) ? f : 0.85;
+}
+extern "C" void fwiedu_set_kernel_timing(int every) { ctx().ktiming = every < 0 ? 0 : every; }
+extern "C" void fwiedu_set_checkpoint_bits(int bits)
+{
+    if (bits != 32 && bits != 16) fail("<error>: checkpoint bits must be 32 or 16 (got %d)", bits);
+    ctx().ckpt_bits = bits;
+}
+
+extern "C" double fwiedu_stat(const char *name)
+{
+    const Stats &s = ctx().stats;
+    const std::string n(name ? name : "");
+    if (n == "fd_launches") return (double)s.fd_launches;
+    if (n == "fd_cells") return (double)s.fd_cells;
+    if (n == "fd_timed") return (double)s.fd_timed;
+    if (n == "fd_ms_avg") return s.fd_timed ? s.fd_ms_timed / s.fd_timed : 0.0;
+    if (n == "fd_ms_timed") return s.fd_ms_timed;
+    if (n == "fd_cells_timed") return s.fd_cells_timed;
+    if (n == "fd_cells_per_launch") return s.fd_timed ? s.fd_cells_timed / s.fd_timed : 0.0;
+    if (n == "fd_shots_per_launch") return (double)s.shots_per_launch;
+    if (n == "batch") return (double)s.batch;
+    if (n == "h2d_bytes") return s.h2d_bytes;
+    if (n == "d2h_bytes") return s.d2h_bytes;
+    if (n == "device_bytes") {
+        const Ctx &c = ctx();
+        double b = 0;
+        const DevBuf *all[] = {&c.model[0], &c.model[1], &c.setup, &c.coef,  &c.arena,
+                               &c.acc,      &c.rec,      &c.obs,   &c.src,   &c.tables,
+                               &c.taper,    &c.shot_out, &c.gsum,  &c.scratch, &c.misc[0],
+                               &c.misc[1],  &c.misc[2],  &c.misc[3]};
+        for (const DevBuf *d : all) b += (double)d->cap;
+        return b;
+    }
+    return -1.0;
+}
+
+extern "C" void fwiedu_reset_stats(void) { ctx().stats = Stats(); }
+
+extern "C" void fwiedu_release(void)
+{
+    Ctx &c = ctx();
+    if (!c.ready) return;
+    sync();
+    DevBuf *all[] = {&c.model[0], &c.model[1], &c.setup, &c.coef,    &c.arena,   &c.acc,
+                     &c.rec,      &c.obs,      &c.src,   &c.tables,  &c.taper,   &c.shot_out,
+                     &c.gsum,     &c.scratch,  &c.misc[0], &c.misc[1], &c.misc[2], &c.misc[3]};
+    for (DevBuf *d : all) d->release();
+}
+
+extern "C" int fwiedu_device_ok(void)
+{
+    int n = 0;
+    return (hipGetDeviceCount(&n) == hipSuccess && n > 0) ? 1 : 0;
+}
+
+extern "C" const char *fwiedu_version(void) { return "fwiedu-mi355x 0.1 (gfx950)"; }

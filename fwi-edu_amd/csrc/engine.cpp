@@ -1,0 +1,1072 @@
+// Host engine of the FWI hot path: device buffers, the shot batcher, the
+// zero-copy checkpoint schedule and the reference's propagation entry points.
+//
+// Reference: FWILab2d/FWILab2dCLangV1.2.6/src/propagation.c (engine and shot
+// loop), checkpoint.c (schedule), optimization.c (adjoint source, per-shot and
+// global preconditioning).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <tuple>
+
+#include "engine.hpp"
+
+namespace fwiedu {
+
+// ---------------------------------------------------------------------------
+// runtime
+
+void fail(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vfprintf(stderr, fmt, ap);
+    va_end(ap);
+    fprintf(stderr, "\n");
+    fflush(stderr);
+    exit(1);
+}
+
+void *DevBuf::get(size_t bytes)
+{
+    if (bytes == 0) bytes = 256;
+    if (bytes > cap) {
+        if (p) HIPCHECK(hipFree(p));
+        p = nullptr;
+        cap = 0;
+        HIPCHECK(hipMalloc(&p, bytes));
+        cap = bytes;
+    }
+    return p;
+}
+
+void DevBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+static Ctx g_ctx;
+Ctx &ctx() { return g_ctx; }
+
+void ensure_ready()
+{
+    Ctx &c = g_ctx;
+    if (c.ready) return;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        fail("<error>: no HIP device available (libfwiedu runs the FWI path on the GPU only)");
+    HIPCHECK(hipGetDevice(&c.device));
+    HIPCHECK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
+    c.ready = true;
+}
+
+void sync() { HIPCHECK(hipStreamSynchronize(g_ctx.st)); }
+
+// Host-to-device copies complete before returning: sources are often
+// temporaries, and pageable memory gives no asynchrony to keep.
+void h2d(void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return;
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_ctx.st));
+    HIPCHECK(hipStreamSynchronize(g_ctx.st));
+    g_ctx.stats.h2d_bytes += (double)bytes;
+}
+
+void d2h(void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return;
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_ctx.st));
+    g_ctx.stats.d2h_bytes += (double)bytes;
+}
+
+Geom make_geom(int nx, int nz, int pml, int fs, float dx, float dz, float dt)
+{
+    Geom g;
+    g.nx = nx;
+    g.nz = nz;
+    g.pml = pml;
+    g.fs = fs;
+    g.Nx = nx + 2 * pml;
+    g.Nz = nz + 2 * pml;
+    g.ld = (g.Nz + 63) / 64 * 64;
+    const int N = 3;
+    g.ixs = N;
+    g.ixe = g.Nx - N - 1;
+    g.izs = (fs == 0) ? N : pml;
+    g.ize = g.Nz - N - 1;
+    g.dx = dx;
+    g.dz = dz;
+    g.dt = dt;
+    g.invdx = 1.0f / dx;
+    g.invdz = 1.0f / dz;
+    g.invdxdx = 1.0f / (dx * dx);
+    g.invdzdz = 1.0f / (dz * dz);
+    g.invdtdt = 1.0f / (dt * dt);
+    g.fsz = (long)g.Nx * g.ld;
+    return g;
+}
+
+// absorbWeight (propagation.c:173-190) for l = 1..nlayer, host libm.
+std::vector<float> meal_weights(float alpha, int nl)
+{
+    std::vector<float> w(nl + 1, 0.0f);
+    for (int l = 1; l <= nl; ++l) {
+        const float d = (0.1f * (float)l) / (0.1f * (float)nl);
+        w[l] = expf(logf(2.0f) * powf(d, alpha)) - 1.0f;
+    }
+    return w;
+}
+
+// ln of the reflection coefficient of propagation.c:225.
+float meal_lnR(int nl, float pd)
+{
+    const float Rcoef =
+        powf(10.0f, -1.0f * ((log10f((float)nl) - 1.0f) / (log10f(2.0f))) - 3.0f - 0.5f * pd);
+    return logf(Rcoef);
+}
+
+// The MEAL scalars are evaluated here with the host libm; the per-cell
+// arithmetic runs on the device.
+void build_coefs(const Geom &g, const float *vpw, const float *rhow, float fpeak, float *coef)
+{
+    Ctx &c = g_ctx;
+    const int nl = g.pml;
+    float *base = c.setup.as<float>(4 * g.fsz + nl + 1 + 64);
+    float *vp = base, *rho = base + g.fsz, *D = base + 2 * g.fsz, *Beta = base + 3 * g.fsz;
+    float *wdev = base + 4 * g.fsz;
+    std::vector<float> w = meal_weights(2.5f, nl);
+    const float pd = 2.0f;
+    const float lnR = meal_lnR(nl, pd);
+    const float sqrt3 = sqrtf(3.0f);
+    h2d(wdev, w.data(), w.size() * sizeof(float));
+    launch_pad(vpw, g.nx, g.nz, g.pml, vp, g, c.st);
+    launch_pad(rhow, g.nx, g.nz, g.pml, rho, g, c.st);
+    launch_meal(vp, nullptr, D, Beta, g, nl, wdev, lnR, pd, sqrt3, fpeak, c.st);
+    launch_coefs(vp, rho, D, Beta, coef, g, c.st);
+    launch_grad_invrho(coef, g, c.st);
+}
+
+// ---------------------------------------------------------------------------
+// sampled stencil timing
+
+void fd_launch(int mode, const FdArgs &a, long cells)
+{
+    Ctx &c = g_ctx;
+    // sampled: the reconstruction sweep, the stencil with no accumulation epilogue
+    const bool timed = c.ktiming > 0 && mode == FD_RECON && (c.recon_launches++ % c.ktiming) == 0;
+    if (timed) {
+        if (c.ev_used + 2 > 4096) collect_timing();
+        while (c.ev.size() < c.ev_used + 2) {
+            hipEvent_t e;
+            HIPCHECK(hipEventCreate(&e));
+            c.ev.push_back(e);
+        }
+        HIPCHECK(hipEventRecord(c.ev[c.ev_used], c.st));
+    }
+    launch_fd(mode, a, c.st);
+    if (timed) {
+        HIPCHECK(hipEventRecord(c.ev[c.ev_used + 1], c.st));
+        c.ev_cells.push_back({(long)c.ev_used, cells});
+        c.ev_used += 2;
+    }
+    c.stats.fd_launches++;
+    c.stats.fd_cells += cells;
+    c.stats.shots_per_launch = a.nshot;
+}
+
+void collect_timing()
+{
+    Ctx &c = g_ctx;
+    if (c.ev_cells.empty()) return;
+    sync();
+    for (auto &pc : c.ev_cells) {
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, c.ev[pc.first], c.ev[pc.first + 1]));
+        c.stats.fd_ms_timed += ms;
+        c.stats.fd_cells_timed += (double)pc.second;
+        c.stats.fd_timed++;
+    }
+    c.ev_cells.clear();
+    c.ev_used = 0;
+}
+
+// ---------------------------------------------------------------------------
+// checkpoint schedule (checkpoint.c:29-265), zero-copy on the device.
+//
+// The reference copies (u(kd-1), u(kd)) into storage at every multiple kd of
+// d during the forward pass, and in the reverse pass recomputes a segment from
+// the pair, storing every reconstructed field.  Here the stencil writes those
+// fields straight into their storage slot (no copies), and a recomputed field is
+// kept only when the imaging condition will read it (it % MST == 0).  The
+// fields computed, and the order of the arithmetic, are the reference's.
+struct Schedule {
+    int nt, d;
+    std::vector<int> ckslot;  // it -> forward checkpoint slot, -1 none, -2 the zero field
+    int nck = 0;              // checkpoint slots
+    int nring = 0;            // ring slots (it % MST == 0 inside one segment)
+
+    explicit Schedule(int nt_) : nt(nt_)
+    {
+        d = std::max((int)roundf(sqrtf(2.0f * (float)(nt - 1))), 3);
+        ckslot.assign(nt, -1);
+        if (nt > 0) ckslot[0] = -2;
+        for (int kd = d; kd <= nt - 1; kd += d) {
+            ckslot[kd - 1] = nck++;
+            ckslot[kd] = nck++;
+        }
+        nring = (d + MST - 1) / MST + 1;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// A batch of shots sharing one simulation window (same ix_min/ix_max) and
+// fpeak, i.e. the same coefficient arrays.
+
+struct ShotIn {
+    Record2D *pre;   // predicted / adjoint-source record (output)
+    Record2D *obs;   // observed record (problems 1, 2)
+    int index;       // shot index in the call
+};
+
+struct BatchPlan {
+    Geom g;
+    int S = 0;
+    int nt = 0;
+    bool adjoint = false;       // problem 1
+    int nbuf = 0;               // fields per shot arena
+    long shot_stride = 0;       // floats
+    // arena field indices: adjoint/forward work, the zero field, reconstruction
+    // work, then checkpoint slots and the imaging ring (gradient batches only)
+    enum { A0 = 0, A1, A2, ZERO, F0, F1, F2, CK0 };
+    int ring0 = 0;
+};
+
+static long fields_per_shot(const Schedule &sc, bool adjoint)
+{
+    return adjoint ? (BatchPlan::CK0 + sc.nck + sc.nring) : 4;
+}
+
+struct Tables {
+    RcvTable rcv;
+    InjTable inj;
+};
+
+// Receiver tables of the batch.  trace offsets index the batch record buffer
+// (shot-major, [ntr][ns] per shot).
+static Tables build_tables(const std::vector<ShotIn> &shots, const Geom &g, std::vector<long> &roff,
+                           long &rec_total)
+{
+    Ctx &c = g_ctx;
+    const int S = (int)shots.size();
+    roff.assign(S, 0);
+    rec_total = 0;
+    int nent = 0;
+    for (int s = 0; s < S; ++s) {
+        roff[s] = rec_total;
+        rec_total += (long)shots[s].pre->ntr * shots[s].pre->ns;
+        nent += shots[s].pre->ntr;
+    }
+    std::vector<int> e_shot(nent), e_cell(nent), e_sr(nent);
+    std::vector<long> e_off(nent);
+    // unique (shot, cell) for the injection, receiver order kept within a cell
+    std::vector<int> u_shot, u_cell, u_beg;
+    std::vector<long> u_off;
+    int k = 0;
+    for (int s = 0; s < S; ++s) {
+        const Record2D *r = shots[s].pre;
+        std::map<int, std::vector<int>> bycell;
+        std::vector<int> order;
+        for (int itr = 0; itr < r->ntr; ++itr, ++k) {
+            const int gx = r->igx[itr] + g.pml, gz = r->igz[itr] + g.pml;
+            if (gx < 0 || gx >= g.Nx || gz < 0 || gz >= g.Nz)
+                fail("<error>: receiver %d of shot %d lies outside the padded grid", itr, r->shotID);
+            const int cell = gx * g.ld + gz;
+            e_shot[k] = s;
+            e_cell[k] = cell;
+            e_sr[k] = r->sampleRate;
+            e_off[k] = roff[s] + (long)itr * r->ns;
+            auto it = bycell.find(cell);
+            if (it == bycell.end()) {
+                order.push_back(cell);
+                bycell[cell] = {itr};
+            } else {
+                it->second.push_back(itr);
+            }
+        }
+        for (int cell : order) {
+            u_shot.push_back(s);
+            u_cell.push_back(cell);
+            u_beg.push_back((int)u_off.size());
+            for (int itr : bycell[cell]) u_off.push_back(roff[s] + (long)itr * r->ns);
+        }
+    }
+    u_beg.push_back((int)u_off.size());
+    const int nu = (int)u_cell.size();
+    // one device block: ints then longs
+    const size_t ni = (size_t)3 * nent + 2 * nu + nu + 1;
+    const size_t nl = (size_t)nent + u_off.size();
+    char *base = c.tables.as<char>(ni * sizeof(int) + nl * sizeof(long) + 64);
+    long *lb = reinterpret_cast<long *>(base);
+    int *ib = reinterpret_cast<int *>(base + nl * sizeof(long));
+    std::vector<long> lh(nl);
+    std::copy(e_off.begin(), e_off.end(), lh.begin());
+    std::copy(u_off.begin(), u_off.end(), lh.begin() + nent);
+    std::vector<int> ih;
+    ih.reserve(ni);
+    ih.insert(ih.end(), e_shot.begin(), e_shot.end());
+    ih.insert(ih.end(), e_cell.begin(), e_cell.end());
+    ih.insert(ih.end(), e_sr.begin(), e_sr.end());
+    ih.insert(ih.end(), u_shot.begin(), u_shot.end());
+    ih.insert(ih.end(), u_cell.begin(), u_cell.end());
+    ih.insert(ih.end(), u_beg.begin(), u_beg.end());
+    h2d(lb, lh.data(), nl * sizeof(long));
+    h2d(ib, ih.data(), ih.size() * sizeof(int));
+    Tables t;
+    t.rcv.n = nent;
+    t.rcv.trace_off = lb;
+    t.rcv.shot = ib;
+    t.rcv.cell = ib + nent;
+    t.rcv.sr = ib + 2 * nent;
+    t.inj.n = nu;
+    t.inj.trace_off = lb + nent;
+    t.inj.shot = ib + 3 * nent;
+    t.inj.cell = ib + 3 * nent + nu;
+    t.inj.beg = ib + 3 * nent + 2 * nu;
+    return t;
+}
+
+static long interior_cells(const Geom &g) { return (long)(g.ixe - g.ixs + 1) * (g.ize - g.izs + 1); }
+
+static FdArgs base_args(const Geom &g, const float *coef, int S, long stride)
+{
+    FdArgs a;
+    memset(&a, 0, sizeof(a));
+    a.coef = coef;
+    a.shot_stride = stride;
+    a.nshot = S;
+    a.Nx = g.Nx;
+    a.ld = g.ld;
+    a.ixs = g.ixs;
+    a.ixe = g.ixe;
+    a.izs = g.izs;
+    a.ize = g.ize;
+    a.pml = g.pml;
+    a.fs = g.fs;
+    a.nx = g.nx;
+    a.nz = g.nz;
+    a.invdx = g.invdx;
+    a.invdz = g.invdz;
+    a.invdxdx = g.invdxdx;
+    a.invdzdz = g.invdzdz;
+    a.invdtdt = g.invdtdt;
+    a.dt = g.dt;
+    return a;
+}
+
+static inline float *pick_free(float *const w[3], const float *x, const float *y)
+{
+    for (int i = 0; i < 3; ++i)
+        if (w[i] != x && w[i] != y) return w[i];
+    fail("<error>: internal buffer rotation");
+}
+
+// Taper length and cosine table of taper_traces (common.c:796-895).  L < 0 on
+// return means no taper; a requested length clamped to 0 keeps the reference's
+// 0/0 table entry.
+static std::vector<float> taper_table(int &L, int n)
+{
+    std::vector<float> t;
+    if (L <= 0) {
+        L = -1;
+        return t;
+    }
+    if (L > n / 3) L = n / 3;
+    t.resize(L + 1);
+    const float PIf = (float)3.141592653589793;
+    for (int i = 0; i <= L; ++i) t[i] = 0.5f * (1.0f + cosf(PIf * (float)i / (float)L));
+    return t;
+}
+
+// compute_adjoint_source2d (optimization.c:300-419) for filter == 2, scalars
+// on the host, residual and tapers on the device.  rec/obs/out are [ntr][ns].
+void adjoint_source_device(const Record2D *r, float **vp_full, float freq, int filter,
+                           const float *pre_dev, const float *obs_dev, float *out_dev)
+{
+    Ctx &c = g_ctx;
+    if (filter == 0 || filter == 1)
+        fail("<error>: compute_adjoint_source2d: filter %d (FFT low/band-pass) is not available "
+             "in this build; use filter=2",
+             filter);
+    const int ntr = r->ntr, ns = r->ns;
+    double vsum = 0.0;
+    for (int itr = 0; itr < ntr; ++itr) vsum += vp_full[r->igx[itr]][r->igz[itr]];
+    const float vavg = (float)(vsum / ntr);
+    if (freq == 0.0f) freq = r->fpeak;
+    const float kpass = freq / vavg;
+    int Lt = (int)roundf((2.0f / freq) / r->dt) + 1;
+    int Lx = (int)roundf((2.0f / kpass) / r->dtr) + 1;
+    std::vector<float> cT = taper_table(Lt, ns), cX = taper_table(Lx, ntr);
+    float *tb = c.taper.as<float>(cT.size() + cX.size() + 64);
+    if (cT.empty() && cX.empty()) {
+        launch_adjsrc(pre_dev, obs_dev, out_dev, ntr, ns, tb, -1, tb, -1, c.st);
+        sync();
+        return;
+    }
+    h2d(tb, cT.data(), cT.size() * sizeof(float));
+    h2d(tb + cT.size(), cX.data(), cX.size() * sizeof(float));
+    launch_adjsrc(pre_dev, obs_dev, out_dev, ntr, ns, tb, Lt, tb + cT.size(), Lx, c.st);
+    // the taper table must stay valid until the kernel ran
+    sync();
+}
+
+// Copy a host record [ntr][ns] (contiguous block traces_p[0]) to / from the device.
+static void rec_h2d(float *dst, const Record2D *r)
+{
+    h2d(dst, r->traces_p[0], (size_t)r->ntr * r->ns * sizeof(float));
+}
+static void rec_d2h(Record2D *r, const float *src)
+{
+    d2h(r->traces_p[0], src, (size_t)r->ntr * r->ns * sizeof(float));
+}
+
+// ---------------------------------------------------------------------------
+// One batch of S shots on the device.
+
+struct BatchState {
+    Geom g;
+    const float *coef = nullptr;
+    int S = 0, nt = 0;
+    bool adjoint = false;
+    Schedule *sc = nullptr;
+    long stride = 0;           // floats per shot arena
+    float *arena = nullptr;
+    int *cell_dev = nullptr;
+    float *wav_dev = nullptr;
+    Tables tb;
+    std::vector<long> roff;
+    long rec_total = 0;
+    float *rec = nullptr;      // predicted records, then the adjoint source
+    double *acc = nullptr;     // [S][nx*nz]
+    long nacc = 0;
+    long cells_per_launch = 0;
+    float *F(int k) const { return arena + (long)k * g.fsz; }
+};
+
+static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
+                          const std::vector<Record2D *> &recs, int nt, bool adjoint, Schedule &sc)
+{
+    Ctx &c = g_ctx;
+    b.g = g;
+    b.coef = coef;
+    b.S = (int)recs.size();
+    b.nt = nt;
+    b.adjoint = adjoint;
+    b.sc = &sc;
+    const int S = b.S;
+    b.stride = fields_per_shot(sc, adjoint) * g.fsz;
+    b.arena = c.arena.as<float>((size_t)S * b.stride);
+    HIPCHECK(hipMemsetAsync(b.arena, 0, (size_t)S * b.stride * sizeof(float), c.st));
+
+    std::vector<int> cells(S);
+    std::vector<float> wav((size_t)S * nt);
+    for (int s = 0; s < S; ++s) {
+        const Record2D *r = recs[s];
+        const int isx = r->isx + g.pml, isz = r->isz + g.pml;
+        if (isx < g.ixs || isx > g.ixe || isz < g.izs || isz > g.ize)
+            fail("<error>: source of shot %d at padded (%d,%d) is outside the stencil region "
+                 "[%d,%d]x[%d,%d]",
+                 r->shotID, isx, isz, g.ixs, g.ixe, g.izs, g.ize);
+        if (r->sampleRate < 1) fail("<error>: shot %d has sampleRate %d", r->shotID, r->sampleRate);
+        if (r->ns < (nt - 1) / r->sampleRate + 1 || r->ntr < 1 || !r->src || !r->traces_p)
+            fail("<error>: shot %d: record needs ntr >= 1 and ns >= (nt-1)/sampleRate+1 = %d "
+                 "samples (has ntr=%d, ns=%d)",
+                 r->shotID, (nt - 1) / r->sampleRate + 1, r->ntr, r->ns);
+        cells[s] = isx * g.ld + isz;
+        memcpy(&wav[(size_t)s * nt], r->src, nt * sizeof(float));
+    }
+    char *sb = c.src.as<char>(wav.size() * sizeof(float) + S * sizeof(int) + 256);
+    b.wav_dev = reinterpret_cast<float *>(sb);
+    b.cell_dev = reinterpret_cast<int *>(sb + wav.size() * sizeof(float));
+    h2d(b.wav_dev, wav.data(), wav.size() * sizeof(float));
+    h2d(b.cell_dev, cells.data(), S * sizeof(int));
+
+    std::vector<ShotIn> si(S);
+    for (int s = 0; s < S; ++s) si[s] = {recs[s], nullptr, s};
+    b.tb = build_tables(si, g, b.roff, b.rec_total);
+    b.rec = c.rec.as<float>(b.rec_total + 64);
+    HIPCHECK(hipMemsetAsync(b.rec, 0, b.rec_total * sizeof(float), c.st));
+    b.nacc = (long)g.nx * g.nz;
+    if (adjoint) {
+        b.acc = c.acc.as<double>((size_t)S * b.nacc);
+        HIPCHECK(hipMemsetAsync(b.acc, 0, (size_t)S * b.nacc * sizeof(double), c.st));
+    }
+    b.cells_per_launch = interior_cells(g) * S;
+}
+
+// propagation.c:539-608.  With save_ckpt the fields u(kd-1), u(kd) land in
+// their checkpoint slots; with precond the pseudo-Hessian accumulates in acc.
+static void forward_pass(BatchState &b, bool save_ckpt, bool precond)
+{
+    Ctx &c = g_ctx;
+    const Schedule &sc = *b.sc;
+    FdArgs a = base_args(b.g, b.coef, b.S, b.stride);
+    a.src_cell = b.cell_dev;
+    a.src_wav = b.wav_dev;
+    a.nt = b.nt;
+    a.acc = b.acc;
+    a.acc_stride = b.nacc;
+    float *W[3] = {b.F(BatchPlan::A0), b.F(BatchPlan::A1), b.F(BatchPlan::A2)};
+    const float *zero = b.F(BatchPlan::ZERO);
+    const float *cur = zero, *prev = zero;
+    for (int it = 0; it < b.nt; ++it) {
+        float *out;
+        if (save_ckpt && it + 1 < b.nt && sc.ckslot[it + 1] >= 0)
+            out = b.F(BatchPlan::CK0 + sc.ckslot[it + 1]);
+        else
+            out = pick_free(W, cur, prev);
+        launch_record(b.tb.rcv, cur, b.stride, b.rec, it, c.st);
+        a.out = out;
+        a.cur = cur;
+        a.prev = prev;
+        a.it = it;
+        a.acc_on = (precond && it % MST == 0) ? 1 : 0;
+        fd_launch(FD_FWD, a, b.cells_per_launch);
+        prev = cur;
+        cur = out;
+    }
+}
+
+// propagation.c:633-744.  b.rec holds the adjoint source [ntr][nt] per shot.
+static void adjoint_pass(BatchState &b)
+{
+    Ctx &c = g_ctx;
+    const Schedule &sc = *b.sc;
+    const Geom &g = b.g;
+    const int nt = b.nt;
+    std::vector<char> stored(nt, 0);
+    std::vector<const float *> where(nt, nullptr);
+    for (int it = 0; it < nt; ++it) {
+        if (sc.ckslot[it] == -2) {
+            stored[it] = 1;
+            where[it] = b.F(BatchPlan::ZERO);
+        } else if (sc.ckslot[it] >= 0) {
+            stored[it] = 1;
+            where[it] = b.F(BatchPlan::CK0 + sc.ckslot[it]);
+        }
+    }
+    const int ring0 = BatchPlan::CK0 + sc.nck;
+    float *FW[3] = {b.F(BatchPlan::F0), b.F(BatchPlan::F1), b.F(BatchPlan::F2)};
+    float *AW[3] = {b.F(BatchPlan::A0), b.F(BatchPlan::A1), b.F(BatchPlan::A2)};
+    // the adjoint wavefield starts from zero (propagation.c:504-506); only the
+    // three work fields of every arena are cleared, the checkpoints stay
+    for (int s = 0; s < b.S; ++s)
+        HIPCHECK(hipMemsetAsync(b.arena + (long)s * b.stride, 0, 3 * g.fsz * sizeof(float), c.st));
+    HIPCHECK(hipMemsetAsync(b.acc, 0, (size_t)b.S * b.nacc * sizeof(double), c.st));
+    FdArgs fr = base_args(g, b.coef, b.S, b.stride);
+    fr.src_cell = b.cell_dev;
+    fr.src_wav = b.wav_dev;
+    fr.nt = nt;
+    FdArgs ad = base_args(g, b.coef, b.S, b.stride);
+    ad.acc = b.acc;
+    ad.acc_stride = b.nacc;
+    const float *K = b.coef + CO_K * g.fsz;
+    const float *cur = AW[0], *prev = AW[1];
+    for (int it = nt - 1; it >= 0; --it) {
+        if (!stored[it]) {
+            // get_acoustic2dCheckPointStatus: the segment restarts at the last
+            // multiple of d below it, from the stored pair (kd-1, kd)
+            const int s0 = (it / sc.d) * sc.d;
+            const float *fprev = (s0 == 0) ? b.F(BatchPlan::ZERO) : where[s0 - 1];
+            const float *fcur = where[s0];
+            for (int itt = s0; itt < it; ++itt) {
+                const int t = itt + 1;
+                float *fout = (t % MST == 0) ? b.F(ring0 + (t - s0) / MST) : pick_free(FW, fcur, fprev);
+                fr.out = fout;
+                fr.cur = fcur;
+                fr.prev = fprev;
+                fr.it = itt;
+                fd_launch(FD_RECON, fr, b.cells_per_launch);
+                fprev = fcur;
+                fcur = fout;
+                stored[t] = 1;
+                where[t] = fout;
+            }
+        }
+        launch_inject(b.tb.inj, const_cast<float *>(cur), b.stride, K, b.rec, it, g.dt, g.invdx,
+                      g.invdz, c.st);
+        float *out = pick_free(AW, cur, prev);
+        ad.out = out;
+        ad.cur = cur;
+        ad.prev = prev;
+        ad.it = it;
+        ad.acc_on = (it % MST == 0) ? 1 : 0;
+        ad.u = (it % MST == 0) ? where[it] : nullptr;
+        fd_launch(FD_ADJ, ad, b.cells_per_launch);
+        prev = cur;
+        cur = out;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// batch size
+
+static size_t per_shot_bytes(const Geom &g, const Schedule &sc, bool adjoint, long rec_floats,
+                             int nt)
+{
+    size_t b = (size_t)fields_per_shot(sc, adjoint) * g.fsz * sizeof(float);
+    b += (size_t)2 * rec_floats * sizeof(float) + (size_t)nt * sizeof(float);
+    if (adjoint) b += (size_t)g.nx * g.nz * (sizeof(double) + 2 * sizeof(float));
+    return b;
+}
+
+static int choose_batch(int n, size_t per_shot)
+{
+    Ctx &c = g_ctx;
+    if (c.batch_req > 0) return std::min(n, c.batch_req);
+    size_t fr = 0, tot = 0;
+    HIPCHECK(hipMemGetInfo(&fr, &tot));
+    // buffers the engine already holds are reused
+    size_t held = c.arena.cap + c.acc.cap + c.rec.cap + c.obs.cap + c.src.cap + c.misc[0].cap +
+                  c.misc[1].cap;
+    double budget = ((double)fr + (double)held) * c.memfrac;
+    long S = (long)(budget / (double)per_shot);
+    if (S < 1) S = 1;
+    return (int)std::min<long>(n, S);
+}
+
+// ---------------------------------------------------------------------------
+// groups of shots with identical coefficient arrays
+
+struct GroupKey {
+    int ix_min, ix_max;
+    uint32_t fpeak_bits;
+    bool operator<(const GroupKey &o) const
+    {
+        return std::tie(ix_min, ix_max, fpeak_bits) < std::tie(o.ix_min, o.ix_max, o.fpeak_bits);
+    }
+};
+
+static std::vector<std::vector<int>> group_shots(const Record2D *obs, int Nshot, int nx)
+{
+    std::map<GroupKey, int> idx;
+    std::vector<std::vector<int>> groups;
+    for (int s = 0; s < Nshot; ++s) {
+        const Record2D &r = obs[s];
+        if (r.ix_min < 0 || r.ix_max > nx - 1 || r.ix_min > r.ix_max)
+            fail("<error>: shot %d has an invalid window [%d, %d] for nx=%d", s + 1, r.ix_min,
+                 r.ix_max, nx);
+        uint32_t fb;
+        memcpy(&fb, &r.fpeak, 4);
+        GroupKey k{r.ix_min, r.ix_max, fb};
+        auto it = idx.find(k);
+        if (it == idx.end()) {
+            idx[k] = (int)groups.size();
+            groups.push_back({s});
+        } else {
+            groups[it->second].push_back(s);
+        }
+    }
+    return groups;
+}
+
+static void check_common(int nz, int nx, float dx, float dz, float dt, int nt, int pml, int fs)
+{
+    if (nz < 4 || nx < 1) fail("<error>: model too small (nz=%d, nx=%d)", nz, nx);
+    if (nt < 1) fail("<error>: nt must be positive (nt=%d)", nt);
+    if (pml < 3) fail("<error>: pmlThick must be at least 3 (the stencil half-width), got %d", pml);
+    if (fs != 0 && fs != 1) fail("<error>: freesurface must be 0 or 1, got %d", fs);
+    if (!(dx > 0.0f) || !(dz > 0.0f) || !(dt > 0.0f))
+        fail("<error>: dx, dz, dt must be positive");
+}
+
+
+// The shot loop of iso_acoustic2d_propagation (propagation.c:854-1035) with
+// the choice of where the observed records come from and whether the
+// predicted / adjoint-source records go back to the host.
+struct Call {
+    float **vp, **rho;          // host models (vp also feeds the adjoint-source taper)
+    Record2D *obs, *pre;
+    int Nshot, nz, nx;
+    float dx, dz, dt;
+    int nt, pml, fs;
+    float freq;
+    int filter, precondition, problem;
+    float **grad;               // host gradient (problem 1), may be null with grad_dev
+    const float *obs_dev = nullptr;   // all observed records on the device (session)
+    const long *obs_off = nullptr;    // per-shot offsets into obs_dev
+    bool records_to_host = true;
+    float *grad_dev = nullptr;        // device gradient output (session)
+    double *misfit = nullptr;         // 0.5 * sum of squared adjoint sources (problem 1/2)
+    bool models_on_device = false;    // c.model[] already hold vp, rho
+};
+
+static void propagate_core(const Call &P)
+{
+    Ctx &c = ctx();
+    const int Nshot = P.Nshot, nz = P.nz, nx = P.nx, nt = P.nt, problem = P.problem;
+    const long nmod = (long)nx * nz;
+    float *vpd = c.model[0].as<float>(nmod);
+    float *rhod = c.model[1].as<float>(nmod);
+    if (!P.models_on_device) {
+        h2d(vpd, P.vp[0], nmod * sizeof(float));
+        h2d(rhod, P.rho[0], nmod * sizeof(float));
+    }
+    std::vector<std::vector<int>> groups = group_shots(P.obs, Nshot, nx);
+    long win_max = 0;
+    for (int s = 0; s < Nshot; ++s)
+        win_max = std::max<long>(win_max, (long)(P.obs[s].ix_max - P.obs[s].ix_min + 1) * nz);
+    float *gout = nullptr, *pout = nullptr;
+    if (problem == 1) {
+        float *so = c.shot_out.as<float>((size_t)2 * Nshot * win_max);
+        gout = so;
+        pout = so + (size_t)Nshot * win_max;
+    }
+    double *mis = nullptr;
+    if (P.misfit) {
+        mis = c.gsum.as<double>((size_t)2 * nmod + 2 * Nshot + 64) + 2 * nmod;
+        HIPCHECK(hipMemsetAsync(mis, 0, (size_t)2 * Nshot * sizeof(double), c.st));
+    }
+    Schedule sc(nt);
+    for (auto &grp : groups) {
+        const Record2D &r0 = P.obs[grp[0]];
+        const int nxx = r0.ix_max - r0.ix_min + 1;
+        Geom g = make_geom(nxx, nz, P.pml, P.fs, P.dx, P.dz, P.dt);
+        float *coef = c.coef.as<float>((size_t)CO_N * g.fsz);
+        const float *vpw = vpd + (long)r0.ix_min * nz, *rhow = rhod + (long)r0.ix_min * nz;
+        build_coefs(g, vpw, rhow, r0.fpeak, coef);
+        long recmax = 0;
+        for (int s : grp) recmax = std::max<long>(recmax, (long)P.pre[s].ntr * P.pre[s].ns);
+        const int S = choose_batch((int)grp.size(), per_shot_bytes(g, sc, problem == 1, recmax, nt));
+        c.stats.batch = S;
+        for (size_t b0 = 0; b0 < grp.size(); b0 += S) {
+            const int nb = (int)std::min<size_t>(S, grp.size() - b0);
+            std::vector<Record2D *> recs(nb);
+            for (int k = 0; k < nb; ++k) {
+                Record2D *pr = &P.pre[grp[b0 + k]];
+                if (pr->ntr != P.obs[grp[b0 + k]].ntr || pr->ns != P.obs[grp[b0 + k]].ns)
+                    fail("<error>: predicted and observed records of shot %d differ in shape",
+                         grp[b0 + k] + 1);
+                recs[k] = pr;
+            }
+            BatchState b;
+            prepare_batch(b, g, coef, recs, nt, problem == 1, sc);
+            float *obsd = nullptr;
+            if (problem != 0) {
+                obsd = c.obs.as<float>(b.rec_total + 64);
+                for (int k = 0; k < nb; ++k) {
+                    const int sidx = grp[b0 + k];
+                    if (P.obs_dev)
+                        HIPCHECK(hipMemcpyAsync(obsd + b.roff[k], P.obs_dev + P.obs_off[sidx],
+                                                (size_t)P.obs[sidx].ntr * P.obs[sidx].ns * sizeof(float),
+                                                hipMemcpyDeviceToDevice, c.st));
+                    else
+                        rec_h2d(obsd + b.roff[k], &P.obs[sidx]);
+                }
+            }
+            const long nacc = b.nacc;
+            float *gb = nullptr, *pb = nullptr;
+            if (problem == 1) {
+                float *m = c.misc[0].as<float>((size_t)2 * nb * nacc);
+                gb = m;
+                pb = m + (size_t)nb * nacc;
+            }
+            forward_pass(b, problem == 1, problem == 1 && P.precondition == 1);
+            if (problem == 1 && P.precondition == 1) {
+                // propagation.c:610-623
+                launch_finalize(b.acc, nacc, pb, nacc, nb, vpw, rhow, 0, (int)nacc, 4.0, 1, c.st);
+            }
+            if (problem == 0) {
+                if (P.records_to_host)
+                    for (int k = 0; k < nb; ++k) rec_d2h(recs[k], b.rec + b.roff[k]);
+                continue;
+            }
+            for (int k = 0; k < nb; ++k)
+                adjoint_source_device(recs[k], P.vp, P.freq, P.filter, b.rec + b.roff[k],
+                                      obsd + b.roff[k], b.rec + b.roff[k]);
+            if (mis) {
+                double *scr = c.scratch.as<double>(reduce_scratch_doubles() + (size_t)Nshot * 512 + Nshot + 64);
+                for (int k = 0; k < nb; ++k) {
+                    launch_sum_sq(b.rec + b.roff[k], (long)recs[k]->ntr * recs[k]->ns, scr,
+                                  scr + 2 * 512, c.st);
+                    HIPCHECK(hipMemcpyAsync(mis + 2 * grp[b0 + k], scr + 2 * 512, 2 * sizeof(double),
+                                            hipMemcpyDeviceToDevice, c.st));
+                }
+            }
+            if (P.records_to_host)
+                for (int k = 0; k < nb; ++k) rec_d2h(recs[k], b.rec + b.roff[k]);
+            if (problem == 2) continue;
+            for (int k = 0; k < nb; ++k)
+                if (recs[k]->sampleRate != 1 || recs[k]->ns != nt)
+                    fail("<error>: gradient with sampleRate %d: trace interpolation (FFT) is not "
+                         "available in this build; record every time step",
+                         recs[k]->sampleRate);
+            adjoint_pass(b);
+            launch_finalize(b.acc, nacc, gb, nacc, nb, vpw, rhow, 0, (int)nacc, 2.0, 0, c.st);
+            if (P.precondition == 1) {
+                double *scr = c.scratch.as<double>((size_t)nb * 512 + nb + reduce_scratch_doubles());
+                launch_shot_precondition(gb, pb, nacc, nb, (int)nacc, scr, c.st);
+            }
+            for (int k = 0; k < nb; ++k) {
+                const int sidx = grp[b0 + k];
+                HIPCHECK(hipMemcpyAsync(gout + (size_t)sidx * win_max, gb + (size_t)k * nacc,
+                                        nacc * sizeof(float), hipMemcpyDeviceToDevice, c.st));
+                if (P.precondition == 1)
+                    HIPCHECK(hipMemcpyAsync(pout + (size_t)sidx * win_max, pb + (size_t)k * nacc,
+                                            nacc * sizeof(float), hipMemcpyDeviceToDevice, c.st));
+            }
+        }
+    }
+    if (problem == 1) {
+        // propagation.c:953-1011: shot-order FP64 sums, then global preconditioning
+        std::vector<int> ix0(Nshot), nxx(Nshot);
+        for (int s = 0; s < Nshot; ++s) {
+            ix0[s] = P.obs[s].ix_min;
+            nxx[s] = P.obs[s].ix_max - P.obs[s].ix_min + 1;
+        }
+        int *wi = c.misc[2].as<int>((size_t)2 * Nshot);
+        h2d(wi, ix0.data(), Nshot * sizeof(int));
+        h2d(wi + Nshot, nxx.data(), Nshot * sizeof(int));
+        double *sums = c.gsum.as<double>((size_t)2 * nmod + 2 * Nshot + 64);
+        HIPCHECK(hipMemsetAsync(sums, 0, (size_t)2 * nmod * sizeof(double), c.st));
+        launch_accumulate(gout, win_max, Nshot, wi, wi + Nshot, nz, nx, sums, c.st);
+        float *gf = P.grad_dev ? P.grad_dev : c.misc[3].as<float>((size_t)2 * nmod);
+        float *pf = c.misc[3].as<float>((size_t)2 * nmod) + nmod;
+        launch_to_float(sums, gf, nmod, c.st);
+        if (P.precondition == 1) {
+            launch_accumulate(pout, win_max, Nshot, wi, wi + Nshot, nz, nx, sums + nmod, c.st);
+            launch_to_float(sums + nmod, pf, nmod, c.st);
+            double *scr = c.scratch.as<double>(reduce_scratch_doubles() + (size_t)Nshot * 512 + Nshot + 64);
+            launch_global_precondition(gf, pf, nmod, 0.005f, scr, c.st);
+        }
+        if (P.grad && P.grad[0]) d2h(P.grad[0], gf, nmod * sizeof(float));
+    }
+    if (mis) {
+        std::vector<double> h((size_t)2 * Nshot);
+        d2h(h.data(), mis, h.size() * sizeof(double));
+        sync();
+        double f = 0.0;
+        for (int s = 0; s < Nshot; ++s) f += h[2 * s];
+        *P.misfit = 0.5 * f;
+    }
+    sync();
+    collect_timing();
+}
+
+}  // namespace fwiedu
+
+using namespace fwiedu;
+
+extern "C" void iso_acoustic2d_propagation(float **vp, float **rho, Record2D *record2d_obs,
+                                           int Nshot, int nz, int nx, float dx, float dz, float dt,
+                                           int nt, int pmlThick, int freesurface, float freq,
+                                           float filter, int precondition, int problem, int iter,
+                                           int verbose, Record2D *record2d_pre, float **grad)
+{
+    (void)iter;
+    (void)verbose;
+    check_common(nz, nx, dx, dz, dt, nt, pmlThick, freesurface);
+    if (Nshot <= 0) fail("<error>: Nshot must be positive");
+    if (problem < 0 || problem > 2) fail("<error>: problem must be 0, 1 or 2 (got %d)", problem);
+    if (problem == 1 && (!grad || !grad[0])) fail("<error>: problem 1 needs a gradient array");
+    ensure_ready();
+    Call P;
+    P.vp = vp;
+    P.rho = rho;
+    P.obs = record2d_obs;
+    P.pre = record2d_pre;
+    P.Nshot = Nshot;
+    P.nz = nz;
+    P.nx = nx;
+    P.dx = dx;
+    P.dz = dz;
+    P.dt = dt;
+    P.nt = nt;
+    P.pml = pmlThick;
+    P.fs = freesurface;
+    P.freq = freq;
+    P.filter = (int)filter;  // float here, int in compute_adjoint_source2d
+    P.precondition = precondition;
+    P.problem = problem;
+    P.grad = grad;
+    propagate_core(P);
+}
+
+// ---------------------------------------------------------------------------
+// Device-resident session: model, observed records and all wavefield state in
+// HBM; repeated gradients move no traces over PCIe.
+
+struct Session {
+    Call P;
+    std::vector<float> vp, rho;            // host copies (the taper reads vp on the host)
+    std::vector<float *> vrow, rrow;
+    std::vector<Record2D> obs, pre;        // headers; traces stay on the device
+    DevBuf obs_dev;
+    std::vector<long> off;
+    DevBuf grad_dev;
+};
+
+extern "C" void *fwiedu_session_create(float **vp, float **rho, Record2D *record2d_obs, int Nshot,
+                                       int nz, int nx, float dx, float dz, float dt, int nt,
+                                       int pmlThick, int freesurface, float freq, float filter,
+                                       int precondition)
+{
+    check_common(nz, nx, dx, dz, dt, nt, pmlThick, freesurface);
+    if (Nshot <= 0) fail("<error>: Nshot must be positive");
+    ensure_ready();
+    Session *S = new Session();
+    const long nmod = (long)nx * nz;
+    S->vp.assign(vp[0], vp[0] + nmod);
+    S->rho.assign(rho[0], rho[0] + nmod);
+    S->vrow.resize(nx);
+    S->rrow.resize(nx);
+    for (int ix = 0; ix < nx; ++ix) {
+        S->vrow[ix] = S->vp.data() + (long)ix * nz;
+        S->rrow[ix] = S->rho.data() + (long)ix * nz;
+    }
+    S->obs.assign(record2d_obs, record2d_obs + Nshot);
+    S->pre = S->obs;
+    S->off.resize(Nshot);
+    long tot = 0;
+    for (int s = 0; s < Nshot; ++s) {
+        S->off[s] = tot;
+        tot += (long)record2d_obs[s].ntr * record2d_obs[s].ns;
+    }
+    float *d = S->obs_dev.as<float>(tot + 64);
+    for (int s = 0; s < Nshot; ++s)
+        h2d(d + S->off[s], record2d_obs[s].traces_p[0],
+            (size_t)record2d_obs[s].ntr * record2d_obs[s].ns * sizeof(float));
+    Call &P = S->P;
+    P.vp = S->vrow.data();
+    P.rho = S->rrow.data();
+    P.obs = S->obs.data();
+    P.pre = S->pre.data();
+    P.Nshot = Nshot;
+    P.nz = nz;
+    P.nx = nx;
+    P.dx = dx;
+    P.dz = dz;
+    P.dt = dt;
+    P.nt = nt;
+    P.pml = pmlThick;
+    P.fs = freesurface;
+    P.freq = freq;
+    P.filter = (int)filter;
+    P.precondition = precondition;
+    P.problem = 1;
+    P.obs_dev = d;
+    P.obs_off = S->off.data();
+    P.records_to_host = false;
+    P.grad_dev = S->grad_dev.as<float>(nmod + 64);
+    return S;
+}
+
+extern "C" void fwiedu_session_set_model(void *sp, float **vp, float **rho)
+{
+    Session *S = static_cast<Session *>(sp);
+    const long nmod = (long)S->P.nx * S->P.nz;
+    memcpy(S->vp.data(), vp[0], nmod * sizeof(float));
+    memcpy(S->rho.data(), rho[0], nmod * sizeof(float));
+}
+
+// One gradient (problem 1) on resident data; returns the misfit of the
+// processed residual.  grad (host) may be null: the gradient then stays on
+// the device.
+extern "C" float fwiedu_session_gradient(void *sp, float **grad)
+{
+    Session *S = static_cast<Session *>(sp);
+    double f = 0.0;
+    Call P = S->P;
+    P.grad = grad;
+    P.misfit = &f;
+    propagate_core(P);
+    return (float)f;
+}
+
+extern "C" void fwiedu_session_destroy(void *sp)
+{
+    Session *S = static_cast<Session *>(sp);
+    if (!S) return;
+    sync();
+    S->obs_dev.release();
+    S->grad_dev.release();
+    delete S;
+}
+
+// Per-shot engine (propagation.c:356-775) with the reference's host checkpoint.
+extern "C" void iso_acoustic2d_propagation_engine(Record2D *record2d, Acoustic2dCheckPoint *chkpt,
+                                                  float **vpm, float **rhom, int nx, int nz,
+                                                  float dx, float dz, float dt, int nt,
+                                                  int pmlThick, int freesurface, int precondition,
+                                                  int mode, int iter, int verbose, float **grad)
+{
+    (void)iter;
+    (void)verbose;
+    check_common(nz, nx, dx, dz, dt, nt, pmlThick, freesurface);
+    if (mode < 0 || mode > 2) fail("<error>: engine mode must be 0, 1 or 2 (got %d)", mode);
+    if (mode != 0 && !chkpt) fail("<error>: engine mode %d needs a checkpoint", mode);
+    if ((mode == 2 || (mode == 1 && precondition == 1)) && (!grad || !grad[0]))
+        fail("<error>: engine mode %d needs an output array", mode);
+    ensure_ready();
+    Ctx &c = ctx();
+    const long nmod = (long)nx * nz;
+    float *vpd = c.model[0].as<float>(nmod);
+    float *rhod = c.model[1].as<float>(nmod);
+    h2d(vpd, vpm[0], nmod * sizeof(float));
+    h2d(rhod, rhom[0], nmod * sizeof(float));
+    Geom g = make_geom(nx, nz, pmlThick, freesurface, dx, dz, dt);
+    float *coef = c.coef.as<float>((size_t)CO_N * g.fsz);
+    build_coefs(g, vpd, rhod, record2d->fpeak, coef);
+    Schedule sc(nt);
+    BatchState b;
+    std::vector<Record2D *> recs = {record2d};
+    prepare_batch(b, g, coef, recs, nt, mode != 0, sc);
+    const long nacc = b.nacc;
+    if (mode == 0 || mode == 1) {
+        forward_pass(b, mode == 1, mode == 1 && precondition == 1);
+        rec_d2h(record2d, b.rec);
+        if (mode == 1) {
+            // hand the pairs (kd-1, kd) to the host checkpoint, status set
+            if (chkpt->nt != nt || !chkpt->p || !chkpt->status)
+                fail("<error>: checkpoint not allocated for nt=%d", nt);
+            for (int it = 0; it < nt; ++it) {
+                if (sc.ckslot[it] == -1) continue;
+                if (sc.ckslot[it] == -2)
+                    memset(chkpt->p[it][0], 0, (size_t)g.Nx * g.Nz * sizeof(float));
+                else
+                    HIPCHECK(hipMemcpy2DAsync(chkpt->p[it][0], g.Nz * sizeof(float),
+                                              b.F(BatchPlan::CK0 + sc.ckslot[it]),
+                                              g.ld * sizeof(float), g.Nz * sizeof(float), g.Nx,
+                                              hipMemcpyDeviceToHost, c.st));
+                chkpt->status[it] = 1;
+            }
+            if (precondition == 1) {
+                float *pb = c.misc[0].as<float>(nacc);
+                launch_finalize(b.acc, nacc, pb, nacc, 1, vpd, rhod, 0, (int)nacc, 4.0, 1, c.st);
+                d2h(grad[0], pb, nacc * sizeof(float));
+            }
+        }
+    } else {
+        if (record2d->sampleRate != 1 || record2d->ns != nt)
+            fail("<error>: adjoint with sampleRate %d needs trace interpolation (FFT), not "
+                 "available in this build",
+                 record2d->sampleRate);
+        if (chkpt->nt != nt || !chkpt->p) fail("<error>: checkpoint not allocated for nt=%d", nt);
+        for (int it = 0; it < nt; ++it) {
+            if (sc.ckslot[it] < 0) continue;
+            HIPCHECK(hipMemcpy2DAsync(b.F(BatchPlan::CK0 + sc.ckslot[it]), g.ld * sizeof(float),
+                                      chkpt->p[it][0], g.Nz * sizeof(float), g.Nz * sizeof(float),
+                                      g.Nx, hipMemcpyHostToDevice, c.st));
+        }
+        rec_h2d(b.rec, record2d);
+        adjoint_pass(b);
+        float *gb = c.misc[0].as<float>(nacc);
+        launch_finalize(b.acc, nacc, gb, nacc, 1, vpd, rhod, 0, (int)nacc, 2.0, 0, c.st);
+        d2h(grad[0], gb, nacc * sizeof(float));
+    }
+    sync();
+    collect_timing();
+}

@@ -1,0 +1,102 @@
+// Host-side runtime shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/fwiedu.h"
+#include "kernels.hpp"
+
+namespace fwiedu {
+
+// SU err() convention: message on stderr, exit(1).
+[[noreturn]] void fail(const char *fmt, ...);
+
+#define HIPCHECK(x)                                                                      \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess)                                                            \
+            ::fwiedu::fail("<error>: %s failed: %s (%s:%d)", #x, hipGetErrorString(e_),  \
+                           __FILE__, __LINE__);                                          \
+    } while (0)
+
+// A grow-only device allocation.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    void *get(size_t bytes);
+    template <class T> T *as(size_t n) { return static_cast<T *>(get(n * sizeof(T))); }
+    void release();
+};
+
+struct Stats {
+    long fd_launches = 0;
+    long fd_cells = 0;        // interior cell-updates issued (all shots)
+    long fd_timed = 0;
+    double fd_ms_timed = 0;
+    double fd_cells_timed = 0;
+    long batch = 0;
+    long shots_per_launch = 0;
+    double device_bytes = 0;
+    double h2d_bytes = 0, d2h_bytes = 0;
+};
+
+struct Ctx {
+    bool ready = false;
+    int device = 0;
+    hipStream_t st = nullptr;
+    int batch_req = 0;
+    double memfrac = 0.85;
+    int ktiming = 0;
+    long recon_launches = 0;
+    int ckpt_bits = 32;
+    Stats stats;
+    std::vector<hipEvent_t> ev;   // pool of events for sampled kernel timing
+    size_t ev_used = 0;
+    std::vector<std::pair<long, long>> ev_cells;  // (event pair index, cells)
+    // device buffers
+    DevBuf model[2];              // full vp, rho [nx][nz]
+    DevBuf setup;                 // padded vp, rho, D, Beta + weights
+    DevBuf coef;                  // CO_N fields
+    DevBuf arena;                 // shot arenas
+    DevBuf acc;                   // double accumulators
+    DevBuf rec, obs;              // records of the batch
+    DevBuf src;                   // wavelets of the batch
+    DevBuf tables;                // receiver tables
+    DevBuf taper;                 // taper tables
+    DevBuf shot_out;              // per-shot gradient and pseudo-Hessian of all shots
+    DevBuf gsum;                  // double sums
+    DevBuf scratch;               // reductions
+    DevBuf misc[4];
+};
+
+Ctx &ctx();
+void ensure_ready();
+void sync();
+void h2d(void *dst, const void *src, size_t bytes);
+void d2h(void *dst, const void *src, size_t bytes);
+
+// Geometry of a padded window (engine loop bounds, propagation.c:443-537).
+Geom make_geom(int nx, int nz, int pml, int fs, float dx, float dz, float dt);
+
+std::vector<float> meal_weights(float alpha, int nl);
+float meal_lnR(int nl, float pd);
+
+// Build the 7 coefficient arrays of a window on the device.
+void build_coefs(const Geom &g, const float *vpw_dev, const float *rhow_dev, float fpeak,
+                 float *coef_dev);
+
+// compute_adjoint_source2d with filter 2 on device records [ntr][ns].
+void adjoint_source_device(const Record2D *r, float **vp_full, float freq, int filter,
+                           const float *pre_dev, const float *obs_dev, float *out_dev);
+
+// Sampled timing of stencil launches.
+void fd_launch(int mode, const FdArgs &a, long cells);
+void collect_timing();
+
+}  // namespace fwiedu

@@ -1,0 +1,681 @@
+// HIP kernels for gfx950 (CDNA4) of the 2D acoustic FWI hot path.
+//
+// Arithmetic follows the reference expressions operand for operand
+// (FWILab2d/FWILab2dCLangV1.2.6/src/propagation.c, optimization.c, common.c) and
+// FMA contraction is off in this file, so every FP32 result is the IEEE result
+// of the same sequence of adds and multiplies as a strict C evaluation of the
+// reference.  The stencil is HBM-bound (about 1.3 flop per byte), so the fused
+// multiply-adds would buy nothing.
+#include "kernels.hpp"
+
+#include <math.h>
+
+#pragma clang fp contract(off)
+
+namespace fwiedu {
+
+// ---------------------------------------------------------------------------
+// The stencil step.  One block = 4 waves of 64 lanes.  A tile is 64 rows of z
+// (one row per lane, so every global access of a wave is one coalesced 256 B
+// column segment) by TX = 16 columns of x (4 per wave).  p1 of the tile plus
+// its 3-cell halo is staged in LDS once; the 13-point cross is read from there.
+// p0, the 7 coefficients and the output go straight between HBM and registers.
+//
+// Epilogue, fused per cell (propagation.c):
+//   FD_FWD   : +source in double (l.558), free-surface mirror (l.76-101),
+//              pseudo-Hessian every MST steps (l.583-593)
+//   FD_RECON : free-surface mirror, then +source in float (l.655-662)
+//   FD_ADJ   : free-surface mirror, imaging every MST steps (l.700-713)
+// The mirror is done by the cell being mirrored: a cell at row pml+k,
+// 1 <= k <= pml, also writes -value to row pml-k; row pml itself becomes -0.
+constexpr int TZ = 64;
+constexpr int NWAVE = 4;
+constexpr int CX = 4;
+constexpr int TX = NWAVE * CX;
+constexpr int HX = TX + 6;
+constexpr int HZ = TZ + 6;
+
+template <int MODE>
+__global__ __launch_bounds__(256) void fd_kernel(FdArgs a)
+{
+    __shared__ float t[HX][HZ];
+    const float C1 = 0.77793115f, C2 = -0.17388691f, C3 = 0.02338713f;
+    const float D0 = -2.8215452f, D1 = 1.57663238f, D2 = -0.18347238f, D3 = 0.01761260f;
+
+    const int tile = blockIdx.x;
+    const int s = blockIdx.y;
+    const int xt = tile % a.xtiles, zt = tile / a.xtiles;
+    const int x0 = a.ixs + xt * TX, z0 = a.izs + zt * TZ;
+    const long so = (long)s * a.shot_stride;
+    const float *__restrict__ p1 = a.cur + so;
+    const float *__restrict__ p0 = a.prev + so;
+    float *__restrict__ p2 = a.out + so;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ld = a.ld;
+
+    // stage p1: columns x0-3 .. x0+TX+2, rows z0-3 .. z0+66 (x0 >= 3, z0 >= 3)
+    for (int c = w; c < HX; c += NWAVE) {
+        const int ix = x0 - 3 + c, iz = z0 + lane;
+        float v = 0.f;
+        if (ix < a.Nx && iz < ld) v = p1[(long)ix * ld + iz];
+        t[c][lane + 3] = v;
+    }
+    for (int idx = threadIdx.x; idx < HX * 6; idx += 256) {
+        const int c = idx / 6, h = idx - c * 6;
+        const int r = h < 3 ? h : TZ + h;
+        const int ix = x0 - 3 + c, iz = z0 - 3 + r;
+        float v = 0.f;
+        if (ix < a.Nx && iz < ld) v = p1[(long)ix * ld + iz];
+        t[c][r] = v;
+    }
+    __syncthreads();
+
+    const float *__restrict__ cf = a.coef;
+    const long fsz = (long)a.Nx * ld;
+    int scell = -1;
+    if (MODE != FD_ADJ && a.src_cell) scell = a.src_cell[s];
+
+#pragma unroll
+    for (int j = 0; j < CX; ++j) {
+        const int c = 3 + w * CX + j, r = lane + 3;
+        const int ix = x0 + w * CX + j, iz = z0 + lane;
+        if (ix > a.ixe || iz > a.ize) continue;
+        const long i = (long)ix * ld + iz;
+        const float pc = t[c][r];
+        const float dpdz = (C1 * (t[c][r + 1] - t[c][r - 1]) + C2 * (t[c][r + 2] - t[c][r - 2]) +
+                            C3 * (t[c][r + 3] - t[c][r - 3])) * a.invdz;
+        const float dpdx = (C1 * (t[c + 1][r] - t[c - 1][r]) + C2 * (t[c + 2][r] - t[c - 2][r]) +
+                            C3 * (t[c + 3][r] - t[c - 3][r])) * a.invdx;
+        const float dzz = (D0 * pc + D1 * (t[c][r + 1] + t[c][r - 1]) +
+                           D2 * (t[c][r + 2] + t[c][r - 2]) + D3 * (t[c][r + 3] + t[c][r - 3])) *
+                          a.invdzdz;
+        const float dxx = (D0 * pc + D1 * (t[c + 1][r] + t[c - 1][r]) +
+                           D2 * (t[c + 2][r] + t[c - 2][r]) + D3 * (t[c + 3][r] + t[c - 3][r])) *
+                          a.invdxdx;
+        const float p0v = p0[i];
+        const float v = cf[CO_C1 * fsz + i] * pc + cf[CO_C2 * fsz + i] * p0v +
+                        cf[CO_C3 * fsz + i] * cf[CO_K * fsz + i] *
+                            (cf[CO_IRX * fsz + i] * dpdx + cf[CO_IRZ * fsz + i] * dpdz +
+                             cf[CO_IR * fsz + i] * (dxx + dzz));
+
+        float val = v;
+        if (MODE == FD_FWD && (int)i == scell) {
+            const double srcv = 1.0 * (double)a.src_wav[(long)s * a.nt + a.it] *
+                                (double)cf[CO_K * fsz + i] * (double)a.dt * (double)a.dt *
+                                (double)a.invdx * (double)a.invdz;
+            val = (float)((double)v + srcv);
+        }
+        float stored = val;
+        if (a.fs) {
+            const int k = iz - a.pml;
+            if (k == 0)
+                stored = -0.0f;
+            else if (k <= a.pml)
+                p2[(long)ix * ld + (a.pml - k)] = -val;
+        }
+        if (MODE == FD_RECON && (int)i == scell) {
+            stored = stored + a.src_wav[(long)s * a.nt + a.it] * cf[CO_K * fsz + i] * a.dt * a.dt *
+                                  a.invdx * a.invdz;
+        }
+        p2[i] = stored;
+
+        if (MODE != FD_RECON && a.acc_on) {
+            const int px = ix - a.pml, pz = iz - a.pml;
+            if (px >= 0 && px < a.nx && pz >= 0 && pz < a.nz) {
+                double *acc = a.acc + (long)s * a.acc_stride + (long)px * a.nz + pz;
+                if (MODE == FD_FWD) {
+                    const float q = (stored - 2.0f * pc + p0v) * a.invdtdt;
+                    *acc += (double)(q * q);
+                } else {
+                    const float q = (stored - 2.0f * pc + p0v) * a.invdtdt * a.u[so + i];
+                    *acc += (double)q;
+                }
+            }
+        }
+    }
+}
+
+void launch_fd(int mode, const FdArgs &a0, hipStream_t st)
+{
+    FdArgs a = a0;
+    a.xtiles = (a.ixe - a.ixs + 1 + TX - 1) / TX;
+    a.ztiles = (a.ize - a.izs + 1 + TZ - 1) / TZ;
+    dim3 grid(a.xtiles * a.ztiles, a.nshot);
+    if (mode == FD_FWD)
+        fd_kernel<FD_FWD><<<grid, 256, 0, st>>>(a);
+    else if (mode == FD_RECON)
+        fd_kernel<FD_RECON><<<grid, 256, 0, st>>>(a);
+    else
+        fd_kernel<FD_ADJ><<<grid, 256, 0, st>>>(a);
+}
+
+// ---------------------------------------------------------------------------
+// Set-up kernels (propagation.c:103-171, 192-354, 508-532; common.c:110-176,416-479)
+
+static inline dim3 grid2(const Geom &g, int ldz) { return dim3((ldz + 63) / 64, g.Nx); }
+
+__global__ void pad_kernel(const float *win, int nx, int nz, int pml, float *out, int Nz, int ld)
+{
+    const int iz = blockIdx.x * 64 + threadIdx.x, ix = blockIdx.y;
+    if (iz >= ld) return;
+    float v = 0.f;
+    if (iz < Nz) {
+        int sx = ix - pml, sz = iz - pml;
+        sx = sx < 0 ? 0 : (sx > nx - 1 ? nx - 1 : sx);
+        sz = sz < 0 ? 0 : (sz > nz - 1 ? nz - 1 : sz);
+        v = win[(long)sx * nz + sz];
+    }
+    out[(long)ix * ld + iz] = v;
+}
+
+void launch_pad(const float *win, int nx, int nz, int pml, float *out, const Geom &g,
+                hipStream_t st)
+{
+    pad_kernel<<<grid2(g, g.ld), 64, 0, st>>>(win, nx, nz, pml, out, g.Nz, g.ld);
+}
+
+__global__ void meal_kernel(const float *vp, const float *vs, float *D, float *Beta, int NX,
+                            int NZ, int ld, int nl, const float *wt, float lnR, float pd,
+                            float sqrt3, float fpeak, float dx, float dz)
+{
+    const int iz = blockIdx.x * 64 + threadIdx.x, ix = blockIdx.y;
+    if (iz >= ld) return;
+    const long i = (long)ix * ld + iz;
+    if (iz >= NZ) {
+        D[i] = 0.f;
+        Beta[i] = 1.f;
+        return;
+    }
+    const float v = vp[i];
+    float Vs = v / sqrt3;
+    if (vs && vs[i] != 0.0f) Vs = vs[i];
+    const float thx = (float)nl * dx, thz = (float)nl * dz;
+    const float PPWfd = 10.0f;
+    float Dx = 0.f, Dz = 0.f, Bx = 1.f, Bz = 1.f;
+    // top then bottom (propagation.c:285-309), left then right (:311-335)
+    if (iz < nl) {
+        const float w = wt[nl - iz];
+        const float d0 = -1.0f * (1.0f + pd) * v * lnR / (2.0f * thz);
+        const float b = 2.0f * (Vs / (dz * fpeak)) / PPWfd;
+        Dz = d0 * w;
+        Bz = 1.0f + (b - 1.0f) * w;
+    }
+    if (iz >= NZ - nl) {
+        const float w = wt[iz - (NZ - nl - 1)];
+        const float d0 = -1.0f * (1.0f + pd) * v * lnR / (2.0f * thz);
+        const float b = 2.0f * (Vs / (dz * fpeak)) / PPWfd;
+        Dz = d0 * w;
+        Bz = 1.0f + (b - 1.0f) * w;
+    }
+    if (ix < nl) {
+        const float w = wt[nl - ix];
+        const float d0 = -1.0f * (1.0f + pd) * v * lnR / (2.0f * thx);
+        const float b = 2.0f * (Vs / (dx * fpeak)) / PPWfd;
+        Dx = d0 * w;
+        Bx = 1.0f + (b - 1.0f) * w;
+    }
+    if (ix >= NX - nl) {
+        const float w = wt[ix - (NX - nl - 1)];
+        const float d0 = -1.0f * (1.0f + pd) * v * lnR / (2.0f * thx);
+        const float b = 2.0f * (Vs / (dx * fpeak)) / PPWfd;
+        Dx = d0 * w;
+        Bx = 1.0f + (b - 1.0f) * w;
+    }
+    D[i] = sqrtf(Dx * Dx + Dz * Dz);
+    Beta[i] = sqrtf((Bx - 1.0f) * (Bx - 1.0f) + (Bz - 1.0f) * (Bz - 1.0f)) + 1.0f;
+}
+
+void launch_meal(const float *vp, const float *vs, float *D, float *Beta, const Geom &g, int nlayer,
+                 const float *w_dev, float lnR, float pd, float sqrt3, float fpeak, hipStream_t st)
+{
+    meal_kernel<<<grid2(g, g.ld), 64, 0, st>>>(vp, vs, D, Beta, g.Nx, g.Nz, g.ld, nlayer, w_dev,
+                                               lnR, pd, sqrt3, fpeak, g.dx, g.dz);
+}
+
+__global__ void coef_kernel(float *vp, float *rho, float *D, const float *Beta, float *coef,
+                            int Nz, int ld, long fsz, float dt)
+{
+    const int iz = blockIdx.x * 64 + threadIdx.x, ix = blockIdx.y;
+    if (iz >= ld) return;
+    const long i = (long)ix * ld + iz;
+    if (iz >= Nz) {
+        for (int k = 0; k < CO_N; ++k) coef[k * fsz + i] = 0.f;
+        return;
+    }
+    const float b = Beta[i];
+    const float d = D[i] / b;
+    const float v = vp[i] / b;
+    const float r = rho[i] * b;
+    D[i] = d;
+    vp[i] = v;
+    rho[i] = r;
+    // propagation.c:525-527: the products are FP32, the rest FP64
+    const float dt2D2 = dt * dt * d * d;
+    const float dtD = dt * d;
+    const float dt2 = dt * dt;
+    coef[CO_C1 * fsz + i] = (float)((2.0 - (double)dt2D2) / (1.0 + (double)dtD));
+    coef[CO_C2 * fsz + i] = (float)(-1.0 * (1.0 - (double)dtD) / (1.0 + (double)dtD));
+    coef[CO_C3 * fsz + i] = (float)((double)dt2 / (1.0 + (double)dtD));
+    coef[CO_K * fsz + i] = r * v * v;
+    coef[CO_IR * fsz + i] = 1.0f / r;
+}
+
+void launch_coefs(float *vp, float *rho, float *D, const float *Beta, float *coef, const Geom &g,
+                  hipStream_t st)
+{
+    coef_kernel<<<grid2(g, g.ld), 64, 0, st>>>(vp, rho, D, Beta, coef, g.Nz, g.ld, g.fsz, g.dt);
+}
+
+__global__ void grad_invrho_kernel(float *coef, int Nx, int Nz, int ld, long fsz, float dx,
+                                   float dz)
+{
+    const int iz = blockIdx.x * 64 + threadIdx.x, ix = blockIdx.y;
+    if (iz >= Nz) return;
+    const float *f = coef + CO_IR * fsz;
+    const long i = (long)ix * ld + iz;
+    float gx, gz;
+    if (ix >= 6 && ix < Nx - 6 && iz >= 6 && iz < Nz - 6) {
+        const float C1 = 0.91067892f, C2 = -0.34187892f, C3 = 0.13833962f;
+        const float C4 = -0.04880710f, C5 = 0.01302148f, C6 = -0.00199047f;
+        const float invdx = 1.0f / dx, invdz = 1.0f / dz;
+        gx = (C1 * (f[i + ld] - f[i - ld]) + C2 * (f[i + 2 * ld] - f[i - 2 * ld]) +
+              C3 * (f[i + 3 * ld] - f[i - 3 * ld]) + C4 * (f[i + 4 * ld] - f[i - 4 * ld]) +
+              C5 * (f[i + 5 * ld] - f[i - 5 * ld]) + C6 * (f[i + 6 * ld] - f[i - 6 * ld])) *
+             invdx;
+        gz = (C1 * (f[i + 1] - f[i - 1]) + C2 * (f[i + 2] - f[i - 2]) + C3 * (f[i + 3] - f[i - 3]) +
+              C4 * (f[i + 4] - f[i - 4]) + C5 * (f[i + 5] - f[i - 5]) + C6 * (f[i + 6] - f[i - 6])) *
+             invdz;
+    } else {
+        if (ix == 0)
+            gx = (f[i + ld] - f[i]) / dx;
+        else if (ix == Nx - 1)
+            gx = (f[i] - f[i - ld]) / dx;
+        else
+            gx = (f[i + ld] - f[i - ld]) / (2.0f * dx);
+        if (iz == 0)
+            gz = (f[i + 1] - f[i]) / dz;
+        else if (iz == Nz - 1)
+            gz = (f[i] - f[i - 1]) / dz;
+        else
+            gz = (f[i + 1] - f[i - 1]) / (2.0f * dz);
+    }
+    coef[CO_IRX * fsz + i] = gx;
+    coef[CO_IRZ * fsz + i] = gz;
+}
+
+void launch_grad_invrho(float *coef, const Geom &g, hipStream_t st)
+{
+    grad_invrho_kernel<<<grid2(g, g.Nz), 64, 0, st>>>(coef, g.Nx, g.Nz, g.ld, g.fsz, g.dx, g.dz);
+}
+
+// ---------------------------------------------------------------------------
+// Receivers (propagation.c:567-576 record p1; :676-682 adjoint injection)
+
+__global__ void record_kernel(RcvTable t, const float *f0, long stride, float *rec, int it)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= t.n) return;
+    const int sr = t.sr[e];
+    if (it % sr) return;
+    rec[t.trace_off[e] + it / sr] = f0[(long)t.shot[e] * stride + t.cell[e]];
+}
+
+void launch_record(const RcvTable &t, const float *field0, long shot_stride, float *rec, int it,
+                   hipStream_t st)
+{
+    if (t.n <= 0) return;
+    record_kernel<<<(t.n + 255) / 256, 256, 0, st>>>(t, field0, shot_stride, rec, it);
+}
+
+__global__ void inject_kernel(InjTable t, float *f0, long stride, const float *K, const float *adj,
+                              int it, float dt, float invdx, float invdz)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= t.n) return;
+    const int cell = t.cell[u];
+    float *p = f0 + (long)t.shot[u] * stride + cell;
+    float v = *p;
+    const float k = K[cell];
+    for (int e = t.beg[u]; e < t.beg[u + 1]; ++e)
+        v -= adj[t.trace_off[e] + it] * k * dt * dt * invdx * invdz;
+    *p = v;
+}
+
+void launch_inject(const InjTable &t, float *field0, long shot_stride, const float *K,
+                   const float *adj, int it, float dt, float invdx, float invdz, hipStream_t st)
+{
+    if (t.n <= 0) return;
+    inject_kernel<<<(t.n + 255) / 256, 256, 0, st>>>(t, field0, shot_stride, K, adj, it, dt, invdx,
+                                                      invdz);
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint source (optimization.c:335-345, 417-418; taper_traces common.c:796-895)
+
+__global__ void adjsrc_kernel(const float *pre, const float *obs, float *out, int ntr, int ns,
+                              const float *cT, int Lt, const float *cX, int Lx)
+{
+    const long n = (long)ntr * ns;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int itr = (int)(e / ns), it = (int)(e - (long)itr * ns);
+    float r = pre[e] - obs[e];
+    if (Lt >= 0) {  // Lt < 0: no time taper
+        if (it <= Lt) r *= cT[Lt - it];
+        if (it >= ns - Lt - 1) r *= cT[it - (ns - Lt - 1)];
+    }
+    if (Lx >= 0) {
+        if (itr <= Lx) r *= cX[Lx - itr];
+        if (itr >= ntr - Lx - 1) r *= cX[itr - (ntr - Lx - 1)];
+    }
+    out[e] = r;
+}
+
+void launch_adjsrc(const float *pre, const float *obs, float *out, int ntr, int ns,
+                   const float *coefT, int Lt, const float *coefX, int Lx, hipStream_t st)
+{
+    const long n = (long)ntr * ns;
+    if (n <= 0) return;
+    adjsrc_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(pre, obs, out, ntr, ns, coefT, Lt,
+                                                                coefX, Lx);
+}
+
+// ---------------------------------------------------------------------------
+// Gradient finalisation and preconditioning (propagation.c:610-623, 732-743,
+// 953-993; optimization.c:13-97)
+
+__global__ void finalize_kernel(const double *acc, long acc_stride, float *out, long out_stride,
+                                const float *vpw, const float *rhow, int n, double scale,
+                                int squared)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, s = blockIdx.y;
+    if (i >= n) return;
+    const float rv = rhow[i] * vpw[i];
+    const double den = squared ? (double)(rv * rv) : (double)rv;
+    out[(long)s * out_stride + i] = (float)(scale * acc[(long)s * acc_stride + i] / den);
+}
+
+void launch_finalize(const double *acc, long acc_stride, float *out, long out_stride, int nshot,
+                     const float *vpw, const float *rhow, long, int n, double scale, int squared,
+                     hipStream_t st)
+{
+    dim3 grid((n + 255) / 256, nshot);
+    finalize_kernel<<<grid, 256, 0, st>>>(acc, acc_stride, out, out_stride, vpw, rhow, n, scale,
+                                          squared);
+}
+
+// Deterministic reductions: a fixed grid of RB blocks, each thread strides
+// through the input in a fixed order, then a fixed-shape tree in LDS, then one
+// block folds the RB partials.  The result depends only on n.
+constexpr int RB = 512;
+constexpr int RT = 256;
+size_t reduce_scratch_doubles() { return (size_t)4 * RB + 64; }
+
+template <int OP>
+__device__ inline void term(const float *a, const float *b, long i, double &s0, double &s1)
+{
+    if (OP == 0) {  // sum of float squares
+        const float v = a[i];
+        s0 += (double)(v * v);
+    } else if (OP == 1) {  // sum
+        s0 += (double)a[i];
+    } else if (OP == 2) {  // step length: r0*(r0-r1), (r0-r1)^2
+        const float r0 = a[i];
+        const float d = r0 - b[i];
+        s0 += (double)(r0 * d);
+        s1 += (double)(d * d);
+    } else {  // sum of double squares
+        s0 += (double)a[i] * (double)a[i];
+    }
+}
+
+__device__ inline void block_sum2(double &s0, double &s1, double *sh)
+{
+    const int t = threadIdx.x;
+    sh[t] = s0;
+    sh[RT + t] = s1;
+    __syncthreads();
+    for (int k = RT / 2; k > 0; k >>= 1) {
+        if (t < k) {
+            sh[t] += sh[t + k];
+            sh[RT + t] += sh[RT + t + k];
+        }
+        __syncthreads();
+    }
+    s0 = sh[0];
+    s1 = sh[RT];
+}
+
+template <int OP>
+__global__ __launch_bounds__(RT) void reduce_stage1(const float *a, const float *b, long n,
+                                                    double *part)
+{
+    __shared__ double sh[2 * RT];
+    double s0 = 0.0, s1 = 0.0;
+    for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)RB * RT)
+        term<OP>(a, b, i, s0, s1);
+    block_sum2(s0, s1, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = s0;
+        part[RB + blockIdx.x] = s1;
+    }
+}
+
+__global__ __launch_bounds__(RT) void reduce_stage2(const double *part, double *res)
+{
+    __shared__ double sh[2 * RT];
+    double s0 = 0.0, s1 = 0.0;
+    for (int i = threadIdx.x; i < RB; i += RT) {
+        s0 += part[i];
+        s1 += part[RB + i];
+    }
+    block_sum2(s0, s1, sh);
+    if (threadIdx.x == 0) {
+        res[0] = s0;
+        res[1] = s1;
+    }
+}
+
+template <int OP>
+static void reduce(const float *a, const float *b, long n, double *scratch, double *res,
+                   hipStream_t st)
+{
+    reduce_stage1<OP><<<RB, RT, 0, st>>>(a, b, n, scratch);
+    reduce_stage2<<<1, RT, 0, st>>>(scratch, res);
+}
+
+void launch_sum_sq(const float *x, long n, double *scratch, double *res, hipStream_t st)
+{
+    reduce<0>(x, nullptr, n, scratch, res, st);
+}
+void launch_sum(const float *x, long n, double *scratch, double *res, hipStream_t st)
+{
+    reduce<1>(x, nullptr, n, scratch, res, st);
+}
+void launch_sum_step(const float *a, const float *b, long n, double *scratch, double *res,
+                     hipStream_t st)
+{
+    reduce<2>(a, b, n, scratch, res, st);
+}
+
+// per-shot sums: grid (RB, nshot) -> part[s][RB] -> sums[s]
+__global__ __launch_bounds__(RT) void shot_sum_stage1(const float *p, long stride, int n,
+                                                      double *part)
+{
+    __shared__ double sh[2 * RT];
+    const int s = blockIdx.y;
+    double s0 = 0.0, s1 = 0.0;
+    for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)RB * RT)
+        s0 += (double)p[(long)s * stride + i];
+    block_sum2(s0, s1, sh);
+    if (threadIdx.x == 0) part[(long)s * RB + blockIdx.x] = s0;
+}
+
+__global__ __launch_bounds__(RT) void shot_sum_stage2(const double *part, double *sums)
+{
+    __shared__ double sh[2 * RT];
+    const int s = blockIdx.x;
+    double s0 = 0.0, s1 = 0.0;
+    for (int i = threadIdx.x; i < RB; i += RT) s0 += part[(long)s * RB + i];
+    block_sum2(s0, s1, sh);
+    if (threadIdx.x == 0) sums[s] = s0;
+}
+
+__global__ void shot_precond_apply(float *g, float *p, long stride, int n, const double *sums)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, s = blockIdx.y;
+    if (i >= n) return;
+    const float avg = (float)(sums[s] / n);
+    const long k = (long)s * stride + i;
+    const float f = 1.0f + p[k] / (7.0f * avg);
+    g[k] = g[k] / f;
+    p[k] = p[k] / f;
+}
+
+void launch_shot_precondition(float *g, float *p, long stride, int nshot, int n, double *scratch,
+                              hipStream_t st)
+{
+    // scratch holds nshot*RB partials followed by nshot sums
+    double *part = scratch, *sums = scratch + (long)nshot * RB;
+    shot_sum_stage1<<<dim3(RB, nshot), RT, 0, st>>>(p, stride, n, part);
+    shot_sum_stage2<<<nshot, RT, 0, st>>>(part, sums);
+    shot_precond_apply<<<dim3((n + 255) / 256, nshot), 256, 0, st>>>(g, p, stride, n, sums);
+}
+
+__global__ void accumulate_kernel(const float *v, long stride, int nshot, const int *ix0,
+                                  const int *nxx, int nz, int nx, double *sum)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)nx * nz) return;
+    const int ix = (int)(e / nz), iz = (int)(e - (long)ix * nz);
+    double acc = sum[e];
+    for (int s = 0; s < nshot; ++s) {
+        const int lx = ix - ix0[s];
+        if (lx >= 0 && lx < nxx[s]) acc += v[(long)s * stride + (long)lx * nz + iz];
+    }
+    sum[e] = acc;
+}
+
+void launch_accumulate(const float *v, long stride, int nshot, const int *ix0_dev,
+                       const int *nxx_dev, int nz, int nx, double *sum, hipStream_t st)
+{
+    const long n = (long)nx * nz;
+    accumulate_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(v, stride, nshot, ix0_dev,
+                                                                   nxx_dev, nz, nx, sum);
+}
+
+__global__ void to_float_kernel(const double *in, float *out, long n)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) out[e] = (float)in[e];
+}
+
+void launch_to_float(const double *in, float *out, long n, hipStream_t st)
+{
+    to_float_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(in, out, n);
+}
+
+__global__ void global_precond_apply(float *g, const float *p, long n, float alpha,
+                                     const double *res)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const float rms = (float)sqrt(res[0] / (double)n);
+    g[e] = g[e] / (p[e] + alpha * rms);
+}
+
+void launch_global_precondition(float *g, const float *p, long n, float alpha, double *scratch,
+                                hipStream_t st)
+{
+    if (alpha <= 0.0f) alpha = 0.005f;
+    double *res = scratch + 2 * RB;
+    reduce<3>(p, nullptr, n, scratch, res, st);
+    global_precond_apply<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(g, p, n, alpha, res);
+}
+
+// min / max (exact in any order)
+__global__ __launch_bounds__(RT) void minmax_stage1(const float *x, long n, float *part)
+{
+    __shared__ float mn[RT], mx[RT];
+    float a = 0.0f, b = 0.0f;  // optimization.c:203-204 start both at 0
+    for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)RB * RT) {
+        a = fminf(a, x[i]);
+        b = fmaxf(b, x[i]);
+    }
+    mn[threadIdx.x] = a;
+    mx[threadIdx.x] = b;
+    __syncthreads();
+    for (int k = RT / 2; k > 0; k >>= 1) {
+        if (threadIdx.x < k) {
+            mn[threadIdx.x] = fminf(mn[threadIdx.x], mn[threadIdx.x + k]);
+            mx[threadIdx.x] = fmaxf(mx[threadIdx.x], mx[threadIdx.x + k]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = mn[0];
+        part[RB + blockIdx.x] = mx[0];
+    }
+}
+
+__global__ void minmax_stage2(const float *part, float *res)
+{
+    if (threadIdx.x) return;
+    float a = 0.0f, b = 0.0f;
+    for (int i = 0; i < RB; ++i) {
+        a = fminf(a, part[i]);
+        b = fmaxf(b, part[RB + i]);
+    }
+    res[0] = a;
+    res[1] = b;
+}
+
+void launch_minmax(const float *x, long n, float *scratch, float *res, hipStream_t st)
+{
+    minmax_stage1<<<RB, RT, 0, st>>>(x, n, scratch);
+    minmax_stage2<<<1, 64, 0, st>>>(scratch, res);
+}
+
+// histogram of optimization.c:229-233 (integer counts: order-free)
+__global__ void histogram_kernel(const float *g, long n, float mn, float interval, int *hist)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const double x = (double)((g[e] - mn) / interval) + 0.5;
+    int pos = (x >= 0.0) ? (int)x : 0;  // NaN (all-zero gradient) lands in bin 0
+    if (pos > 1000) pos = 1000;
+    atomicAdd(&hist[pos], 1);
+}
+
+void launch_histogram(const float *g, long n, float mn, float interval, int *hist, hipStream_t st)
+{
+    histogram_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(g, n, mn, interval, hist);
+}
+
+__global__ void scale_kernel(const float *in, float *out, long n, float s)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) out[e] = -s * in[e];
+}
+
+void launch_scale(const float *in, float *out, long n, float s, hipStream_t st)
+{
+    scale_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(in, out, n, s);
+}
+
+// ---------------------------------------------------------------------------
+__global__ void freesurface_kernel(float *p, int Nx, int ld, int pml)
+{
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ix >= Nx) return;
+    float *c = p + (long)ix * ld;
+    c[pml] = 0.0f;
+    for (int iz = 0; iz <= pml; ++iz) c[pml - iz] = -c[pml + iz];
+}
+
+void launch_freesurface(float *p, int Nx, int ld, int pml, hipStream_t st)
+{
+    freesurface_kernel<<<(Nx + 63) / 64, 64, 0, st>>>(p, Nx, ld, pml);
+}
+
+}  // namespace fwiedu

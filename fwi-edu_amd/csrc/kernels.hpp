@@ -1,0 +1,124 @@
+// Device kernels of the FWI hot path and their host-side launchers.
+//
+// Field layout in HBM: every padded field is [Nx][ld] floats, z fastest, with
+// ld = Nz rounded up to a multiple of 64 so each column starts on a 256-byte
+// boundary and a wave's 64 lanes read one column segment in one coalesced
+// request.  A batch of S shots keeps one arena per shot; the same buffer role of
+// shot s sits at  base + s * shot_stride.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fwiedu {
+
+enum FdMode { FD_FWD = 0, FD_RECON = 1, FD_ADJ = 2 };
+
+// Geometry of one padded simulation window.
+struct Geom {
+    int nx, nz;          // physical window (nz x nx)
+    int pml, fs;         // absorbing-layer width, free surface flag
+    int Nx, Nz, ld;      // padded extents and leading dimension
+    int ixs, ixe, izs, ize;  // stencil bounds (propagation.c:534-537)
+    float dx, dz, dt;
+    float invdx, invdz, invdxdx, invdzdz, invdtdt;
+    long fsz;            // floats per field = Nx * ld
+};
+
+// The seven coefficient arrays, consecutive blocks of fsz floats.
+enum { CO_C1 = 0, CO_C2, CO_C3, CO_K, CO_IR, CO_IRX, CO_IRZ, CO_N };
+
+struct FdArgs {
+    const float *coef;      // CO_N * fsz
+    float *out;             // shot 0 pointers
+    const float *cur;
+    const float *prev;
+    long shot_stride;       // floats between shot arenas
+    int nshot;
+    int Nx, ld, ixs, ixe, izs, ize, pml, fs, nx, nz;
+    float invdx, invdz, invdxdx, invdzdz, invdtdt, dt;
+    int xtiles, ztiles;
+    // source (FD_FWD / FD_RECON)
+    const int *src_cell;    // [nshot] flat padded index or -1
+    const float *src_wav;   // [nshot][nt]
+    int nt, it;
+    // pseudo-Hessian (FD_FWD) / imaging (FD_ADJ) every MST steps
+    double *acc;            // shot 0, [nx][nz]
+    long acc_stride;
+    const float *u;         // FD_ADJ: forward field at `it`, shot 0
+    int acc_on;
+};
+
+void launch_fd(int mode, const FdArgs &a, hipStream_t st);
+
+// ---- set-up --------------------------------------------------------------
+// window model [nx][nz] (contiguous) -> padded [Nx][ld] by edge replication
+void launch_pad(const float *win, int nx, int nz, int pml, float *out, const Geom &g,
+                hipStream_t st);
+// MEAL D / Beta with host-computed scalars (layer weights w[1..n], ln R, sqrt 3)
+void launch_meal(const float *vp, const float *vs, float *D, float *Beta, const Geom &g,
+                 int nlayer, const float *w_dev, float lnR, float pd, float sqrt3, float fpeak,
+                 hipStream_t st);
+// D/=Beta, vp/=Beta, rho*=Beta, coef1..3 (double from FP32 products), K, 1/rho
+void launch_coefs(float *vp, float *rho, float *D, const float *Beta, float *coef, const Geom &g,
+                  hipStream_t st);
+// d(1/rho)/dx, d(1/rho)/dz: 2nd-order everywhere, 12-point optimised inside
+void launch_grad_invrho(float *coef, const Geom &g, hipStream_t st);
+
+// ---- receivers -----------------------------------------------------------
+struct RcvTable {
+    int n;                  // entries
+    const int *shot;        // [n]
+    const int *cell;        // [n] flat padded index
+    const long *trace_off;  // [n] offset of this trace in the record buffer
+    const int *sr;          // [n] sample rate of the entry's shot
+};
+void launch_record(const RcvTable &t, const float *field0, long shot_stride, float *rec, int it,
+                   hipStream_t st);
+struct InjTable {
+    int n;                  // unique (shot, cell) pairs
+    const int *shot, *cell; // [n]
+    const int *beg;         // [n+1] into trace_off (receiver order kept)
+    const long *trace_off;  // offset of the trace's sample 0 in the adjoint-source buffer
+};
+void launch_inject(const InjTable &t, float *field0, long shot_stride, const float *K,
+                   const float *adj, int it, float dt, float invdx, float invdz, hipStream_t st);
+
+// ---- adjoint source ------------------------------------------------------
+// out = taper(pre - obs) for one shot [ntr][ns]; the cosine tables are built
+// on the host (coefT[0..Lt], coefX[0..Lx]), Lt/Lx < 0 means no taper.
+void launch_adjsrc(const float *pre, const float *obs, float *out, int ntr, int ns,
+                   const float *coefT, int Lt, const float *coefX, int Lx, hipStream_t st);
+
+// ---- gradient finalisation / preconditioning ------------------------------
+// g[i] = (float)(scale * acc[i] / (double)(rho*vp) or /(double)((rho*vp)^2))
+void launch_finalize(const double *acc, long acc_stride, float *out, long out_stride, int nshot,
+                     const float *vpw, const float *rhow, long win_stride, int n, double scale,
+                     int squared, hipStream_t st);
+// per-shot sum of p (double) -> avg -> g /= 1 + p/(7 avg), p /= same
+void launch_shot_precondition(float *g, float *p, long stride, int nshot, int n, double *scratch,
+                              hipStream_t st);
+// sum[ix0+i] += v[s][i] over shots in order (window offsets per shot)
+void launch_accumulate(const float *v, long stride, int nshot, const int *ix0_dev,
+                       const int *nxx_dev, int nz, int nx, double *sum, hipStream_t st);
+void launch_to_float(const double *in, float *out, long n, hipStream_t st);
+// grad = grad / (p + alpha * rms(p)), rms from a deterministic double reduction
+void launch_global_precondition(float *g, const float *p, long n, float alpha, double *scratch,
+                                hipStream_t st);
+
+// ---- reductions ----------------------------------------------------------
+// deterministic double sums; results land in res[0..k) on the device
+void launch_sum_sq(const float *x, long n, double *scratch, double *res, hipStream_t st);
+void launch_sum_step(const float *a, const float *b, long n, double *scratch, double *res,
+                     hipStream_t st);
+void launch_sum(const float *x, long n, double *scratch, double *res, hipStream_t st);
+void launch_minmax(const float *x, long n, float *scratch, float *res, hipStream_t st);
+void launch_histogram(const float *g, long n, float mn, float interval, int *hist,
+                      hipStream_t st);
+void launch_scale(const float *in, float *out, long n, float s, hipStream_t st);
+size_t reduce_scratch_doubles();
+
+// ---- small helpers -------------------------------------------------------
+void launch_freesurface(float *p, int Nx, int ld, int pml, hipStream_t st);
+
+}  // namespace fwiedu

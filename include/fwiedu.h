@@ -1,0 +1,180 @@
+/*
+ * fwiedu.h -- C ABI of the MI355X FWI hot-path library (libfwiedu.so).
+ *
+ * The first part declares, with the same names, argument order and meaning,
+ * the entry points of FWILab2d C v1.2.6 (FWILab2d/FWILab2dCLangV1.2.6/include/)
+ * that the reference's drivers call on the 2D acoustic FWI path.  A driver
+ * written against the reference headers links against this library instead;
+ * the work runs on the GPU (HIP, gfx950) and the results are written back into
+ * the caller's host arrays exactly where the reference writes them.
+ *
+ * 2D arrays are Seismic Unix alloc2float(n1, n2) blocks: a row-pointer table
+ * over ONE contiguous block, n1 (z) fastest.  As in the reference, the library
+ * reads and writes through a[0] and assumes that contiguity.
+ *
+ * Errors follow the reference's err() convention: a message on stderr and
+ * exit(1).  HIP failures print the HIP error string.
+ *
+ * The second part (fwiedu_*) has no counterpart in the reference: it selects
+ * batching, checkpoint precision and timing, and exposes device-level
+ * measurements.
+ */
+#pragma once
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- types ------------------------------------------------------------- */
+
+/* Record2D: same fields, order and types as observation.h:18-48. */
+typedef struct
+{
+    int shotID;
+    float sx;
+    float sz;
+    int isx;
+    int isz;
+    float fpeak;
+    float *src;
+    float *gx;
+    float *gz;
+    int *igx;
+    int *igz;
+    int sampleRate;
+    int ntr;
+    int ns;
+    float dtr;
+    float dt;
+    int ix_min;
+    int ix_max;
+    float **traces_vx;
+    float **traces_vz;
+    float **traces_p;
+    float **precond;
+    float **grad;
+} Record2D;
+
+/* Acoustic2dCheckPoint: same fields as checkpoint.h:16-26 (host storage). */
+typedef struct
+{
+    int nt;
+    int dCheckPoint;
+    int nCheckPoint;
+    int ntPoint;
+    int *status;
+    float memCost;
+    float ***p;
+    float ***storage_p;
+} Acoustic2dCheckPoint;
+
+#define MST 5 /* propagation.h:21 */
+
+/* ---- propagation.h ----------------------------------------------------- */
+
+/* Replaces iso_acoustic2d_propagation, propagation.h:133-141 / propagation.c:777-1036.
+ * problem 0: model records into record2d_pre[].traces_p.
+ * problem 1: gradient; record2d_pre[].traces_p holds the adjoint source on return
+ *            and grad[nx][nz] the (preconditioned when precondition==1) gradient.
+ * problem 2: model + adjoint source (line-search trial) into record2d_pre[].traces_p.
+ * Shots that share a simulation window and fpeak advance together on the GPU. */
+void iso_acoustic2d_propagation(float **vp, float **rho, Record2D *record2d_obs, int Nshot, int nz,
+                                int nx, float dx, float dz, float dt, int nt, int pmlThick,
+                                int freesurface, float freq, float filter, int precondition,
+                                int problem, int iter, int verbose, Record2D *record2d_pre,
+                                float **grad);
+
+/* Replaces iso_acoustic2d_propagation_engine, propagation.h:95-100 / propagation.c:356-775.
+ * Note the (nx, nz) order.  mode 0 forward; mode 1 forward that fills the host
+ * checkpoint chkpt (pairs at multiples of dCheckPoint, status set) and, when
+ * precondition==1, the pseudo-Hessian into grad[nx][nz]; mode 2 adjoint that
+ * reads the checkpoint pairs from chkpt and writes the gradient to grad. */
+void iso_acoustic2d_propagation_engine(Record2D *record2d, Acoustic2dCheckPoint *chkpt,
+                                       float **vpm, float **rhom, int nx, int nz, float dx,
+                                       float dz, float dt, int nt, int pmlThick, int freesurface,
+                                       int precondition, int mode, int iter, int verbose,
+                                       float **grad);
+
+/* Replaces acoustic_iso2d_fd_it_6order, propagation.h:23-28: one sweep of the
+ * stencil over ix in [ixStart, ixEnd], iz in [izStart, izEnd] of host row
+ * tables, run on the GPU.  Only the cells the sweep reads (the box widened by
+ * 3 in each direction for p1) and writes are touched, so any row table works. */
+void acoustic_iso2d_fd_it_6order(float **p2, float **p1, float **p0, float **invrho,
+                                 float **invrhox, float **invrhoz, float **K, float **coef1,
+                                 float **coef2, float **coef3, float invdx, float invdz,
+                                 float invdxdx, float invdzdz, int ixStart, int ixEnd, int izStart,
+                                 int izEnd);
+
+/* Replaces acoustic_iso2d_freesurface, propagation.h:58. */
+void acoustic_iso2d_freesurface(float **p2, int Nx, int Nz, int pmlThick);
+
+/* Replaces mealCoefs2d, propagation.h:72-74 (vs == NULL or vs > 0 per cell). */
+void mealCoefs2d(float **D, float **Beta, float **vp, float **vs, float fpeak, float eal_alpha,
+                 float dz, float dx, int NZ, int NX, int nlayer);
+
+/* ---- checkpoint.h (host-side bookkeeping, same semantics) --------------- */
+void acoustic2dCheckPoint_init(Acoustic2dCheckPoint *chkpt);                  /* checkpoint.h:29 */
+void alloc_acoustic2dCheckPoint(Acoustic2dCheckPoint *chkpt, int NZ, int NX, int nt); /* :41 */
+void free_acoustic2dCheckPoint(Acoustic2dCheckPoint *chkpt);                  /* :58 */
+void save_acoustic2dCheckPoint(Acoustic2dCheckPoint *chkpt, float **p0, float **p1, int NP,
+                               int it);                                        /* :71 */
+void get_acoustic2dCheckPoint(Acoustic2dCheckPoint *chkpt, float **p0, float **p1, int NP,
+                              int it);                                         /* :88 */
+void get_acoustic2dWavefield(Acoustic2dCheckPoint *chkpt, float **p, int NP, int it); /* :104 */
+void save_acoustic2dWavefield(Acoustic2dCheckPoint *chkpt, float **p, int NP, int it); /* :120 */
+int get_acoustic2dCheckPointStatus(Acoustic2dCheckPoint *chkpt, int it);     /* :136 */
+float calculate_memory_cost_checkpoint(int nt, int NX, int NZ);              /* :152 */
+
+/* ---- optimization.h (GPU reductions / elementwise) ---------------------- */
+
+/* optimization.h:17-18.  filter 2 (none) is supported; 0/1 (FFT band filters)
+ * end with an error. */
+void compute_adjoint_source2d(Record2D *record2d_pre, Record2D *record2d_obs, float **vp,
+                              float freq, int filter, Record2D *record2d_adjsource);
+void apply_gradient_precondition(float *grad, const float *precond, int n, float alpha); /* :40 */
+void apply_gradient_shot_precondition(float *grad, float *precond, int n);             /* :60 */
+float calculate_objective_value(Record2D *record2d, int Nshot);                       /* :79 */
+float compute_model_perturbation(float fraction, float *slowness, float *gradient, float *out,
+                                 int len);                                             /* :93 */
+float calculate_step_length(Record2D *record2d_pre0, Record2D *record2d_pre1, int Nshot); /* :112 */
+
+/* ---- fwiedu_*: no counterpart in the reference -------------------------- */
+
+/* Shots advanced per launch by iso_acoustic2d_propagation (0 = as many as fit
+ * in the device-memory budget; 1 = one shot per launch, the plain path). */
+void fwiedu_set_batch(int shots_per_launch);
+/* Fraction of free device memory the engine may use (default 0.85). */
+void fwiedu_set_memory_fraction(double frac);
+/* Sample every `every`-th reconstruction-sweep launch (the stencil with no
+ * accumulation epilogue) with HIP events on the library's stream (0 = off). */
+void fwiedu_set_kernel_timing(int every);
+/* Checkpoint storage precision: 32 (default) or 16 (FP16 with a per-snapshot scale). */
+void fwiedu_set_checkpoint_bits(int bits);
+/* Named counters of the last calls: "fd_launches", "fd_timed", "fd_ms_avg",
+ * "fd_cells_per_launch", "fd_shots_per_launch", "batch", "device_bytes" ...
+ * Unknown names return -1. */
+double fwiedu_stat(const char *name);
+void fwiedu_reset_stats(void);
+/* Device-resident gradient session: the model, the observed records and all
+ * wavefield state stay in HBM between calls, so repeated gradients (FWI
+ * iterations, serving) move no traces over PCIe.  Same arguments and meaning as
+ * iso_acoustic2d_propagation with problem 1; the records are copied at create. */
+void *fwiedu_session_create(float **vp, float **rho, Record2D *record2d_obs, int Nshot, int nz,
+                            int nx, float dx, float dz, float dt, int nt, int pmlThick,
+                            int freesurface, float freq, float filter, int precondition);
+/* Replace the model (host arrays, copied). */
+void fwiedu_session_set_model(void *session, float **vp, float **rho);
+/* One gradient; returns 0.5 * sum of the squared adjoint sources (the misfit of
+ * the processed residual).  grad may be NULL (the gradient stays on the device). */
+float fwiedu_session_gradient(void *session, float **grad);
+void fwiedu_session_destroy(void *session);
+/* Free all device memory held by the library. */
+void fwiedu_release(void);
+/* 1 if a HIP device is present and usable. */
+int fwiedu_device_ok(void);
+/* Library version string. */
+const char *fwiedu_version(void);
+
+#ifdef __cplusplus
+}
+#endif

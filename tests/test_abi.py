@@ -1,0 +1,102 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol the header
+declares, its host-side checkpoint bookkeeping matches the oracle's schedule,
+and bad arguments end the process the way the reference's err() does -- all
+without a GPU."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from fwiedu import lib as flib
+from fwiedu.records import Array2D
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "fwiedu.h")
+
+
+def header_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?(?:void|int|float|double|char)\s*\**\s*(\w+)\s*\(", txt,
+                       flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_header_symbol():
+    L = flib.load()
+    names = header_functions()
+    assert len(names) >= 29
+    assert sorted(names) == sorted(flib.EXPORTED)
+    for n in names:
+        assert hasattr(L, n), n
+    assert b"gfx950" in L.fwiedu_version()
+
+
+def test_host_checkpoint_bookkeeping(oracles):
+    """alloc_acoustic2dCheckPoint & co. (checkpoint.c:29-299) against the oracle's plan."""
+    L = flib.load()
+    for nt in (7, 240, 1500, 4001):
+        ck = flib.Acoustic2dCheckPoint()
+        L.acoustic2dCheckPoint_init(ctypes.byref(ck))
+        NZ, NX = 9, 5
+        L.alloc_acoustic2dCheckPoint(ctypes.byref(ck), NZ, NX, nt)
+        d, nc, ntp, _ = oracles["f32"].checkpoint_plan(nt)
+        assert (ck.dCheckPoint, ck.nCheckPoint, ck.ntPoint) == (d, nc, ntp)
+        assert ck.memCost == pytest.approx(oracles["f32"].memory_cost_checkpoint(nt, NX, NZ), rel=1e-6)
+        assert L.calculate_memory_cost_checkpoint(nt, NX, NZ) == pytest.approx(ck.memCost, rel=1e-7)
+        # save a pair at it = d and read it back; status drives the recompute search
+        Np = NZ * NX
+        p0 = Array2D(np.random.default_rng(0).standard_normal((NX, NZ)).astype(np.float32))
+        p1 = Array2D(np.random.default_rng(1).standard_normal((NX, NZ)).astype(np.float32))
+        if d < nt:
+            L.save_acoustic2dCheckPoint(ctypes.byref(ck), p0.pp, p1.pp, Np, d)
+            q0 = Array2D(np.zeros((NX, NZ), np.float32))
+            q1 = Array2D(np.zeros((NX, NZ), np.float32))
+            L.get_acoustic2dCheckPoint(ctypes.byref(ck), q0.pp, q1.pp, Np, d)
+            assert np.array_equal(q0.a, p0.a) and np.array_equal(q1.a, p1.a)
+            assert L.get_acoustic2dCheckPointStatus(ctypes.byref(ck), d) == -1
+            assert L.get_acoustic2dCheckPointStatus(ctypes.byref(ck), d - 1) == -1
+            if d + 1 < nt:
+                assert L.get_acoustic2dCheckPointStatus(ctypes.byref(ck), d + 1) == d
+        L.free_acoustic2dCheckPoint(ctypes.byref(ck))
+        assert not ck.p and not ck.status
+
+
+def _run_child(code):
+    env = dict(os.environ)
+    env["PYTHONPATH"] = os.pathsep.join([os.path.join(ROOT, "fwi-edu_amd"), ROOT])
+    return subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                          timeout=120)
+
+
+BAD = """
+import numpy as np
+from fwiedu import lib as flib, synth
+from fwiedu.records import Array2D
+L = flib.load()
+w = synth.workload('tiny')
+vp, rho = Array2D(w.vp_true), Array2D(w.rho_true)
+obs = w.shots()
+L.iso_acoustic2d_propagation(vp.pp, rho.pp, obs.recs, w.nshot, w.nz, w.nx, w.dx, w.dz, w.dt,
+                             w.nt, {pml}, {fs}, 0.0, 2.0, 1, {problem}, 1, 0, obs.recs, None)
+print('returned')
+"""
+
+
+@pytest.mark.parametrize("pml,fs,problem,msg", [
+    (2, 0, 0, "pmlThick must be at least 3"),
+    (12, 3, 0, "freesurface must be 0 or 1"),
+    (12, 0, 5, "problem must be 0, 1 or 2"),
+    (12, 0, 1, "problem 1 needs a gradient array"),
+])
+def test_invalid_arguments_exit_like_err(pml, fs, problem, msg):
+    """Argument errors are checked before any device call and exit(1) with a
+    message, like the reference's SU err()."""
+    r = _run_child(BAD.format(pml=pml, fs=fs, problem=problem))
+    assert r.returncode == 1, (r.stdout, r.stderr)
+    assert msg in r.stderr
+    assert "returned" not in r.stdout

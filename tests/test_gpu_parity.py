@@ -1,0 +1,296 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle on the
+same seeded inputs.
+
+Tolerances (relative L2 unless noted):
+  vs the strict-order FP32 oracle (f32s, same operation order as the kernels):
+      records, adjoint sources, gradients <= 1e-6 (bit-exact expected)
+  vs the reference-flags FP32 oracle (f32, -O3 -ffast-math):
+      records 1e-5, gradients 1e-4, misfit and step length 1e-5
+  vs the FP64 oracle: stated, and bounded by 1e-4 (records) / 1e-3 (gradients)
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from fwiedu import lib as flib
+from fwiedu import synth
+from fwiedu.records import Array2D, fptr
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    n = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (n if n > 0 else 1.0))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def L():
+    lib = flib.load()
+    assert lib.fwiedu_device_ok() == 1, "no HIP device"
+    return lib
+
+
+def observed(o_or_lib, w, is_lib):
+    obs = w.shots()
+    vp, rho = Array2D(w.vp_true), Array2D(w.rho_true)
+    if is_lib:
+        flib.propagate(o_or_lib, w, vp, rho, obs, 0)
+    else:
+        o_or_lib.propagation(vp.pp, rho.pp, obs.recs, w.nshot, w.nz, w.nx, w.dx, w.dz, w.dt, w.nt,
+                             w.pml, w.freesurface, w.freq, float(w.filter), w.precondition, 0,
+                             obs.recs, None)
+    return obs
+
+
+def gradient(o_or_lib, w, obs, is_lib, precondition=None):
+    pc = w.precondition if precondition is None else precondition
+    vp, rho = Array2D(w.vp_start), Array2D(w.rho_start)
+    pre = obs.header_copy()
+    g = Array2D(np.zeros((w.nx, w.nz), np.float32))
+    if is_lib:
+        flib.propagate(o_or_lib, w, vp, rho, obs, 1, pre, g, precondition=pc)
+    else:
+        o_or_lib.propagation(vp.pp, rho.pp, obs.recs, w.nshot, w.nz, w.nx, w.dx, w.dz, w.dt, w.nt,
+                             w.pml, w.freesurface, w.freq, float(w.filter), pc, 1, pre.recs, g.pp)
+    return pre, g.a.copy()
+
+
+def recs(shotset):
+    return np.concatenate([t.ravel() for t in shotset.all_traces()])
+
+
+# --------------------------------------------------------------------------- pieces
+def test_fd_sweep_bit_exact(L, oracles):
+    say("case: acoustic_iso2d_fd_it_6order 40x70 random")
+    rng = np.random.default_rng(11)
+    Nx, Nz = 40, 70
+    arrs = [rng.standard_normal((Nx, Nz)).astype(np.float32) for _ in range(9)]
+    for a in arrs[2:]:
+        a *= 0.1
+    p1, p0, ir, irx, irz, K, c1, c2, c3 = arrs
+    out_g = Array2D(np.full((Nx, Nz), 7.0, np.float32))
+    tabs = [Array2D(a) for a in arrs]
+    h = (1 / 12.5, 1 / 10.0, 1 / 156.25, 1 / 100.0)
+    L.acoustic_iso2d_fd_it_6order(out_g.pp, tabs[0].pp, tabs[1].pp, tabs[2].pp, tabs[3].pp,
+                                  tabs[4].pp, tabs[5].pp, tabs[6].pp, tabs[7].pp, tabs[8].pp,
+                                  *h, 3, Nx - 4, 5, Nz - 4)
+    out_o = np.full((Nx, Nz), 7.0, np.float32)
+    oracles["f32s"].fd_step(fptr(out_o), fptr(p1), fptr(p0), fptr(ir), fptr(irx), fptr(irz),
+                            fptr(K), fptr(c1), fptr(c2), fptr(c3), Nx, Nz, *h, 3, Nx - 4, 5, Nz - 4)
+    assert np.array_equal(out_g.a, out_o)
+
+
+def test_freesurface_exact(L, oracles):
+    say("case: acoustic_iso2d_freesurface")
+    Nx, Nz, pml = 9, 40, 12
+    p = np.random.default_rng(2).standard_normal((Nx, Nz)).astype(np.float32)
+    a = Array2D(p.copy())
+    L.acoustic_iso2d_freesurface(a.pp, Nx, Nz, pml)
+    q = p.copy()
+    oracles["f32s"].freesurface(fptr(q), Nx, Nz, pml)
+    assert np.array_equal(a.a, q)
+
+
+def test_meal_matches_oracle(L, oracles):
+    say("case: mealCoefs2d 70x90 n=20")
+    NX, NZ, n = 70, 90, 20
+    vp = np.random.default_rng(4).uniform(1500, 4500, (NX, NZ)).astype(np.float32)
+    D, B = Array2D(np.zeros((NX, NZ), np.float32)), Array2D(np.zeros((NX, NZ), np.float32))
+    L.mealCoefs2d(D.pp, B.pp, Array2D(vp).pp, None, 10.0, 2.5, 12.5, 12.5, NZ, NX, n)
+    Do, Bo = np.zeros_like(vp), np.zeros_like(vp)
+    oracles["f32s"].meal_coefs(fptr(Do), fptr(Bo), fptr(vp), 10.0, 12.5, 12.5, NZ, NX, n)
+    assert np.array_equal(D.a, Do) and np.array_equal(B.a, Bo)
+
+
+# --------------------------------------------------------------------------- modeling
+CASES = {
+    "tiny_fs0": dict(name="tiny"),
+    "tiny_fs1_T2": dict(name="tiny", freesurface=1, shot_depth_cells=1, rcv_depth_cells=1),
+    "c0": dict(name="c0"),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_modeling_parity(L, oracles, case):
+    p = dict(CASES[case])
+    w = synth.workload(p.pop("name"), **p)
+    say(f"case: modeling {case} {w.nz}x{w.nx} nt={w.nt} shots={w.nshot}")
+    g = recs(observed(L, w, True))
+    errs = {k: rel(g, recs(observed(oracles[k], w, False))) for k in ("f32s", "f32", "f64")}
+    say("  rel L2 records:", errs)
+    assert errs["f32s"] <= 1e-6
+    assert errs["f32"] <= 1e-5
+    assert errs["f64"] <= 1e-4
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_gradient_parity(L, oracles, case):
+    p = dict(CASES[case])
+    w = synth.workload(p.pop("name"), **p)
+    say(f"case: gradient {case} {w.nz}x{w.nx} nt={w.nt} shots={w.nshot}")
+    obs = observed(oracles["f32s"], w, False)
+    pre_g, g_g = gradient(L, w, obs, True)
+    f_g = L.calculate_objective_value(pre_g.recs, w.nshot)
+    out = {}
+    for k in ("f32s", "f32", "f64"):
+        pre_o, g_o = gradient(oracles[k], w, obs, False)
+        out[k] = (rel(g_g, g_o), rel(recs(pre_g), recs(pre_o)),
+                  abs(f_g - oracles[k].objective(pre_o.recs, w.nshot)) / oracles[k].objective(pre_o.recs, w.nshot))
+    say("  (grad, adjoint source, misfit) rel errors:", out)
+    assert max(out["f32s"]) <= 1e-6
+    assert out["f32"][0] <= 1e-4 and out["f32"][2] <= 1e-5
+    assert out["f64"][0] <= 1e-3
+
+
+def test_line_search_parity(L, oracles):
+    """problem 2 trial, objective and closed-form step (acoustic2d_fwi.c:418-452)."""
+    w = synth.workload("tiny", nshot=2)
+    say("case: line search tiny")
+    obs = observed(oracles["f32s"], w, False)
+    pre0_g, grad_g = gradient(L, w, obs, True)
+    o = oracles["f32s"]
+    pre0_o, grad_o = gradient(o, w, obs, False)
+    slow = (np.float32(1) / w.vp_start).astype(np.float32)
+    pert_g = np.zeros_like(slow)
+    pert_o = np.zeros_like(slow)
+    sc_g = L.compute_model_perturbation(0.01, fptr(slow), fptr(grad_g), fptr(pert_g), slow.size)
+    sc_o = o.model_perturbation(0.01, fptr(slow), fptr(grad_o), fptr(pert_o), slow.size)
+    assert abs(sc_g - sc_o) <= 1e-5 * abs(sc_o)
+    vp_trial = np.clip((np.float32(1) / (slow + pert_o)).astype(np.float32), w.vpmin, w.vpmax)
+    vp_trial[:, :w.waterdepth] = w.vp_water
+    vpt, rht = Array2D(vp_trial), Array2D(synth.gardner(vp_trial))
+    pre1_g, pre1_o = obs.header_copy(), obs.header_copy()
+    flib.propagate(L, w, vpt, rht, obs, 2, pre1_g)
+    o.propagation(vpt.pp, rht.pp, obs.recs, w.nshot, w.nz, w.nx, w.dx, w.dz, w.dt, w.nt, w.pml,
+                  w.freesurface, w.freq, float(w.filter), w.precondition, 2, pre1_o.recs, None)
+    assert rel(recs(pre1_g), recs(pre1_o)) <= 1e-6
+    f2g, f2o = L.calculate_objective_value(pre1_g.recs, w.nshot), o.objective(pre1_o.recs, w.nshot)
+    assert abs(f2g - f2o) <= 1e-5 * f2o
+    a_g = L.calculate_step_length(pre0_g.recs, pre1_g.recs, w.nshot)
+    a_o = o.step_length(pre0_o.recs, pre1_o.recs, w.nshot)
+    say("  step", a_g, a_o, "misfit", f2g, f2o)
+    assert abs(a_g - a_o) <= 1e-5 * abs(a_o)
+
+
+def test_batched_equals_plain_and_deterministic(L, oracles):
+    """One shot per launch and all shots per launch give bit-identical results,
+    and a repeat is bit-identical too (fixed-order reductions, no float atomics)."""
+    w = synth.workload("tiny", nshot=4)
+    say("case: batch 1 vs batch all, tiny 4 shots")
+    obs = observed(oracles["f32s"], w, False)
+    res = []
+    for b in (1, 0, 0):
+        L.fwiedu_set_batch(b)
+        pre, g = gradient(L, w, obs, True)
+        res.append((recs(pre), g))
+    L.fwiedu_set_batch(0)
+    assert flib.stat("batch") == 4
+    for r in res[1:]:
+        assert np.array_equal(r[0], res[0][0]) and np.array_equal(r[1], res[0][1])
+
+
+def test_shot_windows_grouping(L, oracles):
+    """Shots whose receiver spans give different simulation windows
+    (create_grid_observe2d's +-30-cell window) run as separate groups; the
+    global sums still follow shot order."""
+    w = synth.workload("tiny", nx=120, nshot=3)
+    say("case: windows tiny nx=120")
+    from fwiedu.records import ShotSet
+    dx = np.float32(w.dx)
+    cols = [20, 60, 100]
+    sx = [np.float32(c) * dx for c in cols]
+    sz = [np.float32(2) * dx] * 3
+    gx = [np.arange(max(0, c - 15), min(w.nx, c + 15), dtype=np.float32) * dx for c in cols]
+    gz = [np.full(g.size, np.float32(2) * dx, np.float32) for g in gx]
+
+    def shotset():
+        return ShotSet(sx, sz, gx, gz, w.src, w.nt, w.dt, w.dx, w.dz, w.nx, w.fpeak)
+
+    o = oracles["f32s"]
+    obs = shotset()
+    vp, rho = Array2D(w.vp_true), Array2D(w.rho_true)
+    o.propagation(vp.pp, rho.pp, obs.recs, 3, w.nz, w.nx, w.dx, w.dz, w.dt, w.nt, w.pml,
+                  w.freesurface, 0.0, 2.0, 1, 0, obs.recs, None)
+    assert len({(obs.recs[i].ix_min, obs.recs[i].ix_max) for i in range(3)}) == 3
+    pre_g, g_g = gradient(L, w, obs, True)
+    pre_o, g_o = gradient(o, w, obs, False)
+    assert rel(g_g, g_o) <= 1e-6 and rel(recs(pre_g), recs(pre_o)) <= 1e-6
+
+
+def test_engine_modes_with_host_checkpoint(L, oracles):
+    """iso_acoustic2d_propagation_engine modes 1 then 2 through the host
+    checkpoint give the single-shot gradient of the oracle."""
+    w = synth.workload("tiny", nshot=1, freesurface=1, shot_depth_cells=1)
+    say("case: engine modes 1/2 tiny fs1")
+    o = oracles["f32s"]
+    obs = observed(o, w, False)
+    pre_o, g_o = gradient(o, w, obs, False, precondition=0)
+    # the product, per shot
+    pre = obs.header_copy()
+    r = pre.recs[0]
+    nxx = r.ix_max - r.ix_min + 1
+    vpw = Array2D(w.vp_start[r.ix_min:r.ix_max + 1])
+    rhw = Array2D(w.rho_start[r.ix_min:r.ix_max + 1])
+    ck = flib.Acoustic2dCheckPoint()
+    L.acoustic2dCheckPoint_init(ctypes.byref(ck))
+    L.alloc_acoustic2dCheckPoint(ctypes.byref(ck), w.nz + 2 * w.pml, nxx + 2 * w.pml, w.nt)
+    pc = Array2D(np.zeros((nxx, w.nz), np.float32))
+    args = (vpw.pp, rhw.pp, nxx, w.nz, w.dx, w.dz, w.dt, w.nt, w.pml, w.freesurface, 1)
+    L.iso_acoustic2d_propagation_engine(ctypes.byref(r), ctypes.byref(ck), *args, 1, 1, 0, pc.pp)
+    assert ck.status[0] == 1 and ck.status[ck.dCheckPoint] == 1 and ck.status[1] == 0
+    vpf = Array2D(w.vp_start)
+    L.compute_adjoint_source2d(ctypes.byref(r), ctypes.byref(obs.recs[0]), vpf.pp, 0.0, 2,
+                               ctypes.byref(r))
+    assert rel(pre.trace(0), pre_o.trace(0)) <= 1e-6
+    gg = Array2D(np.zeros((nxx, w.nz), np.float32))
+    L.iso_acoustic2d_propagation_engine(ctypes.byref(r), ctypes.byref(ck), *args, 2, 1, 0, gg.pp)
+    L.free_acoustic2dCheckPoint(ctypes.byref(ck))
+    full = np.zeros_like(g_o)
+    full[r.ix_min:r.ix_max + 1] = gg.a
+    say("  engine gradient rel", rel(full, g_o))
+    assert rel(full, g_o) <= 1e-6
+
+
+def test_optimization_helpers(L, oracles):
+    say("case: optimisation helpers")
+    o = oracles["f32s"]
+    rng = np.random.default_rng(8)
+    n = 5000
+    g = rng.standard_normal(n).astype(np.float32)
+    p = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    g1, p1, g2, p2 = g.copy(), p.copy(), g.copy(), p.copy()
+    L.apply_gradient_shot_precondition(fptr(g1), fptr(p1), n)
+    o.shot_precondition(fptr(g2), fptr(p2), n)
+    assert np.array_equal(g1, g2) and np.array_equal(p1, p2)
+    g1, g2 = g.copy(), g.copy()
+    L.apply_gradient_precondition(fptr(g1), fptr(p), n, 0.005)
+    o.global_precondition(fptr(g2), fptr(p), n, 0.005)
+    assert np.array_equal(g1, g2)
+
+
+@pytest.mark.slow
+def test_c1_subset_parity(L, oracles):
+    """221 x 601, nt 4001, free surface, source at depth 1 cell: 2 of the 120 shots."""
+    w = synth.workload("c1", nshot=2)
+    say(f"case: c1 subset {w.nz}x{w.nx} nt={w.nt} shots=2")
+    obs_o = observed(oracles["f32"], w, False)
+    g = recs(observed(L, w, True))
+    e_rec = rel(g, recs(obs_o))
+    pre_g, g_g = gradient(L, w, obs_o, True)
+    pre_o, g_o = gradient(oracles["f32"], w, obs_o, False)
+    f_g = L.calculate_objective_value(pre_g.recs, w.nshot)
+    f_o = oracles["f32"].objective(pre_o.recs, w.nshot)
+    say("  records", e_rec, "grad", rel(g_g, g_o), "misfit", abs(f_g - f_o) / f_o)
+    assert e_rec <= 1e-5
+    assert rel(g_g, g_o) <= 1e-4
+    assert abs(f_g - f_o) <= 1e-5 * f_o
