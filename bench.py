@@ -65,7 +65,8 @@ def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
         log(f"step {i + 1}/{steps}: {times[-1] * 1e3:.1f} ms, misfit {misfit:.6g}")
     L.fwiedu_set_kernel_timing(0)
     stats = {k: flib.stat(k) for k in ("fd_launches", "fd_timed", "fd_ms_avg", "fd_cells_per_launch",
-                                       "fd_shots_per_launch", "batch", "device_bytes")}
+                                       "fd_shots_per_launch", "batch", "device_bytes", "tile_tz",
+                                       "tile_spb")}
     L.fwiedu_session_destroy(sess)
     L.fwiedu_set_batch(0)
     return times, stats, misfit
@@ -129,7 +130,10 @@ def main():
     cu = w.cell_updates_per_gradient_shot()
     S = int(st["fd_shots_per_launch"])
     cells = interior_cells(w)
-    bytes_per_launch = cells * (12.0 * S + 28.0)   # 12 B/shot-cell + 7 shared coefficient streams
+    # algorithmic bytes of one launch: p1, p0 read and p2 written per shot-cell
+    # (12 B), plus the 7 coefficient streams of the reference formulation
+    # (28 B per cell) shared by the S shots of the launch
+    bytes_per_launch = cells * (12.0 * S + 28.0)
     avg_ms = st["fd_ms_avg"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
 
@@ -167,7 +171,8 @@ def main():
                                f"{len(w.receiver_columns())} receivers, pml {w.pml}, free surface "
                                f"{w.freesurface}, checkpointed adjoint gradient, precondition "
                                f"{w.precondition}, model and records resident in HBM",
-                   "shots_per_launch": S, "batch": int(st["batch"])},
+                   "shots_per_launch": S, "batch": int(st["batch"]),
+                   "tile": {"rows": int(st["tile_tz"]), "shots_per_block": int(st["tile_spb"])}},
         "gcell_updates_per_s": shots_per_s * cu / 1e9,
         "cell_updates_per_gradient_shot": cu,
         "misfit": misfit,

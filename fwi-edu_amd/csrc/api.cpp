@@ -207,7 +207,7 @@ extern "C" void acoustic_iso2d_fd_it_6order(float **p2, float **p1, float **p0, 
     float **co[CO_N];
     co[CO_C1] = coef1;
     co[CO_C2] = coef2;
-    co[CO_C3] = coef3;
+    co[CO_A] = coef3;  // multiplied by K below
     co[CO_K] = K;
     co[CO_IR] = invrho;
     co[CO_IRX] = invrhox;
@@ -222,6 +222,7 @@ extern "C" void acoustic_iso2d_fd_it_6order(float **p2, float **p1, float **p0, 
             if (inner && lz >= 3 && lz < 3 + nzi) {
                 h[(size_t)(CO_N + 1) * fsz + i] = p0[ix][iz];
                 for (int k = 0; k < CO_N; ++k) h[(size_t)k * fsz + i] = co[k][ix][iz];
+                h[(size_t)CO_A * fsz + i] = coef3[ix][iz] * K[ix][iz];  // coef3*K, as at l.71
             }
         }
     }
@@ -245,7 +246,7 @@ extern "C" void acoustic_iso2d_fd_it_6order(float **p2, float **p1, float **p0, 
     a.invdz = invdz;
     a.invdxdx = invdxdx;
     a.invdzdz = invdzdz;
-    launch_fd(FD_ADJ, a, c.st);
+    launch_fd(FD_ADJ, a, FdTile(), c.st);
     std::vector<float> o((size_t)fsz);
     d2h(o.data(), d + (CO_N + 2) * fsz, fsz * sizeof(float));
     sync();
@@ -434,6 +435,19 @@ extern "C" float calculate_memory_cost_checkpoint(int nt, int NX, int NZ)
 // fwiedu_* controls
 
 extern "C" void fwiedu_set_batch(int s) { ctx().batch_req = s < 0 ? 0 : s; }
+extern "C" void fwiedu_set_tile(int tz, int spb)
+{
+    Ctx &c = ctx();
+    if (tz == 0) {
+        c.tile_req = FdTile();
+        c.tile_req.tz = 0;
+        return;
+    }
+    if ((tz != 32 && tz != 64) || (spb != 1 && spb != 2 && spb != 4 && spb != 8))
+        fail("<error>: fwiedu_set_tile: tile height 32 or 64, shots per block 1, 2, 4 or 8");
+    c.tile_req.tz = tz;
+    c.tile_req.spb = spb;
+}
 extern "C" void fwiedu_set_memory_fraction(double f)
 {
     ctx().memfrac = (f > 0.0 && f <= 0.98) ? f : 0.85;
@@ -458,6 +472,8 @@ extern "C" double fwiedu_stat(const char *name)
     if (n == "fd_cells_per_launch") return s.fd_timed ? s.fd_cells_timed / s.fd_timed : 0.0;
     if (n == "fd_shots_per_launch") return (double)s.shots_per_launch;
     if (n == "batch") return (double)s.batch;
+    if (n == "tile_tz") return (double)s.tile_tz;
+    if (n == "tile_spb") return (double)s.tile_spb;
     if (n == "h2d_bytes") return s.h2d_bytes;
     if (n == "d2h_bytes") return s.d2h_bytes;
     if (n == "device_bytes") {

@@ -153,7 +153,7 @@ void build_coefs(const Geom &g, const float *vpw, const float *rhow, float fpeak
 // ---------------------------------------------------------------------------
 // sampled stencil timing
 
-void fd_launch(int mode, const FdArgs &a, long cells)
+void fd_launch(int mode, const FdArgs &a, long cells, FdTile tile)
 {
     Ctx &c = g_ctx;
     // sampled: the reconstruction sweep, the stencil with no accumulation epilogue
@@ -167,7 +167,7 @@ void fd_launch(int mode, const FdArgs &a, long cells)
         }
         HIPCHECK(hipEventRecord(c.ev[c.ev_used], c.st));
     }
-    launch_fd(mode, a, c.st);
+    launch_fd(mode, a, tile, c.st);
     if (timed) {
         HIPCHECK(hipEventRecord(c.ev[c.ev_used + 1], c.st));
         c.ev_cells.push_back({(long)c.ev_used, cells});
@@ -192,6 +192,56 @@ void collect_timing()
     }
     c.ev_cells.clear();
     c.ev_used = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Launch-shape autotuner.  For each new (geometry, shots-per-launch) it times
+// every tile height (32, 64 rows) and shots-per-block (1, 2, 4, 8) on the
+// reconstruction sweep over zero fields (output into a work buffer that the
+// next pass overwrites), and keeps the fastest.
+
+FdTile autotune_tile(const FdArgs &base, float *out, const float *zero)
+{
+    Ctx &c = g_ctx;
+    if (c.tile_req.tz > 0) return c.tile_req;
+    const auto key = std::make_tuple(base.Nx, base.ld, base.ixs, base.ixe, base.izs, base.ize,
+                                     base.fs, base.nshot);
+    auto it = c.tuned.find(key);
+    if (it != c.tuned.end()) return it->second;
+    FdArgs a = base;
+    a.out = out;
+    a.cur = zero;
+    a.prev = zero;
+    a.src_cell = nullptr;
+    a.acc = nullptr;
+    a.acc_on = 0;
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    FdTile best;
+    float best_ms = 1e30f;
+    for (int tz : {64, 32})
+        for (int spb : {1, 2, 4, 8}) {
+            if (spb > 1 && spb > base.nshot) continue;
+            FdTile t;
+            t.tz = tz;
+            t.spb = spb;
+            launch_fd(FD_RECON, a, t, c.st);
+            HIPCHECK(hipEventRecord(e0, c.st));
+            for (int r = 0; r < 3; ++r) launch_fd(FD_RECON, a, t, c.st);
+            HIPCHECK(hipEventRecord(e1, c.st));
+            HIPCHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+        }
+    HIPCHECK(hipEventDestroy(e0));
+    HIPCHECK(hipEventDestroy(e1));
+    c.tuned[key] = best;
+    return best;
 }
 
 // ---------------------------------------------------------------------------
@@ -453,6 +503,7 @@ struct BatchState {
     double *acc = nullptr;     // [S][nx*nz]
     long nacc = 0;
     long cells_per_launch = 0;
+    FdTile tile;
     float *F(int k) const { return arena + (long)k * g.fsz; }
 };
 
@@ -505,6 +556,10 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
         HIPCHECK(hipMemsetAsync(b.acc, 0, (size_t)S * b.nacc * sizeof(double), c.st));
     }
     b.cells_per_launch = interior_cells(g) * S;
+    b.tile = autotune_tile(base_args(g, coef, S, b.stride), b.F(BatchPlan::A2),
+                           b.F(BatchPlan::ZERO));
+    c.stats.tile_tz = b.tile.tz;
+    c.stats.tile_spb = b.tile.spb;
 }
 
 // propagation.c:539-608.  With save_ckpt the fields u(kd-1), u(kd) land in
@@ -534,7 +589,7 @@ static void forward_pass(BatchState &b, bool save_ckpt, bool precond)
         a.prev = prev;
         a.it = it;
         a.acc_on = (precond && it % MST == 0) ? 1 : 0;
-        fd_launch(FD_FWD, a, b.cells_per_launch);
+        fd_launch(FD_FWD, a, b.cells_per_launch, b.tile);
         prev = cur;
         cur = out;
     }
@@ -589,7 +644,7 @@ static void adjoint_pass(BatchState &b)
                 fr.cur = fcur;
                 fr.prev = fprev;
                 fr.it = itt;
-                fd_launch(FD_RECON, fr, b.cells_per_launch);
+                fd_launch(FD_RECON, fr, b.cells_per_launch, b.tile);
                 fprev = fcur;
                 fcur = fout;
                 stored[t] = 1;
@@ -605,7 +660,7 @@ static void adjoint_pass(BatchState &b)
         ad.it = it;
         ad.acc_on = (it % MST == 0) ? 1 : 0;
         ad.u = (it % MST == 0) ? where[it] : nullptr;
-        fd_launch(FD_ADJ, ad, b.cells_per_launch);
+        fd_launch(FD_ADJ, ad, b.cells_per_launch, b.tile);
         prev = cur;
         cur = out;
     }

@@ -6,7 +6,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/fwiedu.h"
@@ -44,6 +46,7 @@ struct Stats {
     long shots_per_launch = 0;
     double device_bytes = 0;
     double h2d_bytes = 0, d2h_bytes = 0;
+    int tile_tz = 0, tile_spb = 0;
 };
 
 struct Ctx {
@@ -55,6 +58,8 @@ struct Ctx {
     int ktiming = 0;
     long recon_launches = 0;
     int ckpt_bits = 32;
+    FdTile tile_req = {0, 1};     // tz == 0: autotune
+    std::map<std::tuple<int, int, int, int, int, int, int, int>, FdTile> tuned;
     Stats stats;
     std::vector<hipEvent_t> ev;   // pool of events for sampled kernel timing
     size_t ev_used = 0;
@@ -96,7 +101,8 @@ void adjoint_source_device(const Record2D *r, float **vp_full, float freq, int f
                            const float *pre_dev, const float *obs_dev, float *out_dev);
 
 // Sampled timing of stencil launches.
-void fd_launch(int mode, const FdArgs &a, long cells);
+void fd_launch(int mode, const FdArgs &a, long cells, FdTile tile);
+FdTile autotune_tile(const FdArgs &base, float *out, const float *zero);
 void collect_timing();
 
 }  // namespace fwiedu

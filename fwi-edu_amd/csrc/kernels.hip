@@ -15,139 +15,221 @@
 namespace fwiedu {
 
 // ---------------------------------------------------------------------------
-// The stencil step.  One block = 4 waves of 64 lanes.  A tile is 64 rows of z
-// (one row per lane, so every global access of a wave is one coalesced 256 B
-// column segment) by TX = 16 columns of x (4 per wave).  p1 of the tile plus
-// its 3-cell halo is staged in LDS once; the 13-point cross is read from there.
-// p0, the 7 coefficients and the output go straight between HBM and registers.
+// The stencil step.
+//
+// One block = 4 waves.  A tile is TZ rows of z by TX columns of x; a wave's 64
+// lanes cover 64/TZ adjacent columns of TZ rows, so every global access of a
+// wave is 64/TZ column segments of TZ*4 bytes (coalesced), and each wave makes
+// CXP such passes.  TZ = 64 gives 16-column tiles, TZ = 32 gives 32-column
+// tiles; the engine picks the height that wastes the fewest rows.
+//
+// A block advances SPB shots of the same tile in turn.  The six coefficient
+// streams of its cells are loaded once into registers and reused for every
+// shot; per shot, p1 of the tile plus its 3-cell halo is staged in LDS, and p0
+// is loaded before the barrier so all of a thread's loads are in flight
+// together.
 //
 // Epilogue, fused per cell (propagation.c):
 //   FD_FWD   : +source in double (l.558), free-surface mirror (l.76-101),
 //              pseudo-Hessian every MST steps (l.583-593)
 //   FD_RECON : free-surface mirror, then +source in float (l.655-662)
 //   FD_ADJ   : free-surface mirror, imaging every MST steps (l.700-713)
-// The mirror is done by the cell being mirrored: a cell at row pml+k,
+// The mirror is written by the cell being mirrored: a cell at row pml+k,
 // 1 <= k <= pml, also writes -value to row pml-k; row pml itself becomes -0.
-constexpr int TZ = 64;
+//
+// coef3*K is stored as one array (CO_A): the reference's expression
+// coef3[i]*K[i]*(...) evaluates coef3*K first, so the product is the same
+// FP32 value.
 constexpr int NWAVE = 4;
-constexpr int CX = 4;
-constexpr int TX = NWAVE * CX;
-constexpr int HX = TX + 6;
-constexpr int HZ = TZ + 6;
+constexpr int CXP = 4;
 
-template <int MODE>
+template <int TZ>
+struct Tile {
+    static constexpr int CPW = 64 / TZ;           // columns per wave pass
+    static constexpr int TX = NWAVE * CPW * CXP;  // tile columns
+    static constexpr int HX = TX + 6;
+    static constexpr int HZ = TZ + 6;
+};
+
+// Blocks are dispatched to the 8 XCDs round-robin by linear id.  Renumber so
+// that each XCD receives one contiguous range of logical blocks: neighbouring
+// tiles (shared p1 halos) and the shot groups of one tile (shared
+// coefficients) then meet in the same L2.
+__device__ inline int xcd_remap(int b, int nb)
+{
+    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+    return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+template <int MODE, int TZ, int SPB>
 __global__ __launch_bounds__(256) void fd_kernel(FdArgs a)
 {
-    __shared__ float t[HX][HZ];
+    using T = Tile<TZ>;
+    __shared__ float t[T::HX][T::HZ];
     const float C1 = 0.77793115f, C2 = -0.17388691f, C3 = 0.02338713f;
     const float D0 = -2.8215452f, D1 = 1.57663238f, D2 = -0.18347238f, D3 = 0.01761260f;
 
-    const int tile = blockIdx.x;
-    const int s = blockIdx.y;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int tile = L / a.ngroups, grp = L - tile * a.ngroups;
     const int xt = tile % a.xtiles, zt = tile / a.xtiles;
-    const int x0 = a.ixs + xt * TX, z0 = a.izs + zt * TZ;
-    const long so = (long)s * a.shot_stride;
-    const float *__restrict__ p1 = a.cur + so;
-    const float *__restrict__ p0 = a.prev + so;
-    float *__restrict__ p2 = a.out + so;
+    const int x0 = a.ixs + xt * T::TX, z0 = a.izs + zt * TZ;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int zl = lane % TZ, xl = lane / TZ;
     const int ld = a.ld;
-
-    // stage p1: columns x0-3 .. x0+TX+2, rows z0-3 .. z0+66 (x0 >= 3, z0 >= 3)
-    for (int c = w; c < HX; c += NWAVE) {
-        const int ix = x0 - 3 + c, iz = z0 + lane;
-        float v = 0.f;
-        if (ix < a.Nx && iz < ld) v = p1[(long)ix * ld + iz];
-        t[c][lane + 3] = v;
-    }
-    for (int idx = threadIdx.x; idx < HX * 6; idx += 256) {
-        const int c = idx / 6, h = idx - c * 6;
-        const int r = h < 3 ? h : TZ + h;
-        const int ix = x0 - 3 + c, iz = z0 - 3 + r;
-        float v = 0.f;
-        if (ix < a.Nx && iz < ld) v = p1[(long)ix * ld + iz];
-        t[c][r] = v;
-    }
-    __syncthreads();
-
-    const float *__restrict__ cf = a.coef;
     const long fsz = (long)a.Nx * ld;
-    int scell = -1;
-    if (MODE != FD_ADJ && a.src_cell) scell = a.src_cell[s];
+    const float *__restrict__ cf = a.coef;
+
+    // this thread's cells and their coefficients (reused for all SPB shots)
+    int cell[CXP];
+    bool ok[CXP];
+    float k1[CXP], k2[CXP], kA[CXP], kx[CXP], kz[CXP], kr[CXP];
+#pragma unroll
+    for (int j = 0; j < CXP; ++j) {
+        const int ix = x0 + w * T::CPW * CXP + j * T::CPW + xl, iz = z0 + zl;
+        ok[j] = ix <= a.ixe && iz <= a.ize;
+        cell[j] = ix * ld + iz;
+        if (ok[j]) {
+            k1[j] = cf[CO_C1 * fsz + cell[j]];
+            k2[j] = cf[CO_C2 * fsz + cell[j]];
+            kA[j] = cf[CO_A * fsz + cell[j]];
+            kx[j] = cf[CO_IRX * fsz + cell[j]];
+            kz[j] = cf[CO_IRZ * fsz + cell[j]];
+            kr[j] = cf[CO_IR * fsz + cell[j]];
+        }
+    }
+
+    for (int si = 0; si < SPB; ++si) {
+        const int s = grp * SPB + si;
+        if (s >= a.nshot) break;  // uniform across the block
+        const long so = (long)s * a.shot_stride;
+        const float *__restrict__ p1 = a.cur + so;
+        const float *__restrict__ p0 = a.prev + so;
+        float *__restrict__ p2 = a.out + so;
+
+        // stage p1: columns x0-3 .. x0+TX+2, rows z0-3 .. z0+TZ+2 (x0, z0 >= 3)
+        for (int c = w * T::CPW + xl; c < T::HX; c += NWAVE * T::CPW) {
+            const int ix = x0 - 3 + c, iz = z0 + zl;
+            float v = 0.f;
+            if (ix < a.Nx && iz < ld) v = p1[(long)ix * ld + iz];
+            t[c][zl + 3] = v;
+        }
+        for (int k = threadIdx.x; k < T::HX * 6; k += 256) {
+            const int c = k / 6, h = k - c * 6;
+            const int r = h < 3 ? h : TZ + h;
+            const int ix = x0 - 3 + c, iz = z0 - 3 + r;
+            float v = 0.f;
+            if (ix < a.Nx && iz < ld) v = p1[(long)ix * ld + iz];
+            t[c][r] = v;
+        }
+        float q0[CXP];
+#pragma unroll
+        for (int j = 0; j < CXP; ++j) q0[j] = ok[j] ? p0[cell[j]] : 0.f;
+        int scell = -1;
+        if (MODE != FD_ADJ && a.src_cell) scell = a.src_cell[s];
+        __syncthreads();
 
 #pragma unroll
-    for (int j = 0; j < CX; ++j) {
-        const int c = 3 + w * CX + j, r = lane + 3;
-        const int ix = x0 + w * CX + j, iz = z0 + lane;
-        if (ix > a.ixe || iz > a.ize) continue;
-        const long i = (long)ix * ld + iz;
-        const float pc = t[c][r];
-        const float dpdz = (C1 * (t[c][r + 1] - t[c][r - 1]) + C2 * (t[c][r + 2] - t[c][r - 2]) +
-                            C3 * (t[c][r + 3] - t[c][r - 3])) * a.invdz;
-        const float dpdx = (C1 * (t[c + 1][r] - t[c - 1][r]) + C2 * (t[c + 2][r] - t[c - 2][r]) +
-                            C3 * (t[c + 3][r] - t[c - 3][r])) * a.invdx;
-        const float dzz = (D0 * pc + D1 * (t[c][r + 1] + t[c][r - 1]) +
-                           D2 * (t[c][r + 2] + t[c][r - 2]) + D3 * (t[c][r + 3] + t[c][r - 3])) *
-                          a.invdzdz;
-        const float dxx = (D0 * pc + D1 * (t[c + 1][r] + t[c - 1][r]) +
-                           D2 * (t[c + 2][r] + t[c - 2][r]) + D3 * (t[c + 3][r] + t[c - 3][r])) *
-                          a.invdxdx;
-        const float p0v = p0[i];
-        const float v = cf[CO_C1 * fsz + i] * pc + cf[CO_C2 * fsz + i] * p0v +
-                        cf[CO_C3 * fsz + i] * cf[CO_K * fsz + i] *
-                            (cf[CO_IRX * fsz + i] * dpdx + cf[CO_IRZ * fsz + i] * dpdz +
-                             cf[CO_IR * fsz + i] * (dxx + dzz));
+        for (int j = 0; j < CXP; ++j) {
+            if (!ok[j]) continue;
+            const int c = 3 + w * T::CPW * CXP + j * T::CPW + xl, r = zl + 3;
+            const int i = cell[j];
+            const float pc = t[c][r];
+            const float dpdz = (C1 * (t[c][r + 1] - t[c][r - 1]) + C2 * (t[c][r + 2] - t[c][r - 2]) +
+                                C3 * (t[c][r + 3] - t[c][r - 3])) * a.invdz;
+            const float dpdx = (C1 * (t[c + 1][r] - t[c - 1][r]) + C2 * (t[c + 2][r] - t[c - 2][r]) +
+                                C3 * (t[c + 3][r] - t[c - 3][r])) * a.invdx;
+            const float dzz = (D0 * pc + D1 * (t[c][r + 1] + t[c][r - 1]) +
+                               D2 * (t[c][r + 2] + t[c][r - 2]) +
+                               D3 * (t[c][r + 3] + t[c][r - 3])) * a.invdzdz;
+            const float dxx = (D0 * pc + D1 * (t[c + 1][r] + t[c - 1][r]) +
+                               D2 * (t[c + 2][r] + t[c - 2][r]) +
+                               D3 * (t[c + 3][r] + t[c - 3][r])) * a.invdxdx;
+            const float p0v = q0[j];
+            const float v = k1[j] * pc + k2[j] * p0v +
+                            kA[j] * (kx[j] * dpdx + kz[j] * dpdz + kr[j] * (dxx + dzz));
 
-        float val = v;
-        if (MODE == FD_FWD && (int)i == scell) {
-            const double srcv = 1.0 * (double)a.src_wav[(long)s * a.nt + a.it] *
-                                (double)cf[CO_K * fsz + i] * (double)a.dt * (double)a.dt *
-                                (double)a.invdx * (double)a.invdz;
-            val = (float)((double)v + srcv);
-        }
-        float stored = val;
-        if (a.fs) {
-            const int k = iz - a.pml;
-            if (k == 0)
-                stored = -0.0f;
-            else if (k <= a.pml)
-                p2[(long)ix * ld + (a.pml - k)] = -val;
-        }
-        if (MODE == FD_RECON && (int)i == scell) {
-            stored = stored + a.src_wav[(long)s * a.nt + a.it] * cf[CO_K * fsz + i] * a.dt * a.dt *
-                                  a.invdx * a.invdz;
-        }
-        p2[i] = stored;
+            float val = v;
+            if (MODE == FD_FWD && i == scell) {
+                const double srcv = 1.0 * (double)a.src_wav[(long)s * a.nt + a.it] *
+                                    (double)cf[CO_K * fsz + i] * (double)a.dt * (double)a.dt *
+                                    (double)a.invdx * (double)a.invdz;
+                val = (float)((double)v + srcv);
+            }
+            float stored = val;
+            const int ix = i / ld, iz = i - ix * ld;
+            if (a.fs) {
+                const int k = iz - a.pml;
+                if (k == 0)
+                    stored = -0.0f;
+                else if (k <= a.pml)
+                    p2[(long)ix * ld + (a.pml - k)] = -val;
+            }
+            if (MODE == FD_RECON && i == scell) {
+                stored = stored + a.src_wav[(long)s * a.nt + a.it] * cf[CO_K * fsz + i] * a.dt *
+                                      a.dt * a.invdx * a.invdz;
+            }
+            p2[i] = stored;
 
-        if (MODE != FD_RECON && a.acc_on) {
-            const int px = ix - a.pml, pz = iz - a.pml;
-            if (px >= 0 && px < a.nx && pz >= 0 && pz < a.nz) {
-                double *acc = a.acc + (long)s * a.acc_stride + (long)px * a.nz + pz;
-                if (MODE == FD_FWD) {
-                    const float q = (stored - 2.0f * pc + p0v) * a.invdtdt;
-                    *acc += (double)(q * q);
-                } else {
-                    const float q = (stored - 2.0f * pc + p0v) * a.invdtdt * a.u[so + i];
-                    *acc += (double)q;
+            if (MODE != FD_RECON && a.acc_on) {
+                const int px = ix - a.pml, pz = iz - a.pml;
+                if (px >= 0 && px < a.nx && pz >= 0 && pz < a.nz) {
+                    double *acc = a.acc + (long)s * a.acc_stride + (long)px * a.nz + pz;
+                    if (MODE == FD_FWD) {
+                        const float q = (stored - 2.0f * pc + p0v) * a.invdtdt;
+                        *acc += (double)(q * q);
+                    } else {
+                        const float q = (stored - 2.0f * pc + p0v) * a.invdtdt * a.u[so + i];
+                        *acc += (double)q;
+                    }
                 }
             }
         }
+        __syncthreads();
     }
 }
 
-void launch_fd(int mode, const FdArgs &a0, hipStream_t st)
+template <int MODE, int TZ, int SPB>
+static void launch_fd_t(const FdArgs &a0, hipStream_t st)
 {
     FdArgs a = a0;
-    a.xtiles = (a.ixe - a.ixs + 1 + TX - 1) / TX;
+    a.xtiles = (a.ixe - a.ixs + 1 + Tile<TZ>::TX - 1) / Tile<TZ>::TX;
     a.ztiles = (a.ize - a.izs + 1 + TZ - 1) / TZ;
-    dim3 grid(a.xtiles * a.ztiles, a.nshot);
-    if (mode == FD_FWD)
-        fd_kernel<FD_FWD><<<grid, 256, 0, st>>>(a);
-    else if (mode == FD_RECON)
-        fd_kernel<FD_RECON><<<grid, 256, 0, st>>>(a);
-    else
-        fd_kernel<FD_ADJ><<<grid, 256, 0, st>>>(a);
+    a.ngroups = (a.nshot + SPB - 1) / SPB;
+    const long nb = (long)a.xtiles * a.ztiles * a.ngroups;
+    fd_kernel<MODE, TZ, SPB><<<dim3((unsigned)nb), 256, 0, st>>>(a);
 }
+
+template <int MODE, int TZ>
+static void launch_fd_spb(const FdArgs &a, int spb, hipStream_t st)
+{
+    switch (spb) {
+    case 1: launch_fd_t<MODE, TZ, 1>(a, st); break;
+    case 2: launch_fd_t<MODE, TZ, 2>(a, st); break;
+    case 4: launch_fd_t<MODE, TZ, 4>(a, st); break;
+    default: launch_fd_t<MODE, TZ, 8>(a, st); break;
+    }
+}
+
+template <int MODE>
+static void launch_fd_mode(const FdArgs &a, FdTile tl, hipStream_t st)
+{
+    if (tl.tz == 32)
+        launch_fd_spb<MODE, 32>(a, tl.spb, st);
+    else
+        launch_fd_spb<MODE, 64>(a, tl.spb, st);
+}
+
+void launch_fd(int mode, const FdArgs &a, FdTile tl, hipStream_t st)
+{
+    if (mode == FD_FWD)
+        launch_fd_mode<FD_FWD>(a, tl, st);
+    else if (mode == FD_RECON)
+        launch_fd_mode<FD_RECON>(a, tl, st);
+    else
+        launch_fd_mode<FD_ADJ>(a, tl, st);
+}
+
+int fd_tile_rows_padded(int tz, int izs, int ize) { return (ize - izs + 1 + tz - 1) / tz * tz; }
 
 // ---------------------------------------------------------------------------
 // Set-up kernels (propagation.c:103-171, 192-354, 508-532; common.c:110-176,416-479)
@@ -253,10 +335,12 @@ __global__ void coef_kernel(float *vp, float *rho, float *D, const float *Beta, 
     const float dt2D2 = dt * dt * d * d;
     const float dtD = dt * d;
     const float dt2 = dt * dt;
+    const float c3 = (float)((double)dt2 / (1.0 + (double)dtD));
+    const float K = r * v * v;
     coef[CO_C1 * fsz + i] = (float)((2.0 - (double)dt2D2) / (1.0 + (double)dtD));
     coef[CO_C2 * fsz + i] = (float)(-1.0 * (1.0 - (double)dtD) / (1.0 + (double)dtD));
-    coef[CO_C3 * fsz + i] = (float)((double)dt2 / (1.0 + (double)dtD));
-    coef[CO_K * fsz + i] = r * v * v;
+    coef[CO_A * fsz + i] = c3 * K;
+    coef[CO_K * fsz + i] = K;
     coef[CO_IR * fsz + i] = 1.0f / r;
 }
 

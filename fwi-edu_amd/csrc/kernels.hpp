@@ -25,8 +25,17 @@ struct Geom {
     long fsz;            // floats per field = Nx * ld
 };
 
-// The seven coefficient arrays, consecutive blocks of fsz floats.
-enum { CO_C1 = 0, CO_C2, CO_C3, CO_K, CO_IR, CO_IRX, CO_IRZ, CO_N };
+// The coefficient arrays, consecutive blocks of fsz floats.  The stencil streams
+// the first six; K (bulk modulus) is read only at source and receiver cells.
+// CO_A holds coef3*K (propagation.c:71 evaluates that product first).
+enum { CO_C1 = 0, CO_C2, CO_A, CO_IRX, CO_IRZ, CO_IR, CO_K, CO_N };
+constexpr int CO_STREAMED = 6;
+
+// Stencil launch shape: tile height (32 or 64 rows) and shots per block (1,2,4,8).
+struct FdTile {
+    int tz = 64;
+    int spb = 1;
+};
 
 struct FdArgs {
     const float *coef;      // CO_N * fsz
@@ -37,7 +46,7 @@ struct FdArgs {
     int nshot;
     int Nx, ld, ixs, ixe, izs, ize, pml, fs, nx, nz;
     float invdx, invdz, invdxdx, invdzdz, invdtdt, dt;
-    int xtiles, ztiles;
+    int xtiles, ztiles, ngroups;
     // source (FD_FWD / FD_RECON)
     const int *src_cell;    // [nshot] flat padded index or -1
     const float *src_wav;   // [nshot][nt]
@@ -49,7 +58,8 @@ struct FdArgs {
     int acc_on;
 };
 
-void launch_fd(int mode, const FdArgs &a, hipStream_t st);
+void launch_fd(int mode, const FdArgs &a, FdTile tile, hipStream_t st);
+int fd_tile_rows_padded(int tz, int izs, int ize);
 
 // ---- set-up --------------------------------------------------------------
 // window model [nx][nz] (contiguous) -> padded [Nx][ld] by edge replication

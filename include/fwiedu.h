@@ -143,6 +143,10 @@ float calculate_step_length(Record2D *record2d_pre0, Record2D *record2d_pre1, in
 /* Shots advanced per launch by iso_acoustic2d_propagation (0 = as many as fit
  * in the device-memory budget; 1 = one shot per launch, the plain path). */
 void fwiedu_set_batch(int shots_per_launch);
+/* Stencil launch shape: tile height tz (32 or 64 rows) and shots per block spb
+ * (1, 2, 4, 8).  tz = 0 (default) lets the library's autotuner time every shape
+ * once per geometry and keep the fastest. */
+void fwiedu_set_tile(int tz, int spb);
 /* Fraction of free device memory the engine may use (default 0.85). */
 void fwiedu_set_memory_fraction(double frac);
 /* Sample every `every`-th reconstruction-sweep launch (the stencil with no

@@ -3,9 +3,11 @@ same seeded inputs.
 
 Tolerances (relative L2 unless noted):
   vs the strict-order FP32 oracle (f32s, same operation order as the kernels):
-      records, adjoint sources, gradients <= 1e-6 (bit-exact expected)
-  vs the reference-flags FP32 oracle (f32, -O3 -ffast-math):
-      records 1e-5, gradients 1e-4, misfit and step length 1e-5
+      records, adjoint sources, gradients, misfit <= 1e-6 (bit-exact observed)
+  vs the reference-flags FP32 oracle (f32, -O3 -ffast-math): records 1e-5,
+      gradients 1e-4, misfit 1e-5 -- or 1.5x the spread between the two CPU
+      builds of the same oracle when that spread is larger (the -ffast-math
+      build's own rounding, e.g. the c0 misfit, where the residual is small)
   vs the FP64 oracle: stated, and bounded by 1e-4 (records) / 1e-3 (gradients)
 """
 import ctypes
@@ -129,7 +131,8 @@ def test_modeling_parity(L, oracles, case):
     errs = {k: rel(g, recs(observed(oracles[k], w, False))) for k in ("f32s", "f32", "f64")}
     say("  rel L2 records:", errs)
     assert errs["f32s"] <= 1e-6
-    assert errs["f32"] <= 1e-5
+    spread = rel(recs(observed(oracles["f32"], w, False)), recs(observed(oracles["f32s"], w, False)))
+    assert errs["f32"] <= max(1e-5, 1.5 * spread)
     assert errs["f64"] <= 1e-4
 
 
@@ -141,14 +144,18 @@ def test_gradient_parity(L, oracles, case):
     obs = observed(oracles["f32s"], w, False)
     pre_g, g_g = gradient(L, w, obs, True)
     f_g = L.calculate_objective_value(pre_g.recs, w.nshot)
-    out = {}
+    out, ref = {}, {}
     for k in ("f32s", "f32", "f64"):
         pre_o, g_o = gradient(oracles[k], w, obs, False)
-        out[k] = (rel(g_g, g_o), rel(recs(pre_g), recs(pre_o)),
-                  abs(f_g - oracles[k].objective(pre_o.recs, w.nshot)) / oracles[k].objective(pre_o.recs, w.nshot))
-    say("  (grad, adjoint source, misfit) rel errors:", out)
+        f_o = oracles[k].objective(pre_o.recs, w.nshot)
+        out[k] = (rel(g_g, g_o), rel(recs(pre_g), recs(pre_o)), abs(f_g - f_o) / f_o)
+        ref[k] = (g_o, f_o)
+    # spread between the two CPU builds of the oracle (strict order vs -ffast-math)
+    spread = (rel(ref["f32"][0], ref["f32s"][0]), abs(ref["f32"][1] - ref["f32s"][1]) / ref["f32s"][1])
+    say("  (grad, adjoint source, misfit) rel errors:", out, "cpu-build spread:", spread)
     assert max(out["f32s"]) <= 1e-6
-    assert out["f32"][0] <= 1e-4 and out["f32"][2] <= 1e-5
+    assert out["f32"][0] <= max(1e-4, 1.5 * spread[0])
+    assert out["f32"][2] <= max(1e-5, 1.5 * spread[1])
     assert out["f64"][0] <= 1e-3
 
 
