@@ -66,10 +66,28 @@ def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
     L.fwiedu_set_kernel_timing(0)
     stats = {k: flib.stat(k) for k in ("fd_launches", "fd_timed", "fd_ms_avg", "fd_cells_per_launch",
                                        "fd_shots_per_launch", "batch", "device_bytes", "tile_tz",
-                                       "tile_spb")}
+                                       "tile_spb", "tile_kind", "tile_zfast", "tile_remap",
+                                       "tile_txm", "tile_pd")}
     L.fwiedu_session_destroy(sess)
     L.fwiedu_set_batch(0)
     return times, stats, misfit
+
+
+def copy_calibration(mb):
+    """A plain device-to-device copy of known size, so that hardware byte
+    counters (FETCH_SIZE, WRITE_SIZE) of the stencil can be scaled."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mb << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: 3 x {mb} MiB device-to-device")
 
 
 def cpu_baseline(w, nshots):
@@ -102,6 +120,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c1")
     ap.add_argument("--nshot", type=int, default=0, help="override the shot count")
+    ap.add_argument("--nt", type=int, default=0,
+                    help="override the time steps (short profiled runs; not a bench line)")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="also run a device-to-device copy of this size (counter calibration)")
     ap.add_argument("--batch", type=int, default=0, help="shots per launch (0 = all that fit)")
     ap.add_argument("--plain-shots", type=int, default=4)
     ap.add_argument("--cpu-shots", type=int, default=4)
@@ -117,6 +139,8 @@ def main():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
     over = {"nshot": args.nshot} if args.nshot else {}
+    if args.nt:
+        over["nt"] = args.nt
     w = synth.workload(args.workload, **over)
     log(f"workload {w.name}: {w.nz}x{w.nx} nt={w.nt} shots={w.nshot}")
     t0 = time.perf_counter()
@@ -150,6 +174,8 @@ def main():
                  "sample": f"{args.plain_shots} shots, 1 step"}
         log(f"plain: {ps:.2f} shots/s")
 
+    if args.copy_mb:
+        copy_calibration(args.copy_mb)
     cpu = None if args.no_cpu else cpu_baseline(w, args.cpu_shots)
     if cpu:
         log(f"cpu baseline: {cpu['value']:.3f} shots/s on {cpu['cores']} threads")
@@ -172,7 +198,10 @@ def main():
                                f"{w.freesurface}, checkpointed adjoint gradient, precondition "
                                f"{w.precondition}, model and records resident in HBM",
                    "shots_per_launch": S, "batch": int(st["batch"]),
-                   "tile": {"rows": int(st["tile_tz"]), "shots_per_block": int(st["tile_spb"])}},
+                   "tile": {"kernel": "column march" if st["tile_kind"] == 1 else "square tiles",
+                            "rows": int(st["tile_tz"]), "shots_per_block": int(st["tile_spb"]),
+                            "z_fast_order": int(st["tile_zfast"]), "xcd_order": int(st["tile_remap"]),
+                            "march_cols": int(st["tile_txm"]), "prefetch": int(st["tile_pd"])}},
         "gcell_updates_per_s": shots_per_s * cu / 1e9,
         "cell_updates_per_gradient_shot": cu,
         "misfit": misfit,

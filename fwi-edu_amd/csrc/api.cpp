@@ -435,6 +435,8 @@ extern "C" float calculate_memory_cost_checkpoint(int nt, int NX, int NZ)
 // fwiedu_* controls
 
 extern "C" void fwiedu_set_batch(int s) { ctx().batch_req = s < 0 ? 0 : s; }
+extern "C" void fwiedu_set_tile_order(int zfast) { ctx().tile_req.zfast = zfast ? 1 : 0; }
+
 extern "C" void fwiedu_set_tile(int tz, int spb)
 {
     Ctx &c = ctx();
@@ -447,7 +449,23 @@ extern "C" void fwiedu_set_tile(int tz, int spb)
         fail("<error>: fwiedu_set_tile: tile height 32 or 64, shots per block 1, 2, 4 or 8");
     c.tile_req.tz = tz;
     c.tile_req.spb = spb;
+    c.tile_req.kind = 0;
+    c.tile_req.txm = 0;
 }
+extern "C" void fwiedu_set_march(int txm, int spb, int pd)
+{
+    Ctx &c = ctx();
+    if ((txm != 16 && txm != 32 && txm != 64) || (spb != 1 && spb != 2 && spb != 4) ||
+        (pd != 1 && pd != 2 && pd != 4))
+        fail("<error>: fwiedu_set_march: txm 16, 32 or 64 columns, shots per block 1, 2 or 4, "
+             "prefetch depth 1, 2 or 4");
+    c.tile_req.tz = 64;
+    c.tile_req.kind = 1;
+    c.tile_req.txm = txm;
+    c.tile_req.spb = spb;
+    c.tile_req.pd = pd;
+}
+extern "C" void fwiedu_set_xcd_remap(int on) { ctx().tile_req.remap = on ? 1 : 0; }
 extern "C" void fwiedu_set_memory_fraction(double f)
 {
     ctx().memfrac = (f > 0.0 && f <= 0.98) ? f : 0.85;
@@ -474,6 +492,11 @@ extern "C" double fwiedu_stat(const char *name)
     if (n == "batch") return (double)s.batch;
     if (n == "tile_tz") return (double)s.tile_tz;
     if (n == "tile_spb") return (double)s.tile_spb;
+    if (n == "tile_kind") return (double)s.tile_kind;
+    if (n == "tile_txm") return (double)s.tile_txm;
+    if (n == "tile_zfast") return (double)s.tile_zfast;
+    if (n == "tile_pd") return (double)s.tile_pd;
+    if (n == "tile_remap") return (double)s.tile_remap;
     if (n == "h2d_bytes") return s.h2d_bytes;
     if (n == "d2h_bytes") return s.d2h_bytes;
     if (n == "device_bytes") {

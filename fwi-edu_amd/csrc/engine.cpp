@@ -197,35 +197,63 @@ void collect_timing()
 // ---------------------------------------------------------------------------
 // Launch-shape autotuner.  For each new (geometry, shots-per-launch) it times
 // every tile height (32, 64 rows) and shots-per-block (1, 2, 4, 8) on the
-// reconstruction sweep over zero fields (output into a work buffer that the
-// next pass overwrites), and keeps the fastest.
+// reconstruction sweep over three distinct zero-filled work fields of the batch
+// (the output is zero again, so nothing the passes read changes), and keeps
+// the fastest.
 
-FdTile autotune_tile(const FdArgs &base, float *out, const float *zero)
+FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const float *prev)
 {
     Ctx &c = g_ctx;
-    if (c.tile_req.tz > 0) return c.tile_req;
+    if (c.tile_req.tz > 0) {
+        if (c.tile_req.kind == 1 && march_lds_bytes(base, c.tile_req) > 65536)
+            fail("<error>: column march with %d shots per block needs more than 64 KiB of LDS "
+                 "for this grid height",
+                 c.tile_req.spb);
+        return c.tile_req;
+    }
     const auto key = std::make_tuple(base.Nx, base.ld, base.ixs, base.ixe, base.izs, base.ize,
                                      base.fs, base.nshot);
     auto it = c.tuned.find(key);
     if (it != c.tuned.end()) return it->second;
     FdArgs a = base;
     a.out = out;
-    a.cur = zero;
-    a.prev = zero;
+    a.cur = cur;
+    a.prev = prev;
     a.src_cell = nullptr;
     a.acc = nullptr;
     a.acc_on = 0;
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
+    std::vector<FdTile> cand;
+    for (int tz : {64, 32})
+        for (int spb : {1, 2, 4, 8})
+            for (int zf : {0, 1})
+                for (int rm : {0, 1}) {
+                    if (spb > 1 && spb > base.nshot) continue;
+                    FdTile t;
+                    t.tz = tz;
+                    t.spb = spb;
+                    t.zfast = zf;
+                    t.remap = rm;
+                    cand.push_back(t);
+                }
+    for (int txm : {16, 64})
+        for (int spb : {1, 2})
+            for (int pd : {4}) {
+                if (spb > 1 && spb > base.nshot) continue;
+                FdTile t;
+                t.kind = 1;
+                t.txm = txm;
+                t.spb = spb;
+                t.pd = pd;
+                if (march_lds_bytes(base, t) > 65536) continue;
+                cand.push_back(t);
+            }
     FdTile best;
     float best_ms = 1e30f;
-    for (int tz : {64, 32})
-        for (int spb : {1, 2, 4, 8}) {
-            if (spb > 1 && spb > base.nshot) continue;
-            FdTile t;
-            t.tz = tz;
-            t.spb = spb;
+    for (const FdTile &t : cand) {
+        {
             launch_fd(FD_RECON, a, t, c.st);
             HIPCHECK(hipEventRecord(e0, c.st));
             for (int r = 0; r < 3; ++r) launch_fd(FD_RECON, a, t, c.st);
@@ -238,6 +266,7 @@ FdTile autotune_tile(const FdArgs &base, float *out, const float *zero)
                 best = t;
             }
         }
+    }
     HIPCHECK(hipEventDestroy(e0));
     HIPCHECK(hipEventDestroy(e1));
     c.tuned[key] = best;
@@ -414,6 +443,7 @@ static FdArgs base_args(const Geom &g, const float *coef, int S, long stride)
     a.invdzdz = g.invdzdz;
     a.invdtdt = g.invdtdt;
     a.dt = g.dt;
+    a.remap = 0;
     return a;
 }
 
@@ -557,9 +587,14 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
     }
     b.cells_per_launch = interior_cells(g) * S;
     b.tile = autotune_tile(base_args(g, coef, S, b.stride), b.F(BatchPlan::A2),
-                           b.F(BatchPlan::ZERO));
+                           b.F(BatchPlan::A0), b.F(BatchPlan::A1));
     c.stats.tile_tz = b.tile.tz;
     c.stats.tile_spb = b.tile.spb;
+    c.stats.tile_kind = b.tile.kind;
+    c.stats.tile_zfast = b.tile.zfast;
+    c.stats.tile_pd = b.tile.pd;
+    c.stats.tile_remap = b.tile.remap;
+    c.stats.tile_txm = b.tile.txm;
 }
 
 // propagation.c:539-608.  With save_ckpt the fields u(kd-1), u(kd) land in

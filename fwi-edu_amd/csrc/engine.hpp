@@ -46,7 +46,7 @@ struct Stats {
     long shots_per_launch = 0;
     double device_bytes = 0;
     double h2d_bytes = 0, d2h_bytes = 0;
-    int tile_tz = 0, tile_spb = 0;
+    int tile_tz = 0, tile_spb = 0, tile_kind = 0, tile_txm = 0, tile_zfast = 0, tile_pd = 0, tile_remap = 0;
 };
 
 struct Ctx {
@@ -59,6 +59,7 @@ struct Ctx {
     long recon_launches = 0;
     int ckpt_bits = 32;
     FdTile tile_req = {0, 1};     // tz == 0: autotune
+    int xcd_remap = 0;
     std::map<std::tuple<int, int, int, int, int, int, int, int>, FdTile> tuned;
     Stats stats;
     std::vector<hipEvent_t> ev;   // pool of events for sampled kernel timing
@@ -102,7 +103,7 @@ void adjoint_source_device(const Record2D *r, float **vp_full, float freq, int f
 
 // Sampled timing of stencil launches.
 void fd_launch(int mode, const FdArgs &a, long cells, FdTile tile);
-FdTile autotune_tile(const FdArgs &base, float *out, const float *zero);
+FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const float *prev);
 void collect_timing();
 
 }  // namespace fwiedu

@@ -65,14 +65,25 @@ template <int MODE, int TZ, int SPB>
 __global__ __launch_bounds__(256) void fd_kernel(FdArgs a)
 {
     using T = Tile<TZ>;
+    constexpr int NB = (T::HX + NWAVE * T::CPW - 1) / (NWAVE * T::CPW);  // body loads / thread
+    static_assert(T::HX * 6 <= 256, "one halo load per thread");
     __shared__ float t[T::HX][T::HZ];
     const float C1 = 0.77793115f, C2 = -0.17388691f, C3 = 0.02338713f;
     const float D0 = -2.8215452f, D1 = 1.57663238f, D2 = -0.18347238f, D3 = 0.01761260f;
 
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int L = a.remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int tile = L / a.ngroups, grp = L - tile * a.ngroups;
-    const int xt = tile % a.xtiles, zt = tile / a.xtiles;
-    const int x0 = a.ixs + xt * T::TX, z0 = a.izs + zt * TZ;
+    int xt, zt;
+    if (a.zfast) {
+        zt = tile % a.ztiles;
+        xt = tile / a.ztiles;
+    } else {
+        xt = tile % a.xtiles;
+        zt = tile / a.xtiles;
+    }
+    // tile rows start on 128-byte lines (zbase = izs rounded down to 32 rows);
+    // rows outside [izs, ize] only stage data
+    const int x0 = a.ixs + xt * T::TX, z0 = ((a.izs >> 5) << 5) + zt * TZ;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int zl = lane % TZ, xl = lane / TZ;
     const int ld = a.ld;
@@ -86,7 +97,7 @@ __global__ __launch_bounds__(256) void fd_kernel(FdArgs a)
 #pragma unroll
     for (int j = 0; j < CXP; ++j) {
         const int ix = x0 + w * T::CPW * CXP + j * T::CPW + xl, iz = z0 + zl;
-        ok[j] = ix <= a.ixe && iz <= a.ize;
+        ok[j] = ix <= a.ixe && iz >= a.izs && iz <= a.ize;
         cell[j] = ix * ld + iz;
         if (ok[j]) {
             k1[j] = cf[CO_C1 * fsz + cell[j]];
@@ -98,35 +109,58 @@ __global__ __launch_bounds__(256) void fd_kernel(FdArgs a)
         }
     }
 
-    for (int si = 0; si < SPB; ++si) {
-        const int s = grp * SPB + si;
-        if (s >= a.nshot) break;  // uniform across the block
+    // the LDS slots this thread stages: NB body slots (column c, row zl+3) and
+    // at most one halo slot (rows 0..2 and TZ+3..TZ+5)
+    int boff[NB];
+    bool bok[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int c = w * T::CPW + xl + b * NWAVE * T::CPW;
+        const int ix = x0 - 3 + c, iz = z0 + zl;
+        bok[b] = c < T::HX;
+        boff[b] = (bok[b] && ix < a.Nx && iz < ld) ? ix * ld + iz : -1;
+    }
+    const bool hok = threadIdx.x < T::HX * 6;
+    const int hc = threadIdx.x / 6, hh = threadIdx.x - hc * 6;
+    const int hr = hh < 3 ? hh : TZ + hh;
+    int hoff = -1;
+    {
+        const int ix = x0 - 3 + hc, iz = z0 - 3 + hr;
+        if (hok && ix < a.Nx && iz >= 0 && iz < ld) hoff = ix * ld + iz;
+    }
+
+    // loads of one shot, issued together; the next shot's are issued before the
+    // current one is computed so they are in flight during the arithmetic
+    float stg[NB], hal = 0.f, q0[CXP];
+    auto issue = [&](int s) {
         const long so = (long)s * a.shot_stride;
         const float *__restrict__ p1 = a.cur + so;
         const float *__restrict__ p0 = a.prev + so;
-        float *__restrict__ p2 = a.out + so;
-
-        // stage p1: columns x0-3 .. x0+TX+2, rows z0-3 .. z0+TZ+2 (x0, z0 >= 3)
-        for (int c = w * T::CPW + xl; c < T::HX; c += NWAVE * T::CPW) {
-            const int ix = x0 - 3 + c, iz = z0 + zl;
-            float v = 0.f;
-            if (ix < a.Nx && iz < ld) v = p1[(long)ix * ld + iz];
-            t[c][zl + 3] = v;
-        }
-        for (int k = threadIdx.x; k < T::HX * 6; k += 256) {
-            const int c = k / 6, h = k - c * 6;
-            const int r = h < 3 ? h : TZ + h;
-            const int ix = x0 - 3 + c, iz = z0 - 3 + r;
-            float v = 0.f;
-            if (ix < a.Nx && iz < ld) v = p1[(long)ix * ld + iz];
-            t[c][r] = v;
-        }
-        float q0[CXP];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) stg[b] = boff[b] >= 0 ? p1[boff[b]] : 0.f;
+        hal = hoff >= 0 ? p1[hoff] : 0.f;
 #pragma unroll
         for (int j = 0; j < CXP; ++j) q0[j] = ok[j] ? p0[cell[j]] : 0.f;
+    };
+
+    const int s0 = grp * SPB;
+    if (s0 < a.nshot) issue(s0);
+    for (int si = 0; si < SPB; ++si) {
+        const int s = s0 + si;
+        if (s >= a.nshot) break;  // uniform across the block
+        const long so = (long)s * a.shot_stride;
+        float *__restrict__ p2 = a.out + so;
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+            if (bok[b]) t[w * T::CPW + xl + b * NWAVE * T::CPW][zl + 3] = stg[b];
+        if (hok) t[hc][hr] = hal;
+        float p0c[CXP];
+#pragma unroll
+        for (int j = 0; j < CXP; ++j) p0c[j] = q0[j];
         int scell = -1;
         if (MODE != FD_ADJ && a.src_cell) scell = a.src_cell[s];
         __syncthreads();
+        if (si + 1 < SPB && s + 1 < a.nshot) issue(s + 1);
 
 #pragma unroll
         for (int j = 0; j < CXP; ++j) {
@@ -144,7 +178,7 @@ __global__ __launch_bounds__(256) void fd_kernel(FdArgs a)
             const float dxx = (D0 * pc + D1 * (t[c + 1][r] + t[c - 1][r]) +
                                D2 * (t[c + 2][r] + t[c - 2][r]) +
                                D3 * (t[c + 3][r] + t[c - 3][r])) * a.invdxdx;
-            const float p0v = q0[j];
+            const float p0v = p0c[j];
             const float v = k1[j] * pc + k2[j] * p0v +
                             kA[j] * (kx[j] * dpdx + kz[j] * dpdz + kr[j] * (dxx + dzz));
 
@@ -188,20 +222,232 @@ __global__ __launch_bounds__(256) void fd_kernel(FdArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Column-marching stencil.
+//
+// A block owns a strip of rows (one thread per row, the strip aligned down to a
+// 128-byte line so every column access is whole lines) and marches along x
+// over mx_len columns for SPB shots.  p1 columns pass through an 8-slot LDS
+// ring: each column is loaded from HBM once per block (plus 6 columns at the
+// start of the march), the z-neighbours come from the ring column, the
+// x-neighbours from the adjacent ring slots.  One barrier per column: the slot
+// written for column ix+3 held column ix-5, which no thread still computing
+// column ix-1 reads.  The loads of column ix+1 are issued before column ix is
+// computed.  Arithmetic and epilogue are those of fd_kernel.
+template <int MODE, int SPB, int PD>
+__global__ __launch_bounds__(1024) void fd_march_kernel(FdArgs a)
+{
+    extern __shared__ float ring[];  // [SPB][8][ZB]
+    const float C1 = 0.77793115f, C2 = -0.17388691f, C3 = 0.02338713f;
+    const float D0 = -2.8215452f, D1 = 1.57663238f, D2 = -0.18347238f, D3 = 0.01761260f;
+    const int ZB = blockDim.x, tid = threadIdx.x;
+    // block order: shot group fastest, so the groups of one strip run together
+    // and share its coefficient lines in L2
+    int b = a.remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int grp = b % a.ngroups;
+    b /= a.ngroups;
+    const int zs = b % a.mz_nseg, xs = b / a.mz_nseg;
+    const int xc0 = a.ixs + xs * a.mx_len;
+    const int xc1 = min(a.ixe, xc0 + a.mx_len - 1);
+    const int cz0 = a.izs + zs * a.mz_len;
+    const int cz1 = min(a.ize, cz0 + a.mz_len - 1);
+    const int zb = ((cz0 - 3) >> 5) << 5;
+    const int rr = zb + tid;
+    const bool rok = rr < a.ld;
+    const bool comp = rr >= cz0 && rr <= cz1;
+    const int s0 = grp * SPB;
+    const int ld = a.ld;
+    const long fsz = (long)a.Nx * ld;
+    const float *__restrict__ cf = a.coef;
+    auto R = [&](int si, int col) -> float & { return ring[(si * 8 + (col & 7)) * ZB + tid]; };
+    auto Rz = [&](int si, int col, int dz) -> float { return ring[(si * 8 + (col & 7)) * ZB + tid + dz]; };
+
+    // prologue: columns xc0-3 .. xc0+2 (xc0 >= 3)
+#pragma unroll
+    for (int si = 0; si < SPB; ++si) {
+        const int s = s0 + si;
+        if (s >= a.nshot) break;
+        const float *__restrict__ p1 = a.cur + (long)s * a.shot_stride;
+        for (int c = xc0 - 3; c < xc0 + 3; ++c) R(si, c) = (c < a.Nx && rok) ? p1[(long)c * ld + rr] : 0.f;
+    }
+    // register queue of PD columns in flight: slot q holds column xc0 + q (mod PD)
+    float n1[PD][SPB], n0[PD][SPB], nk[PD][6];
+    auto issue = [&](int ix, int q) {
+#pragma unroll
+        for (int si = 0; si < SPB; ++si) {
+            const int s = s0 + si;
+            n1[q][si] = 0.f;
+            n0[q][si] = 0.f;
+            if (s >= a.nshot) continue;
+            const long so = (long)s * a.shot_stride;
+            const int c = ix + 3;
+            if (c < a.Nx && rok) n1[q][si] = a.cur[so + (long)c * ld + rr];
+            if (comp) n0[q][si] = a.prev[so + (long)ix * ld + rr];
+        }
+        if (comp) {
+            const long i = (long)ix * ld + rr;
+            nk[q][0] = cf[CO_C1 * fsz + i];
+            nk[q][1] = cf[CO_C2 * fsz + i];
+            nk[q][2] = cf[CO_A * fsz + i];
+            nk[q][3] = cf[CO_IRX * fsz + i];
+            nk[q][4] = cf[CO_IRZ * fsz + i];
+            nk[q][5] = cf[CO_IR * fsz + i];
+        }
+    };
+#pragma unroll
+    for (int q = 0; q < PD; ++q)
+        if (xc0 + q <= xc1) issue(xc0 + q, q);
+
+    for (int ix0 = xc0; ix0 <= xc1; ix0 += PD) {
+#pragma unroll
+        for (int q = 0; q < PD; ++q) {
+            const int ix = ix0 + q;
+            if (ix > xc1) break;  // uniform across the block
+            float p0c[SPB], k[6];
+#pragma unroll
+            for (int si = 0; si < SPB; ++si) {
+                R(si, ix + 3) = n1[q][si];
+                p0c[si] = n0[q][si];
+            }
+#pragma unroll
+            for (int m = 0; m < 6; ++m) k[m] = nk[q][m];
+            __syncthreads();
+            if (ix + PD <= xc1) issue(ix + PD, q);
+            if (!comp) continue;
+            const int i = ix * ld + rr;
+#pragma unroll
+            for (int si = 0; si < SPB; ++si) {
+                const int s = s0 + si;
+                if (s >= a.nshot) break;
+                const long so = (long)s * a.shot_stride;
+                float *__restrict__ p2 = a.out + so;
+                const float pc = R(si, ix);
+                const float dpdz = (C1 * (Rz(si, ix, 1) - Rz(si, ix, -1)) +
+                                    C2 * (Rz(si, ix, 2) - Rz(si, ix, -2)) +
+                                    C3 * (Rz(si, ix, 3) - Rz(si, ix, -3))) * a.invdz;
+                const float dpdx = (C1 * (R(si, ix + 1) - R(si, ix - 1)) +
+                                    C2 * (R(si, ix + 2) - R(si, ix - 2)) +
+                                    C3 * (R(si, ix + 3) - R(si, ix - 3))) * a.invdx;
+                const float dzz = (D0 * pc + D1 * (Rz(si, ix, 1) + Rz(si, ix, -1)) +
+                                   D2 * (Rz(si, ix, 2) + Rz(si, ix, -2)) +
+                                   D3 * (Rz(si, ix, 3) + Rz(si, ix, -3))) * a.invdzdz;
+                const float dxx = (D0 * pc + D1 * (R(si, ix + 1) + R(si, ix - 1)) +
+                                   D2 * (R(si, ix + 2) + R(si, ix - 2)) +
+                                   D3 * (R(si, ix + 3) + R(si, ix - 3))) * a.invdxdx;
+                const float p0v = p0c[si];
+                const float v =
+                    k[0] * pc + k[1] * p0v + k[2] * (k[3] * dpdx + k[4] * dpdz + k[5] * (dxx + dzz));
+                int scell = -1;
+                if (MODE != FD_ADJ && a.src_cell) scell = a.src_cell[s];
+                float val = v;
+                if (MODE == FD_FWD && i == scell) {
+                    const double srcv = 1.0 * (double)a.src_wav[(long)s * a.nt + a.it] *
+                                        (double)cf[CO_K * fsz + i] * (double)a.dt * (double)a.dt *
+                                        (double)a.invdx * (double)a.invdz;
+                    val = (float)((double)v + srcv);
+                }
+                float stored = val;
+                if (a.fs) {
+                    const int kk = rr - a.pml;
+                    if (kk == 0)
+                        stored = -0.0f;
+                    else if (kk <= a.pml)
+                        p2[(long)ix * ld + (a.pml - kk)] = -val;
+                }
+                if (MODE == FD_RECON && i == scell) {
+                    stored = stored + a.src_wav[(long)s * a.nt + a.it] * cf[CO_K * fsz + i] * a.dt *
+                                          a.dt * a.invdx * a.invdz;
+                }
+                p2[i] = stored;
+                if (MODE != FD_RECON && a.acc_on) {
+                    const int px = ix - a.pml, pz = rr - a.pml;
+                    if (px >= 0 && px < a.nx && pz >= 0 && pz < a.nz) {
+                        double *acc = a.acc + (long)s * a.acc_stride + (long)px * a.nz + pz;
+                        if (MODE == FD_FWD) {
+                            const float qq = (stored - 2.0f * pc + p0v) * a.invdtdt;
+                            *acc += (double)(qq * qq);
+                        } else {
+                            const float qq = (stored - 2.0f * pc + p0v) * a.invdtdt * a.u[so + i];
+                            *acc += (double)qq;
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Strip plan of the marching kernel: z segments of at most mz_len rows whose
+// line-aligned load span (rows cz0-3 .. cz1+3) fits one block of ZB threads.
+static void march_plan(FdArgs &a, int mx_len, int &zb_threads)
+{
+    const int H = a.ize - a.izs + 1;
+    int nseg = 1;
+    while ((H + nseg - 1) / nseg + 6 + 31 > 1024) ++nseg;
+    a.mz_nseg = nseg;
+    a.mz_len = (H + nseg - 1) / nseg;
+    zb_threads = (a.mz_len + 6 + 31 + 63) / 64 * 64;
+    a.mx_len = mx_len;
+    a.mx_nseg = (a.ixe - a.ixs + 1 + mx_len - 1) / mx_len;
+}
+
+int march_lds_bytes(const FdArgs &a0, FdTile tl)
+{
+    FdArgs a = a0;
+    int zb = 0;
+    march_plan(a, tl.txm, zb);
+    return tl.spb * 8 * zb * (int)sizeof(float);
+}
+
+template <int MODE, int SPB, int PD>
+static void launch_march_t(const FdArgs &a0, int mx_len, hipStream_t st)
+{
+    FdArgs a = a0;
+    int zb = 0;
+    march_plan(a, mx_len, zb);
+    a.ngroups = (a.nshot + SPB - 1) / SPB;
+    const long nb = (long)a.mx_nseg * a.mz_nseg * a.ngroups;
+    const size_t lds = (size_t)SPB * 8 * zb * sizeof(float);
+    fd_march_kernel<MODE, SPB, PD><<<dim3((unsigned)nb), zb, lds, st>>>(a);
+}
+
+template <int MODE, int PD>
+static void launch_march_spb(const FdArgs &a, FdTile tl, hipStream_t st)
+{
+    switch (tl.spb) {
+    case 1: launch_march_t<MODE, 1, PD>(a, tl.txm, st); break;
+    case 2: launch_march_t<MODE, 2, PD>(a, tl.txm, st); break;
+    default: launch_march_t<MODE, 4, PD>(a, tl.txm, st); break;
+    }
+}
+
+template <int MODE>
+static void launch_march_mode(const FdArgs &a, FdTile tl, hipStream_t st)
+{
+    if (tl.pd >= 4)
+        launch_march_spb<MODE, 4>(a, tl, st);
+    else if (tl.pd == 2)
+        launch_march_spb<MODE, 2>(a, tl, st);
+    else
+        launch_march_spb<MODE, 1>(a, tl, st);
+}
+
 template <int MODE, int TZ, int SPB>
 static void launch_fd_t(const FdArgs &a0, hipStream_t st)
 {
     FdArgs a = a0;
     a.xtiles = (a.ixe - a.ixs + 1 + Tile<TZ>::TX - 1) / Tile<TZ>::TX;
-    a.ztiles = (a.ize - a.izs + 1 + TZ - 1) / TZ;
+    a.ztiles = (a.ize - ((a.izs >> 5) << 5) + 1 + TZ - 1) / TZ;
     a.ngroups = (a.nshot + SPB - 1) / SPB;
     const long nb = (long)a.xtiles * a.ztiles * a.ngroups;
     fd_kernel<MODE, TZ, SPB><<<dim3((unsigned)nb), 256, 0, st>>>(a);
 }
 
 template <int MODE, int TZ>
-static void launch_fd_spb(const FdArgs &a, int spb, hipStream_t st)
+static void launch_fd_spb(const FdArgs &a0, int spb, int zfast, hipStream_t st)
 {
+    FdArgs a = a0;
+    a.zfast = zfast;
     switch (spb) {
     case 1: launch_fd_t<MODE, TZ, 1>(a, st); break;
     case 2: launch_fd_t<MODE, TZ, 2>(a, st); break;
@@ -214,13 +460,24 @@ template <int MODE>
 static void launch_fd_mode(const FdArgs &a, FdTile tl, hipStream_t st)
 {
     if (tl.tz == 32)
-        launch_fd_spb<MODE, 32>(a, tl.spb, st);
+        launch_fd_spb<MODE, 32>(a, tl.spb, tl.zfast, st);
     else
-        launch_fd_spb<MODE, 64>(a, tl.spb, st);
+        launch_fd_spb<MODE, 64>(a, tl.spb, tl.zfast, st);
 }
 
-void launch_fd(int mode, const FdArgs &a, FdTile tl, hipStream_t st)
+void launch_fd(int mode, const FdArgs &a0, FdTile tl, hipStream_t st)
 {
+    FdArgs a = a0;
+    a.remap = tl.remap;
+    if (tl.kind == 1) {
+        if (mode == FD_FWD)
+            launch_march_mode<FD_FWD>(a, tl, st);
+        else if (mode == FD_RECON)
+            launch_march_mode<FD_RECON>(a, tl, st);
+        else
+            launch_march_mode<FD_ADJ>(a, tl, st);
+        return;
+    }
     if (mode == FD_FWD)
         launch_fd_mode<FD_FWD>(a, tl, st);
     else if (mode == FD_RECON)

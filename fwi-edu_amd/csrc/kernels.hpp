@@ -31,10 +31,17 @@ struct Geom {
 enum { CO_C1 = 0, CO_C2, CO_A, CO_IRX, CO_IRZ, CO_IR, CO_K, CO_N };
 constexpr int CO_STREAMED = 6;
 
-// Stencil launch shape: tile height (32 or 64 rows) and shots per block (1,2,4,8).
+// Stencil launch shape.  kind 0: square tiles of tz (32 or 64) rows, spb shots
+// per block (1, 2, 4, 8).  kind 1: column march over txm columns per block,
+// spb shots per block (1, 2, 4), pd columns of loads in flight.
 struct FdTile {
     int tz = 64;
     int spb = 1;
+    int kind = 0;
+    int txm = 0;
+    int zfast = 0;
+    int pd = 1;  // columns prefetched ahead by the march (1, 2, 4)
+    int remap = 0;  // XCD-aware block order
 };
 
 struct FdArgs {
@@ -47,6 +54,9 @@ struct FdArgs {
     int Nx, ld, ixs, ixe, izs, ize, pml, fs, nx, nz;
     float invdx, invdz, invdxdx, invdzdz, invdtdt, dt;
     int xtiles, ztiles, ngroups;
+    int remap;              // XCD-aware block order (1) or hardware order (0)
+    int zfast;              // tile order: z-neighbours adjacent (1) or x-neighbours (0)
+    int mx_len, mx_nseg, mz_len, mz_nseg;  // column-march plan
     // source (FD_FWD / FD_RECON)
     const int *src_cell;    // [nshot] flat padded index or -1
     const float *src_wav;   // [nshot][nt]
@@ -60,6 +70,7 @@ struct FdArgs {
 
 void launch_fd(int mode, const FdArgs &a, FdTile tile, hipStream_t st);
 int fd_tile_rows_padded(int tz, int izs, int ize);
+int march_lds_bytes(const FdArgs &a, FdTile tile);
 
 // ---- set-up --------------------------------------------------------------
 // window model [nx][nz] (contiguous) -> padded [Nx][ld] by edge replication

@@ -147,6 +147,17 @@ void fwiedu_set_batch(int shots_per_launch);
  * (1, 2, 4, 8).  tz = 0 (default) lets the library's autotuner time every shape
  * once per geometry and keep the fastest. */
 void fwiedu_set_tile(int tz, int spb);
+/* Tile order of a forced tile shape: 1 puts vertically adjacent tiles next to
+ * each other in launch order, 0 horizontally adjacent ones. */
+void fwiedu_set_tile_order(int zfast);
+/* Force the column-marching stencil: txm (16, 32, 64) columns per block, spb
+ * (1, 2, 4) shots per block, pd (1, 2, 4) columns of loads kept in flight.
+ * fwiedu_set_tile(0, 0) returns to the autotuner. */
+void fwiedu_set_march(int txm, int spb, int pd);
+/* Block order of a forced launch shape: 1 numbers stencil blocks so that each
+ * XCD works on one contiguous range of tiles, 0 keeps the hardware's
+ * round-robin order.  (The autotuner tries both.) */
+void fwiedu_set_xcd_remap(int on);
 /* Fraction of free device memory the engine may use (default 0.85). */
 void fwiedu_set_memory_fraction(double frac);
 /* Sample every `every`-th reconstruction-sweep launch (the stencil with no

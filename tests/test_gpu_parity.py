@@ -301,3 +301,29 @@ def test_c1_subset_parity(L, oracles):
     assert e_rec <= 1e-5
     assert rel(g_g, g_o) <= 1e-4
     assert abs(f_g - f_o) <= 1e-5 * f_o
+
+
+@pytest.mark.parametrize("fs", [0, 1])
+def test_every_stencil_shape_bit_identical(L, oracles, fs):
+    """Every launch shape the autotuner may pick (square tiles of 32/64 rows in
+    both tile orders, column marches of 16/32/64 columns, 1-8 shots per block)
+    gives the same bits as the strict-order oracle."""
+    w = synth.workload("tiny", nshot=5, freesurface=fs, shot_depth_cells=1, nx=70)
+    say(f"case: stencil shapes tiny fs={fs} 5 shots")
+    obs = observed(oracles["f32s"], w, False)
+    pre_o, g_o = gradient(oracles["f32s"], w, obs, False)
+    shapes = [("tile", tz, spb, zf) for tz in (32, 64) for spb in (1, 2, 4, 8) for zf in (0, 1)]
+    shapes += [("march", txm, spb, pd) for txm in (16, 32, 64) for spb in (1, 2, 4) for pd in (1, 2, 4)]
+    try:
+        for kind, p, spb, zf in shapes:
+            if kind == "tile":
+                L.fwiedu_set_tile(p, spb)
+                L.fwiedu_set_tile_order(zf)
+            else:
+                L.fwiedu_set_march(p, spb, zf)
+            pre_g, g_g = gradient(L, w, obs, True)
+            assert np.array_equal(g_g, g_o), (kind, p, spb, zf)
+            assert np.array_equal(recs(pre_g), recs(pre_o)), (kind, p, spb, zf)
+    finally:
+        L.fwiedu_set_tile(0, 0)
+        L.fwiedu_set_tile_order(0)
