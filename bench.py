@@ -67,10 +67,32 @@ def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
     stats = {k: flib.stat(k) for k in ("fd_launches", "fd_timed", "fd_ms_avg", "fd_cells_per_launch",
                                        "fd_shots_per_launch", "batch", "device_bytes", "tile_tz",
                                        "tile_spb", "tile_kind", "tile_zfast", "tile_remap",
-                                       "tile_txm", "tile_pd")}
+                                       "tile_txm", "tile_pd", "tile_wpb", "tile_zrun", "layout")}
     L.fwiedu_session_destroy(sess)
     L.fwiedu_set_batch(0)
     return times, stats, misfit
+
+
+KERNELS = {0: "fd_kernel (square tiles, per-shot arenas)",
+           1: "fd_march_kernel (column march, per-shot arenas)",
+           2: "fd_lanes_kernel (shots in lanes)",
+           3: "fd_lanes_lds_kernel (shots in lanes, LDS row exchange)"}
+
+
+def stencil_shape(st):
+    k = int(st["tile_kind"])
+    d = {"kernel": KERNELS[k], "layout": "shots in lanes" if st["layout"] else "per-shot arenas",
+         "xcd_order": int(st["tile_remap"])}
+    if k == 0:
+        d.update(rows=int(st["tile_tz"]), shots_per_block=int(st["tile_spb"]),
+                 z_fast_order=int(st["tile_zfast"]))
+    elif k == 1:
+        d.update(cols=int(st["tile_txm"]), shots_per_block=int(st["tile_spb"]),
+                 prefetch=int(st["tile_pd"]))
+    else:
+        d.update(cols_per_block=int(st["tile_wpb"]), rows_per_wave=int(st["tile_zrun"]),
+                 prefetch_rows=int(st["tile_pd"]))
+    return d
 
 
 def copy_calibration(mb):
@@ -198,10 +220,7 @@ def main():
                                f"{w.freesurface}, checkpointed adjoint gradient, precondition "
                                f"{w.precondition}, model and records resident in HBM",
                    "shots_per_launch": S, "batch": int(st["batch"]),
-                   "tile": {"kernel": "column march" if st["tile_kind"] == 1 else "square tiles",
-                            "rows": int(st["tile_tz"]), "shots_per_block": int(st["tile_spb"]),
-                            "z_fast_order": int(st["tile_zfast"]), "xcd_order": int(st["tile_remap"]),
-                            "march_cols": int(st["tile_txm"]), "prefetch": int(st["tile_pd"])}},
+                   "stencil": stencil_shape(st)},
         "gcell_updates_per_s": shots_per_s * cu / 1e9,
         "cell_updates_per_gradient_shot": cu,
         "misfit": misfit,
@@ -209,7 +228,8 @@ def main():
         "batched_over_plain": (shots_per_s / plain["gradient_shots_per_s"]) if plain else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                     "kernel": "fd_kernel<FD_RECON> (reconstruction sweep)",
+                     "kernel": KERNELS[int(st["tile_kind"])].split(" ")[0] +
+                               "<FD_RECON> (reconstruction sweep)",
                      "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_ms * 1e3,
                      "launches_timed": int(st["fd_timed"])},
         "cpu_baseline": cpu,

@@ -466,6 +466,35 @@ extern "C" void fwiedu_set_march(int txm, int spb, int pd)
     c.tile_req.pd = pd;
 }
 extern "C" void fwiedu_set_xcd_remap(int on) { ctx().tile_req.remap = on ? 1 : 0; }
+extern "C" void fwiedu_set_lanes(int wpb, int zrun, int pd, int lds)
+{
+    Ctx &c = ctx();
+    const bool zok = zrun == 8 || zrun == 16 || zrun == 32 || zrun == 64 || zrun == 128 ||
+                     zrun == 256;
+    if (lds) {
+        if ((wpb != 4 && wpb != 8 && wpb != 16) || !zok || pd < 1 || pd > 2)
+            fail("<error>: fwiedu_set_lanes (LDS row exchange): 4, 8 or 16 columns per block, "
+                 "8 to 256 rows per block, 1 or 2 rows of loads in flight");
+    } else if ((wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8) || !zok || pd < 1 || pd > 3) {
+        fail("<error>: fwiedu_set_lanes: 1, 2, 4 or 8 columns per block, 8 to 256 rows per wave, "
+             "1 to 3 rows of loads in flight");
+    }
+    const int rm = c.tile_req.remap;
+    c.tile_req = FdTile();
+    c.tile_req.remap = rm;
+    c.tile_req.tz = 64;
+    c.tile_req.kind = lds ? 3 : 2;
+    c.tile_req.wpb = wpb;
+    c.tile_req.zrun = zrun;
+    c.tile_req.pd = pd;
+}
+extern "C" void fwiedu_set_layout(int layout)
+{
+    if (layout < -1 || layout > 1)
+        fail("<error>: fwiedu_set_layout: -1 (automatic), 0 (per-shot arenas) or 1 (shots in "
+             "lanes)");
+    ctx().layout_req = layout;
+}
 extern "C" void fwiedu_set_memory_fraction(double f)
 {
     ctx().memfrac = (f > 0.0 && f <= 0.98) ? f : 0.85;
@@ -497,6 +526,9 @@ extern "C" double fwiedu_stat(const char *name)
     if (n == "tile_zfast") return (double)s.tile_zfast;
     if (n == "tile_pd") return (double)s.tile_pd;
     if (n == "tile_remap") return (double)s.tile_remap;
+    if (n == "tile_wpb") return (double)s.tile_wpb;
+    if (n == "tile_zrun") return (double)s.tile_zrun;
+    if (n == "layout") return (double)s.layout;
     if (n == "h2d_bytes") return s.h2d_bytes;
     if (n == "d2h_bytes") return s.d2h_bytes;
     if (n == "device_bytes") {

@@ -195,16 +195,21 @@ void collect_timing()
 }
 
 // ---------------------------------------------------------------------------
-// Launch-shape autotuner.  For each new (geometry, shots-per-launch) it times
-// every tile height (32, 64 rows) and shots-per-block (1, 2, 4, 8) on the
-// reconstruction sweep over three distinct zero-filled work fields of the batch
+// Launch-shape autotuner.  For each new (geometry, shots-per-launch, layout)
+// it times the launch shapes of the layout (per-shot arenas: tile height,
+// shots per block, block order, column march; shots in lanes: columns per
+// block and rows per wave) on the reconstruction sweep over three distinct zero-filled work fields of the batch
 // (the output is zero again, so nothing the passes read changes), and keeps
 // the fastest.
 
 FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const float *prev)
 {
     Ctx &c = g_ctx;
+    const bool lanes = base.cell_stride > 1;
     if (c.tile_req.tz > 0) {
+        if ((c.tile_req.kind >= 2) != lanes)
+            fail("<error>: internal: stencil shape kind %d does not match the field layout",
+                 c.tile_req.kind);
         if (c.tile_req.kind == 1 && march_lds_bytes(base, c.tile_req) > 65536)
             fail("<error>: column march with %d shots per block needs more than 64 KiB of LDS "
                  "for this grid height",
@@ -212,7 +217,7 @@ FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const flo
         return c.tile_req;
     }
     const auto key = std::make_tuple(base.Nx, base.ld, base.ixs, base.ixe, base.izs, base.ize,
-                                     base.fs, base.nshot);
+                                     base.fs, base.nshot, base.cell_stride);
     auto it = c.tuned.find(key);
     if (it != c.tuned.end()) return it->second;
     FdArgs a = base;
@@ -226,10 +231,39 @@ FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const flo
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
     std::vector<FdTile> cand;
+    if (lanes) {
+        for (int wpb : {4, 8})
+            for (int zrun : {8, 16})
+                for (int rm : {0, 1})
+                    for (int pd : {1, 2}) {
+                        FdTile t;
+                        t.tz = 64;
+                        t.kind = 2;
+                        t.wpb = wpb;
+                        t.zrun = zrun;
+                        t.remap = rm;
+                        t.pd = pd;
+                        cand.push_back(t);
+                    }
+        for (int wpb : {8})
+            for (int zrun : {32})
+                for (int rm : {0})
+                    for (int pd : {2}) {
+                        FdTile t;
+                        t.tz = 64;
+                        t.kind = 3;
+                        t.wpb = wpb;
+                        t.zrun = zrun;
+                        t.remap = rm;
+                        t.pd = pd;
+                        cand.push_back(t);
+                    }
+    }
     for (int tz : {64, 32})
         for (int spb : {1, 2, 4, 8})
             for (int zf : {0, 1})
                 for (int rm : {0, 1}) {
+                    if (lanes) continue;
                     if (spb > 1 && spb > base.nshot) continue;
                     FdTile t;
                     t.tz = tz;
@@ -241,7 +275,7 @@ FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const flo
     for (int txm : {16, 64})
         for (int spb : {1, 2})
             for (int pd : {4}) {
-                if (spb > 1 && spb > base.nshot) continue;
+                if (lanes || (spb > 1 && spb > base.nshot)) continue;
                 FdTile t;
                 t.kind = 1;
                 t.txm = txm;
@@ -420,12 +454,15 @@ static Tables build_tables(const std::vector<ShotIn> &shots, const Geom &g, std:
 
 static long interior_cells(const Geom &g) { return (long)(g.ixe - g.ixs + 1) * (g.ize - g.izs + 1); }
 
-static FdArgs base_args(const Geom &g, const float *coef, int S, long stride)
+static FdArgs base_args(const Geom &g, const float *coef, int S, long shot_stride,
+                        long cell_stride)
 {
     FdArgs a;
     memset(&a, 0, sizeof(a));
     a.coef = coef;
-    a.shot_stride = stride;
+    a.shot_stride = shot_stride;
+    a.cell_stride = cell_stride;
+    a.acc_cell = 1;
     a.nshot = S;
     a.Nx = g.Nx;
     a.ld = g.ld;
@@ -516,13 +553,23 @@ static void rec_d2h(Record2D *r, const float *src)
 // ---------------------------------------------------------------------------
 // One batch of S shots on the device.
 
+// Field layouts.  Per-shot arenas: field k of shot s is the [Nx][ld] array at
+// arena + s*SS + k*KS (SS = fields per shot * fsz, KS = fsz, cell stride 1).
+// Shots in lanes: the batch is padded to Sp = 64*ceil(S/64) shots and field k
+// is [Nx][ld][Sp] at arena + k*KS (KS = fsz*Sp); cell i of shot s is at
+// s + i*Sp (SS = 1, CS = Sp).  Accumulators follow: [S][nx*nz] or
+// [nx*nz][Sp].
 struct BatchState {
     Geom g;
     const float *coef = nullptr;
     int S = 0, nt = 0;
     bool adjoint = false;
+    bool lanes = false;
+    int Sp = 0;                // shots the arena holds (S, or S padded to 64)
     Schedule *sc = nullptr;
-    long stride = 0;           // floats per shot arena
+    long KS = 0, SS = 0, CS = 1;  // field, shot and cell strides (floats)
+    long ASS = 0, ACS = 1;        // accumulator shot and cell strides (doubles)
+    long arena_floats = 0;
     float *arena = nullptr;
     int *cell_dev = nullptr;
     float *wav_dev = nullptr;
@@ -534,11 +581,12 @@ struct BatchState {
     long nacc = 0;
     long cells_per_launch = 0;
     FdTile tile;
-    float *F(int k) const { return arena + (long)k * g.fsz; }
+    float *F(int k) const { return arena + (long)k * KS; }
 };
 
 static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
-                          const std::vector<Record2D *> &recs, int nt, bool adjoint, Schedule &sc)
+                          const std::vector<Record2D *> &recs, int nt, bool adjoint, Schedule &sc,
+                          bool lanes)
 {
     Ctx &c = g_ctx;
     b.g = g;
@@ -547,12 +595,30 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
     b.nt = nt;
     b.adjoint = adjoint;
     b.sc = &sc;
+    b.lanes = lanes;
     const int S = b.S;
-    b.stride = fields_per_shot(sc, adjoint) * g.fsz;
-    b.arena = c.arena.as<float>((size_t)S * b.stride);
-    HIPCHECK(hipMemsetAsync(b.arena, 0, (size_t)S * b.stride * sizeof(float), c.st));
+    const long nf = fields_per_shot(sc, adjoint);
+    b.nacc = (long)g.nx * g.nz;
+    if (lanes) {
+        b.Sp = (S + 63) / 64 * 64;
+        b.KS = g.fsz * b.Sp;
+        b.SS = 1;
+        b.CS = b.Sp;
+        b.ASS = 1;
+        b.ACS = b.Sp;
+    } else {
+        b.Sp = S;
+        b.KS = g.fsz;
+        b.SS = nf * g.fsz;
+        b.CS = 1;
+        b.ASS = b.nacc;
+        b.ACS = 1;
+    }
+    b.arena_floats = nf * g.fsz * b.Sp;
+    b.arena = c.arena.as<float>((size_t)b.arena_floats);
+    HIPCHECK(hipMemsetAsync(b.arena, 0, (size_t)b.arena_floats * sizeof(float), c.st));
 
-    std::vector<int> cells(S);
+    std::vector<int> cells(b.Sp, -1);
     std::vector<float> wav((size_t)S * nt);
     for (int s = 0; s < S; ++s) {
         const Record2D *r = recs[s];
@@ -569,24 +635,23 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
         cells[s] = isx * g.ld + isz;
         memcpy(&wav[(size_t)s * nt], r->src, nt * sizeof(float));
     }
-    char *sb = c.src.as<char>(wav.size() * sizeof(float) + S * sizeof(int) + 256);
+    char *sb = c.src.as<char>(wav.size() * sizeof(float) + cells.size() * sizeof(int) + 256);
     b.wav_dev = reinterpret_cast<float *>(sb);
     b.cell_dev = reinterpret_cast<int *>(sb + wav.size() * sizeof(float));
     h2d(b.wav_dev, wav.data(), wav.size() * sizeof(float));
-    h2d(b.cell_dev, cells.data(), S * sizeof(int));
+    h2d(b.cell_dev, cells.data(), cells.size() * sizeof(int));
 
     std::vector<ShotIn> si(S);
     for (int s = 0; s < S; ++s) si[s] = {recs[s], nullptr, s};
     b.tb = build_tables(si, g, b.roff, b.rec_total);
     b.rec = c.rec.as<float>(b.rec_total + 64);
     HIPCHECK(hipMemsetAsync(b.rec, 0, b.rec_total * sizeof(float), c.st));
-    b.nacc = (long)g.nx * g.nz;
     if (adjoint) {
-        b.acc = c.acc.as<double>((size_t)S * b.nacc);
-        HIPCHECK(hipMemsetAsync(b.acc, 0, (size_t)S * b.nacc * sizeof(double), c.st));
+        b.acc = c.acc.as<double>((size_t)b.Sp * b.nacc);
+        HIPCHECK(hipMemsetAsync(b.acc, 0, (size_t)b.Sp * b.nacc * sizeof(double), c.st));
     }
     b.cells_per_launch = interior_cells(g) * S;
-    b.tile = autotune_tile(base_args(g, coef, S, b.stride), b.F(BatchPlan::A2),
+    b.tile = autotune_tile(base_args(g, coef, S, b.SS, b.CS), b.F(BatchPlan::A2),
                            b.F(BatchPlan::A0), b.F(BatchPlan::A1));
     c.stats.tile_tz = b.tile.tz;
     c.stats.tile_spb = b.tile.spb;
@@ -595,6 +660,9 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
     c.stats.tile_pd = b.tile.pd;
     c.stats.tile_remap = b.tile.remap;
     c.stats.tile_txm = b.tile.txm;
+    c.stats.tile_wpb = b.tile.wpb;
+    c.stats.tile_zrun = b.tile.zrun;
+    c.stats.layout = lanes ? 1 : 0;
 }
 
 // propagation.c:539-608.  With save_ckpt the fields u(kd-1), u(kd) land in
@@ -603,12 +671,13 @@ static void forward_pass(BatchState &b, bool save_ckpt, bool precond)
 {
     Ctx &c = g_ctx;
     const Schedule &sc = *b.sc;
-    FdArgs a = base_args(b.g, b.coef, b.S, b.stride);
+    FdArgs a = base_args(b.g, b.coef, b.S, b.SS, b.CS);
     a.src_cell = b.cell_dev;
     a.src_wav = b.wav_dev;
     a.nt = b.nt;
     a.acc = b.acc;
-    a.acc_stride = b.nacc;
+    a.acc_stride = b.ASS;
+    a.acc_cell = b.ACS;
     float *W[3] = {b.F(BatchPlan::A0), b.F(BatchPlan::A1), b.F(BatchPlan::A2)};
     const float *zero = b.F(BatchPlan::ZERO);
     const float *cur = zero, *prev = zero;
@@ -618,7 +687,7 @@ static void forward_pass(BatchState &b, bool save_ckpt, bool precond)
             out = b.F(BatchPlan::CK0 + sc.ckslot[it + 1]);
         else
             out = pick_free(W, cur, prev);
-        launch_record(b.tb.rcv, cur, b.stride, b.rec, it, c.st);
+        launch_record(b.tb.rcv, cur, b.SS, b.CS, b.rec, it, c.st);
         a.out = out;
         a.cur = cur;
         a.prev = prev;
@@ -653,16 +722,20 @@ static void adjoint_pass(BatchState &b)
     float *AW[3] = {b.F(BatchPlan::A0), b.F(BatchPlan::A1), b.F(BatchPlan::A2)};
     // the adjoint wavefield starts from zero (propagation.c:504-506); only the
     // three work fields of every arena are cleared, the checkpoints stay
-    for (int s = 0; s < b.S; ++s)
-        HIPCHECK(hipMemsetAsync(b.arena + (long)s * b.stride, 0, 3 * g.fsz * sizeof(float), c.st));
-    HIPCHECK(hipMemsetAsync(b.acc, 0, (size_t)b.S * b.nacc * sizeof(double), c.st));
-    FdArgs fr = base_args(g, b.coef, b.S, b.stride);
+    if (b.lanes)
+        HIPCHECK(hipMemsetAsync(b.arena, 0, 3 * b.KS * sizeof(float), c.st));
+    else
+        for (int s = 0; s < b.S; ++s)
+            HIPCHECK(hipMemsetAsync(b.arena + (long)s * b.SS, 0, 3 * g.fsz * sizeof(float), c.st));
+    HIPCHECK(hipMemsetAsync(b.acc, 0, (size_t)b.Sp * b.nacc * sizeof(double), c.st));
+    FdArgs fr = base_args(g, b.coef, b.S, b.SS, b.CS);
     fr.src_cell = b.cell_dev;
     fr.src_wav = b.wav_dev;
     fr.nt = nt;
-    FdArgs ad = base_args(g, b.coef, b.S, b.stride);
+    FdArgs ad = base_args(g, b.coef, b.S, b.SS, b.CS);
     ad.acc = b.acc;
-    ad.acc_stride = b.nacc;
+    ad.acc_stride = b.ASS;
+    ad.acc_cell = b.ACS;
     const float *K = b.coef + CO_K * g.fsz;
     const float *cur = AW[0], *prev = AW[1];
     for (int it = nt - 1; it >= 0; --it) {
@@ -686,8 +759,8 @@ static void adjoint_pass(BatchState &b)
                 where[t] = fout;
             }
         }
-        launch_inject(b.tb.inj, const_cast<float *>(cur), b.stride, K, b.rec, it, g.dt, g.invdx,
-                      g.invdz, c.st);
+        launch_inject(b.tb.inj, const_cast<float *>(cur), b.SS, b.CS, K, b.rec, it, g.dt,
+                      g.invdx, g.invdz, c.st);
         float *out = pick_free(AW, cur, prev);
         ad.out = out;
         ad.cur = cur;
@@ -713,19 +786,40 @@ static size_t per_shot_bytes(const Geom &g, const Schedule &sc, bool adjoint, lo
     return b;
 }
 
-static int choose_batch(int n, size_t per_shot)
+// Shots per launch and field layout.  Shots in lanes pads the batch to a
+// multiple of 64; it is used when at least 48 shots share a launch (below
+// that a third of the lanes or more would idle) unless a layout or a stencil
+// shape is forced.
+static int choose_batch(int n, size_t per_shot, bool &lanes)
 {
     Ctx &c = g_ctx;
-    if (c.batch_req > 0) return std::min(n, c.batch_req);
-    size_t fr = 0, tot = 0;
-    HIPCHECK(hipMemGetInfo(&fr, &tot));
-    // buffers the engine already holds are reused
-    size_t held = c.arena.cap + c.acc.cap + c.rec.cap + c.obs.cap + c.src.cap + c.misc[0].cap +
-                  c.misc[1].cap;
-    double budget = ((double)fr + (double)held) * c.memfrac;
-    long S = (long)(budget / (double)per_shot);
-    if (S < 1) S = 1;
-    return (int)std::min<long>(n, S);
+    long Smax;
+    if (c.batch_req > 0) {
+        Smax = c.batch_req;
+    } else {
+        size_t fr = 0, tot = 0;
+        HIPCHECK(hipMemGetInfo(&fr, &tot));
+        // buffers the engine already holds are reused
+        size_t held = c.arena.cap + c.acc.cap + c.rec.cap + c.obs.cap + c.src.cap +
+                      c.misc[0].cap + c.misc[1].cap;
+        double budget = ((double)fr + (double)held) * c.memfrac;
+        Smax = std::max<long>(1, (long)(budget / (double)per_shot));
+    }
+    long S = std::min<long>(n, Smax);
+    if (c.tile_req.tz > 0)
+        lanes = c.tile_req.kind >= 2;
+    else if (c.layout_req >= 0)
+        lanes = c.layout_req == 1;
+    else
+        lanes = S >= 48;
+    if (lanes && c.batch_req <= 0 && (S + 63) / 64 * 64 > Smax) {
+        // the padded batch must fit the budget too
+        if (Smax >= 64)
+            S = Smax / 64 * 64;
+        else if (c.tile_req.tz <= 0 && c.layout_req < 0)
+            lanes = false;
+    }
+    return (int)S;
 }
 
 // ---------------------------------------------------------------------------
@@ -830,8 +924,11 @@ static void propagate_core(const Call &P)
         build_coefs(g, vpw, rhow, r0.fpeak, coef);
         long recmax = 0;
         for (int s : grp) recmax = std::max<long>(recmax, (long)P.pre[s].ntr * P.pre[s].ns);
-        const int S = choose_batch((int)grp.size(), per_shot_bytes(g, sc, problem == 1, recmax, nt));
+        bool lanes = false;
+        const int S =
+            choose_batch((int)grp.size(), per_shot_bytes(g, sc, problem == 1, recmax, nt), lanes);
         c.stats.batch = S;
+        c.stats.layout = lanes ? 1 : 0;
         for (size_t b0 = 0; b0 < grp.size(); b0 += S) {
             const int nb = (int)std::min<size_t>(S, grp.size() - b0);
             std::vector<Record2D *> recs(nb);
@@ -843,7 +940,7 @@ static void propagate_core(const Call &P)
                 recs[k] = pr;
             }
             BatchState b;
-            prepare_batch(b, g, coef, recs, nt, problem == 1, sc);
+            prepare_batch(b, g, coef, recs, nt, problem == 1, sc, lanes);
             float *obsd = nullptr;
             if (problem != 0) {
                 obsd = c.obs.as<float>(b.rec_total + 64);
@@ -867,7 +964,8 @@ static void propagate_core(const Call &P)
             forward_pass(b, problem == 1, problem == 1 && P.precondition == 1);
             if (problem == 1 && P.precondition == 1) {
                 // propagation.c:610-623
-                launch_finalize(b.acc, nacc, pb, nacc, nb, vpw, rhow, 0, (int)nacc, 4.0, 1, c.st);
+                launch_finalize(b.acc, b.ASS, b.ACS, pb, nacc, nb, vpw, rhow, (int)nacc, 4.0, 1,
+                                c.st);
             }
             if (problem == 0) {
                 if (P.records_to_host)
@@ -895,7 +993,7 @@ static void propagate_core(const Call &P)
                          "available in this build; record every time step",
                          recs[k]->sampleRate);
             adjoint_pass(b);
-            launch_finalize(b.acc, nacc, gb, nacc, nb, vpw, rhow, 0, (int)nacc, 2.0, 0, c.st);
+            launch_finalize(b.acc, b.ASS, b.ACS, gb, nacc, nb, vpw, rhow, (int)nacc, 2.0, 0, c.st);
             if (P.precondition == 1) {
                 double *scr = c.scratch.as<double>((size_t)nb * 512 + nb + reduce_scratch_doubles());
                 launch_shot_precondition(gb, pb, nacc, nb, (int)nacc, scr, c.st);
@@ -1113,7 +1211,10 @@ extern "C" void iso_acoustic2d_propagation_engine(Record2D *record2d, Acoustic2d
     Schedule sc(nt);
     BatchState b;
     std::vector<Record2D *> recs = {record2d};
-    prepare_batch(b, g, coef, recs, nt, mode != 0, sc);
+    if (c.tile_req.tz > 0 && c.tile_req.kind >= 2)
+        fail("<error>: iso_acoustic2d_propagation_engine runs one shot; the shots-in-lanes "
+             "stencil needs a batch");
+    prepare_batch(b, g, coef, recs, nt, mode != 0, sc, false);
     const long nacc = b.nacc;
     if (mode == 0 || mode == 1) {
         forward_pass(b, mode == 1, mode == 1 && precondition == 1);
@@ -1135,7 +1236,7 @@ extern "C" void iso_acoustic2d_propagation_engine(Record2D *record2d, Acoustic2d
             }
             if (precondition == 1) {
                 float *pb = c.misc[0].as<float>(nacc);
-                launch_finalize(b.acc, nacc, pb, nacc, 1, vpd, rhod, 0, (int)nacc, 4.0, 1, c.st);
+                launch_finalize(b.acc, b.ASS, b.ACS, pb, nacc, 1, vpd, rhod, (int)nacc, 4.0, 1, c.st);
                 d2h(grad[0], pb, nacc * sizeof(float));
             }
         }
@@ -1154,7 +1255,7 @@ extern "C" void iso_acoustic2d_propagation_engine(Record2D *record2d, Acoustic2d
         rec_h2d(b.rec, record2d);
         adjoint_pass(b);
         float *gb = c.misc[0].as<float>(nacc);
-        launch_finalize(b.acc, nacc, gb, nacc, 1, vpd, rhod, 0, (int)nacc, 2.0, 0, c.st);
+        launch_finalize(b.acc, b.ASS, b.ACS, gb, nacc, 1, vpd, rhod, (int)nacc, 2.0, 0, c.st);
         d2h(grad[0], gb, nacc * sizeof(float));
     }
     sync();

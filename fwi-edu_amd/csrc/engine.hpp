@@ -47,6 +47,7 @@ struct Stats {
     double device_bytes = 0;
     double h2d_bytes = 0, d2h_bytes = 0;
     int tile_tz = 0, tile_spb = 0, tile_kind = 0, tile_txm = 0, tile_zfast = 0, tile_pd = 0, tile_remap = 0;
+    int tile_wpb = 0, tile_zrun = 0, layout = 0;
 };
 
 struct Ctx {
@@ -59,8 +60,8 @@ struct Ctx {
     long recon_launches = 0;
     int ckpt_bits = 32;
     FdTile tile_req = {0, 1};     // tz == 0: autotune
-    int xcd_remap = 0;
-    std::map<std::tuple<int, int, int, int, int, int, int, int>, FdTile> tuned;
+    int layout_req = -1;          // -1 automatic, 0 per-shot arenas, 1 shots in lanes
+    std::map<std::tuple<int, int, int, int, int, int, int, int, long>, FdTile> tuned;
     Stats stats;
     std::vector<hipEvent_t> ev;   // pool of events for sampled kernel timing
     size_t ev_used = 0;

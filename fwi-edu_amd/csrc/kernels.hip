@@ -207,7 +207,7 @@ __global__ __launch_bounds__(256) void fd_kernel(FdArgs a)
             if (MODE != FD_RECON && a.acc_on) {
                 const int px = ix - a.pml, pz = iz - a.pml;
                 if (px >= 0 && px < a.nx && pz >= 0 && pz < a.nz) {
-                    double *acc = a.acc + (long)s * a.acc_stride + (long)px * a.nz + pz;
+                    double *acc = a.acc + (long)s * a.acc_stride + ((long)px * a.nz + pz) * a.acc_cell;
                     if (MODE == FD_FWD) {
                         const float q = (stored - 2.0f * pc + p0v) * a.invdtdt;
                         *acc += (double)(q * q);
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(1024) void fd_march_kernel(FdArgs a)
                 if (MODE != FD_RECON && a.acc_on) {
                     const int px = ix - a.pml, pz = rr - a.pml;
                     if (px >= 0 && px < a.nx && pz >= 0 && pz < a.nz) {
-                        double *acc = a.acc + (long)s * a.acc_stride + (long)px * a.nz + pz;
+                        double *acc = a.acc + (long)s * a.acc_stride + ((long)px * a.nz + pz) * a.acc_cell;
                         if (MODE == FD_FWD) {
                             const float qq = (stored - 2.0f * pc + p0v) * a.invdtdt;
                             *acc += (double)(qq * qq);
@@ -465,10 +465,362 @@ static void launch_fd_mode(const FdArgs &a, FdTile tl, hipStream_t st)
         launch_fd_spb<MODE, 64>(a, tl.spb, tl.zfast, st);
 }
 
+// ---------------------------------------------------------------------------
+// Shots in wavefront lanes.
+//
+// Fields of a batch are laid out [x][z][shot] (shot fastest, padded to a
+// multiple of 64), so the 64 lanes of a wave are 64 shots of ONE cell: every
+// field access of a wave is one coalesced 256-byte segment, the coefficients
+// of the cell are the same for all lanes and come through scalar loads (one
+// fetch per 64 shots), and a wave owns one column and marches down it, keeping
+// the 7 rows of its z-stencil in registers (one new row per step).
+
+typedef const __attribute__((address_space(4))) float cfloat;
+
+// Field access in the lanes layout through buffer instructions: the row
+// offset (cell * CS * 4 bytes) is wave-uniform and goes in the scalar offset,
+// the lane (shot) offset is a 32-bit vector offset, so a load or store costs
+// no vector address arithmetic.  A field must span less than 4 GiB (checked by
+// the host before it picks this layout).
+typedef __amdgpu_buffer_rsrc_t frsrc;
+__device__ __forceinline__ frsrc field_rsrc(const float *p, long bytes)
+{
+    // dword3 0x00020000: raw 32-bit buffer, the gfx9 family encoding
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), (short)0,
+                                             (int)(unsigned)bytes, 0x00020000);
+}
+__device__ __forceinline__ float lget(frsrc f, long cell, long CS, unsigned voff)
+{
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_raw_buffer_load_b32(f, voff, (int)(cell * CS * 4), 0));
+}
+__device__ __forceinline__ void lput(frsrc f, long cell, long CS, unsigned voff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), f, voff,
+                                          (int)(cell * CS * 4), 0);
+}
+
+// One cell of the stencil for the 64 shots of a wave (shots-in-lanes layout):
+// q0..q6 are rows iz-3..iz+3 of the wave's column, x** the row-iz values of
+// the columns ix-3..ix+3, p0v the previous field; P2 is the output field, U
+// the forward field of the imaging condition and s the shot of the lane.
+// Accumulators are [cell][shot] (acc_stride 1).  Arithmetic order and the
+// epilogue (source, free-surface mirror, accumulation) are those of fd_kernel.
+template <int MODE>
+__device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long fsz, long colb,
+                                             int ix, int iz, unsigned s, int scell, frsrc P2,
+                                             frsrc U, long CS,
+                                             float q0, float q1, float q2, float q3, float q4,
+                                             float q5, float q6, float xm1, float xp1, float xm2,
+                                             float xp2, float xm3, float xp3, float p0v)
+{
+    const float C1 = 0.77793115f, C2 = -0.17388691f, C3 = 0.02338713f;
+    const float D0 = -2.8215452f, D1 = 1.57663238f, D2 = -0.18347238f, D3 = 0.01761260f;
+    const long i = colb + iz;
+    const float k1 = cf[CO_C1 * fsz + i], k2 = cf[CO_C2 * fsz + i], kA = cf[CO_A * fsz + i];
+    const float kx = cf[CO_IRX * fsz + i], kz = cf[CO_IRZ * fsz + i], kr = cf[CO_IR * fsz + i];
+    const float pc = q3;
+    const float dpdz = (C1 * (q4 - q2) + C2 * (q5 - q1) + C3 * (q6 - q0)) * a.invdz;
+    const float dpdx = (C1 * (xp1 - xm1) + C2 * (xp2 - xm2) + C3 * (xp3 - xm3)) * a.invdx;
+    const float dzz = (D0 * pc + D1 * (q4 + q2) + D2 * (q5 + q1) + D3 * (q6 + q0)) * a.invdzdz;
+    const float dxx =
+        (D0 * pc + D1 * (xp1 + xm1) + D2 * (xp2 + xm2) + D3 * (xp3 + xm3)) * a.invdxdx;
+    const float v = k1 * pc + k2 * p0v + kA * (kx * dpdx + kz * dpdz + kr * (dxx + dzz));
+
+    float val = v;
+    if (MODE == FD_FWD && (int)i == scell) {
+        const double srcv = 1.0 * (double)a.src_wav[(long)s * a.nt + a.it] *
+                            (double)cf[CO_K * fsz + i] * (double)a.dt * (double)a.dt *
+                            (double)a.invdx * (double)a.invdz;
+        val = (float)((double)v + srcv);
+    }
+    float stored = val;
+    if (a.fs) {
+        const int kk = iz - a.pml;
+        if (kk == 0)
+            stored = -0.0f;
+        else if (kk <= a.pml)
+            lput(P2, colb + (a.pml - kk), CS, s * 4u, -val);
+    }
+    if (MODE == FD_RECON && (int)i == scell) {
+        stored = stored + a.src_wav[(long)s * a.nt + a.it] * cf[CO_K * fsz + i] * a.dt *
+                              a.dt * a.invdx * a.invdz;
+    }
+    lput(P2, i, CS, s * 4u, stored);
+    if (MODE != FD_RECON && a.acc_on) {
+        const int px = ix - a.pml, pz = iz - a.pml;
+        if (px >= 0 && px < a.nx && pz >= 0 && pz < a.nz) {
+            double *acc = a.acc + ((long)px * a.nz + pz) * a.acc_cell + s;
+            if (MODE == FD_FWD) {
+                const float qq = (stored - 2.0f * pc + p0v) * a.invdtdt;
+                *acc += (double)(qq * qq);
+            } else {
+                const float qq = (stored - 2.0f * pc + p0v) * a.invdtdt * lget(U, i, CS, s * 4u);
+                *acc += (double)qq;
+            }
+        }
+    }
+}
+
+// Shots in lanes, no LDS: the 6 x-neighbours of a wave's cell are loaded from
+// global memory (the columns of the neighbouring waves, mostly L1/L2 hits).
+// The eight loads of the next PD rows are issued before a row is computed, so
+// a wave always has memory requests in flight.
+template <int MODE, int PD>
+__global__ __launch_bounds__(512) void fd_lanes_kernel(FdArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wpb = blockDim.x >> 6;
+    int b = blockIdx.x;
+    if (a.remap) {
+        // hardware dispatch sends block b to XCD b % 8; give each XCD one
+        // contiguous range of the logical order so that the x-halo columns of
+        // a block are those of blocks on the same XCD (same L2)
+        const int per = (int)((gridDim.x + 7) >> 3);
+        b = (b & 7) * per + (b >> 3);
+        if (b >= a.lz_nblk) return;
+    }
+    const int zr = b % a.lz_nrun;
+    b /= a.lz_nrun;
+    const int sg = b % a.lz_ngrp, xg = b / a.lz_ngrp;
+    const int ix = a.ixs + xg * wpb + w;
+    if (ix > a.ixe) return;  // the whole wave
+    const int iz0 = a.izs + zr * a.lz_run;
+    const int iz1 = min(a.ize, iz0 + a.lz_run - 1);
+    const unsigned s = (unsigned)(sg * 64 + lane);
+    const long CS = a.cell_stride;
+    const int ld = a.ld;
+    const long fsz = (long)a.Nx * ld;
+    // coefficients are the same for the 64 lanes: read through the scalar
+    // data cache (constant address space, wave-uniform addresses)
+    cfloat *cf = (cfloat *)a.coef;
+    const long fbytes = fsz * CS * 4;
+    const frsrc P1 = field_rsrc(a.cur, fbytes), P0 = field_rsrc(a.prev, fbytes);
+    const frsrc P2 = field_rsrc(a.out, fbytes);
+    const frsrc U = field_rsrc(MODE == FD_ADJ ? a.u : a.out, fbytes);
+    const unsigned vo = s * 4u;
+    const int scell = (MODE != FD_ADJ && a.src_cell) ? a.src_cell[s] : -1;
+    const long colb = (long)ix * ld;
+
+    // the loads of a row: the new z row (iz+3), the six x neighbours, p0
+    auto load_row = [&](int iz, float *r) {
+        const long i = colb + iz;
+        r[0] = lget(P1, i + 3, CS, vo);
+        r[1] = lget(P1, i - ld, CS, vo);
+        r[2] = lget(P1, i + ld, CS, vo);
+        r[3] = lget(P1, i - 2 * ld, CS, vo);
+        r[4] = lget(P1, i + 2 * ld, CS, vo);
+        r[5] = lget(P1, i - 3 * ld, CS, vo);
+        r[6] = lget(P1, i + 3 * ld, CS, vo);
+        r[7] = lget(P0, i, CS, vo);
+    };
+    float q0 = lget(P1, colb + iz0 - 3, CS, vo), q1 = lget(P1, colb + iz0 - 2, CS, vo);
+    float q2 = lget(P1, colb + iz0 - 1, CS, vo), q3 = lget(P1, colb + iz0, CS, vo);
+    float q4 = lget(P1, colb + iz0 + 1, CS, vo), q5 = lget(P1, colb + iz0 + 2, CS, vo);
+    // software pipeline: the loads of the next PD rows are in flight while a
+    // row is computed (rows past iz1 re-load iz1, which stays in bounds)
+    float nx_[PD][8];
+#pragma unroll
+    for (int p = 0; p < PD; ++p) load_row(min(iz0 + p, iz1), nx_[p]);
+#pragma unroll 2
+    for (int iz = iz0; iz <= iz1; ++iz) {
+        const long i = colb + iz;
+        float r[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) r[k] = nx_[0][k];
+#pragma unroll
+        for (int p = 0; p + 1 < PD; ++p)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) nx_[p][k] = nx_[p + 1][k];
+        load_row(min(iz + PD, iz1), nx_[PD - 1]);
+        const float q6 = r[0];
+        const float xm1 = r[1], xp1 = r[2], xm2 = r[3], xp2 = r[4], xm3 = r[5], xp3 = r[6];
+        const float p0v = r[7];
+        lanes_update<MODE>(a, cf, fsz, colb, ix, iz, s, scell, P2, U, CS, q0, q1, q2, q3, q4, q5, q6,
+                           xm1, xp1, xm2, xp2, xm3, xp3, p0v);
+        q0 = q1;
+        q1 = q2;
+        q2 = q3;
+        q3 = q4;
+        q4 = q5;
+        q5 = q6;
+    }
+}
+
+template <int MODE>
+static void launch_lanes_t(const FdArgs &a0, FdTile tl, hipStream_t st)
+{
+    FdArgs a = a0;
+    const int H = a.ize - a.izs + 1;
+    a.lz_run = tl.zrun;
+    a.lz_nrun = (H + tl.zrun - 1) / tl.zrun;
+    a.lz_ngrp = (a.nshot + 63) / 64;
+    const int nxg = (a.ixe - a.ixs + 1 + tl.wpb - 1) / tl.wpb;
+    const long nb = (long)nxg * a.lz_nrun * a.lz_ngrp;
+    a.lz_nblk = (int)nb;
+    const long nlaunch = a.remap ? (nb + 7) / 8 * 8 : nb;
+    const dim3 grid((unsigned)nlaunch), block(tl.wpb * 64);
+    if (tl.pd >= 3)
+        fd_lanes_kernel<MODE, 3><<<grid, block, 0, st>>>(a);
+    else if (tl.pd == 2)
+        fd_lanes_kernel<MODE, 2><<<grid, block, 0, st>>>(a);
+    else
+        fd_lanes_kernel<MODE, 1><<<grid, block, 0, st>>>(a);
+}
+
+// Shots in lanes with an LDS row exchange.  A block of WPB waves owns WPB
+// adjacent columns and marches down rows [iz0, iz1] in step: at each row every
+// wave writes the centre value of its column (64 shots, 256 B) to LDS and the
+// six halo columns of the block are loaded once (by waves 0..5 mod WPB), so
+// one row of the block costs WPB + 6 global loads of the current field instead
+// of 7 * WPB.  The LDS row is double buffered: one barrier per row.  Waves
+// whose column lies past ixe still load and publish their column (an active
+// neighbour may need it) but compute nothing.
+template <int MODE, int WPB, int PD>
+__global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
+{
+    constexpr int NC = WPB + 6;                    // columns of the LDS row
+    constexpr int NH = (6 + WPB - 1) / WPB;        // halo columns per wave (max)
+    __shared__ float row[2][NC][64];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int b = blockIdx.x;
+    if (a.remap) {
+        const int per = (int)((gridDim.x + 7) >> 3);
+        b = (b & 7) * per + (b >> 3);
+        if (b >= a.lz_nblk) return;  // the whole block
+    }
+    const int zr = b % a.lz_nrun;
+    b /= a.lz_nrun;
+    const int sg = b % a.lz_ngrp, xg = b / a.lz_ngrp;
+    const int x0 = a.ixs + xg * WPB;
+    const int ix = x0 + w;
+    const bool active = ix <= a.ixe;
+    const int iz0 = a.izs + zr * a.lz_run;
+    const int iz1 = min(a.ize, iz0 + a.lz_run - 1);
+    const unsigned s = (unsigned)(sg * 64 + lane);
+    const long CS = a.cell_stride;
+    const int ld = a.ld;
+    const long fsz = (long)a.Nx * ld;
+    cfloat *cf = (cfloat *)a.coef;
+    const long fbytes = fsz * CS * 4;
+    const frsrc P1 = field_rsrc(a.cur, fbytes), P0 = field_rsrc(a.prev, fbytes);
+    const frsrc P2 = field_rsrc(a.out, fbytes);
+    const frsrc U = field_rsrc(MODE == FD_ADJ ? a.u : a.out, fbytes);
+    const unsigned vo = s * 4u;
+    const int scell = (MODE != FD_ADJ && a.src_cell) ? a.src_cell[s] : -1;
+    const long colb = (long)min(ix, a.Nx - 1) * ld;
+    // halo columns carried by this wave: h = w, w + WPB, ... < 6; LDS column
+    // j = h (left halo) or WPB + h (right halo)
+    int hj[NH];
+    long hcol[NH];
+#pragma unroll
+    for (int k = 0; k < NH; ++k) {
+        const int h = w + k * WPB;
+        hj[k] = h < 3 ? h : WPB + h;
+        const int c = h < 3 ? x0 - 3 + h : x0 + WPB + (h - 3);
+        hcol[k] = (long)min(max(c, 0), a.Nx - 1) * ld;
+    }
+    const int nh = w < 6 ? (6 - w + WPB - 1) / WPB : 0;
+
+    float q0 = lget(P1, colb + iz0 - 3, CS, vo), q1 = lget(P1, colb + iz0 - 2, CS, vo);
+    float q2 = lget(P1, colb + iz0 - 1, CS, vo), q3 = lget(P1, colb + iz0, CS, vo);
+    float q4 = lget(P1, colb + iz0 + 1, CS, vo), q5 = lget(P1, colb + iz0 + 2, CS, vo);
+    // prefetched rows: own column z+3, p0 at z, halo columns at z
+    float pn[PD], pp[PD], ph[PD][NH];
+    auto load_row = [&](int iz, int p) {
+        pn[p] = lget(P1, colb + iz + 3, CS, vo);
+        pp[p] = lget(P0, colb + iz, CS, vo);
+#pragma unroll
+        for (int k = 0; k < NH; ++k)
+            if (k < nh) ph[p][k] = lget(P1, hcol[k] + iz, CS, vo);
+    };
+#pragma unroll
+    for (int p = 0; p < PD; ++p) load_row(min(iz0 + p, iz1), p);
+    for (int iz = iz0; iz <= iz1; ++iz) {
+        const int buf = iz & 1;
+        const float q6 = pn[0], p0v = pp[0];
+        row[buf][3 + w][lane] = q3;
+#pragma unroll
+        for (int k = 0; k < NH; ++k)
+            if (k < nh) row[buf][hj[k]][lane] = ph[0][k];
+#pragma unroll
+        for (int p = 0; p + 1 < PD; ++p) {
+            pn[p] = pn[p + 1];
+            pp[p] = pp[p + 1];
+#pragma unroll
+            for (int k = 0; k < NH; ++k) ph[p][k] = ph[p + 1][k];
+        }
+        load_row(min(iz + PD, iz1), PD - 1);
+        __syncthreads();
+        if (active) {
+            const float xm1 = row[buf][2 + w][lane], xp1 = row[buf][4 + w][lane];
+            const float xm2 = row[buf][1 + w][lane], xp2 = row[buf][5 + w][lane];
+            const float xm3 = row[buf][0 + w][lane], xp3 = row[buf][6 + w][lane];
+            lanes_update<MODE>(a, cf, fsz, colb, ix, iz, s, scell, P2, U, CS, q0, q1, q2, q3, q4,
+                               q5, q6, xm1, xp1, xm2, xp2, xm3, xp3, p0v);
+        }
+        q0 = q1;
+        q1 = q2;
+        q2 = q3;
+        q3 = q4;
+        q4 = q5;
+        q5 = q6;
+    }
+}
+
+template <int MODE, int WPB>
+static void launch_lanes_lds_w(const FdArgs &a, const dim3 &grid, int pd, hipStream_t st)
+{
+    if (pd >= 2)
+        fd_lanes_lds_kernel<MODE, WPB, 2><<<grid, WPB * 64, 0, st>>>(a);
+    else
+        fd_lanes_lds_kernel<MODE, WPB, 1><<<grid, WPB * 64, 0, st>>>(a);
+}
+
+template <int MODE>
+static void launch_lanes_lds_t(const FdArgs &a0, FdTile tl, hipStream_t st)
+{
+    FdArgs a = a0;
+    const int H = a.ize - a.izs + 1;
+    a.lz_run = tl.zrun;
+    a.lz_nrun = (H + tl.zrun - 1) / tl.zrun;
+    a.lz_ngrp = (a.nshot + 63) / 64;
+    const int nxg = (a.ixe - a.ixs + 1 + tl.wpb - 1) / tl.wpb;
+    const long nb = (long)nxg * a.lz_nrun * a.lz_ngrp;
+    a.lz_nblk = (int)nb;
+    const dim3 grid((unsigned)(a.remap ? (nb + 7) / 8 * 8 : nb));
+    if (tl.wpb >= 16)
+        launch_lanes_lds_w<MODE, 16>(a, grid, tl.pd, st);
+    else if (tl.wpb >= 8)
+        launch_lanes_lds_w<MODE, 8>(a, grid, tl.pd, st);
+    else
+        launch_lanes_lds_w<MODE, 4>(a, grid, tl.pd, st);
+}
+
 void launch_fd(int mode, const FdArgs &a0, FdTile tl, hipStream_t st)
 {
     FdArgs a = a0;
     a.remap = tl.remap;
+    if (tl.kind == 3) {
+        if (mode == FD_FWD)
+            launch_lanes_lds_t<FD_FWD>(a, tl, st);
+        else if (mode == FD_RECON)
+            launch_lanes_lds_t<FD_RECON>(a, tl, st);
+        else
+            launch_lanes_lds_t<FD_ADJ>(a, tl, st);
+        return;
+    }
+    if (tl.kind == 2) {
+        if (mode == FD_FWD)
+            launch_lanes_t<FD_FWD>(a, tl, st);
+        else if (mode == FD_RECON)
+            launch_lanes_t<FD_RECON>(a, tl, st);
+        else
+            launch_lanes_t<FD_ADJ>(a, tl, st);
+        return;
+    }
     if (tl.kind == 1) {
         if (mode == FD_FWD)
             launch_march_mode<FD_FWD>(a, tl, st);
@@ -652,29 +1004,30 @@ void launch_grad_invrho(float *coef, const Geom &g, hipStream_t st)
 // ---------------------------------------------------------------------------
 // Receivers (propagation.c:567-576 record p1; :676-682 adjoint injection)
 
-__global__ void record_kernel(RcvTable t, const float *f0, long stride, float *rec, int it)
+__global__ void record_kernel(RcvTable t, const float *f0, long stride, long cstride, float *rec,
+                              int it)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= t.n) return;
     const int sr = t.sr[e];
     if (it % sr) return;
-    rec[t.trace_off[e] + it / sr] = f0[(long)t.shot[e] * stride + t.cell[e]];
+    rec[t.trace_off[e] + it / sr] = f0[(long)t.shot[e] * stride + (long)t.cell[e] * cstride];
 }
 
-void launch_record(const RcvTable &t, const float *field0, long shot_stride, float *rec, int it,
-                   hipStream_t st)
+void launch_record(const RcvTable &t, const float *field0, long shot_stride, long cell_stride,
+                   float *rec, int it, hipStream_t st)
 {
     if (t.n <= 0) return;
-    record_kernel<<<(t.n + 255) / 256, 256, 0, st>>>(t, field0, shot_stride, rec, it);
+    record_kernel<<<(t.n + 255) / 256, 256, 0, st>>>(t, field0, shot_stride, cell_stride, rec, it);
 }
 
-__global__ void inject_kernel(InjTable t, float *f0, long stride, const float *K, const float *adj,
-                              int it, float dt, float invdx, float invdz)
+__global__ void inject_kernel(InjTable t, float *f0, long stride, long cstride, const float *K,
+                              const float *adj, int it, float dt, float invdx, float invdz)
 {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= t.n) return;
     const int cell = t.cell[u];
-    float *p = f0 + (long)t.shot[u] * stride + cell;
+    float *p = f0 + (long)t.shot[u] * stride + (long)cell * cstride;
     float v = *p;
     const float k = K[cell];
     for (int e = t.beg[u]; e < t.beg[u + 1]; ++e)
@@ -682,12 +1035,13 @@ __global__ void inject_kernel(InjTable t, float *f0, long stride, const float *K
     *p = v;
 }
 
-void launch_inject(const InjTable &t, float *field0, long shot_stride, const float *K,
-                   const float *adj, int it, float dt, float invdx, float invdz, hipStream_t st)
+void launch_inject(const InjTable &t, float *field0, long shot_stride, long cell_stride,
+                   const float *K, const float *adj, int it, float dt, float invdx, float invdz,
+                   hipStream_t st)
 {
     if (t.n <= 0) return;
-    inject_kernel<<<(t.n + 255) / 256, 256, 0, st>>>(t, field0, shot_stride, K, adj, it, dt, invdx,
-                                                      invdz);
+    inject_kernel<<<(t.n + 255) / 256, 256, 0, st>>>(t, field0, shot_stride, cell_stride, K, adj,
+                                                      it, dt, invdx, invdz);
 }
 
 // ---------------------------------------------------------------------------
@@ -725,24 +1079,25 @@ void launch_adjsrc(const float *pre, const float *obs, float *out, int ntr, int 
 // Gradient finalisation and preconditioning (propagation.c:610-623, 732-743,
 // 953-993; optimization.c:13-97)
 
-__global__ void finalize_kernel(const double *acc, long acc_stride, float *out, long out_stride,
-                                const float *vpw, const float *rhow, int n, double scale,
-                                int squared)
+__global__ void finalize_kernel(const double *acc, long acc_stride, long acc_cell, float *out,
+                                long out_stride, const float *vpw, const float *rhow, int n,
+                                double scale, int squared)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, s = blockIdx.y;
     if (i >= n) return;
     const float rv = rhow[i] * vpw[i];
     const double den = squared ? (double)(rv * rv) : (double)rv;
-    out[(long)s * out_stride + i] = (float)(scale * acc[(long)s * acc_stride + i] / den);
+    out[(long)s * out_stride + i] =
+        (float)(scale * acc[(long)s * acc_stride + (long)i * acc_cell] / den);
 }
 
-void launch_finalize(const double *acc, long acc_stride, float *out, long out_stride, int nshot,
-                     const float *vpw, const float *rhow, long, int n, double scale, int squared,
-                     hipStream_t st)
+void launch_finalize(const double *acc, long acc_stride, long acc_cell, float *out,
+                     long out_stride, int nshot, const float *vpw, const float *rhow, int n,
+                     double scale, int squared, hipStream_t st)
 {
     dim3 grid((n + 255) / 256, nshot);
-    finalize_kernel<<<grid, 256, 0, st>>>(acc, acc_stride, out, out_stride, vpw, rhow, n, scale,
-                                          squared);
+    finalize_kernel<<<grid, 256, 0, st>>>(acc, acc_stride, acc_cell, out, out_stride, vpw, rhow,
+                                          n, scale, squared);
 }
 
 // Deterministic reductions: a fixed grid of RB blocks, each thread strides

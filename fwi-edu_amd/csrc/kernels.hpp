@@ -33,7 +33,9 @@ constexpr int CO_STREAMED = 6;
 
 // Stencil launch shape.  kind 0: square tiles of tz (32 or 64) rows, spb shots
 // per block (1, 2, 4, 8).  kind 1: column march over txm columns per block,
-// spb shots per block (1, 2, 4), pd columns of loads in flight.
+// spb shots per block (1, 2, 4), pd columns of loads in flight.  kind 2: shots
+// in wavefront lanes (fields laid out [x][z][shot]), wpb columns per block,
+// each wave marching zrun rows.
 struct FdTile {
     int tz = 64;
     int spb = 1;
@@ -42,6 +44,8 @@ struct FdTile {
     int zfast = 0;
     int pd = 1;  // columns prefetched ahead by the march (1, 2, 4)
     int remap = 0;  // XCD-aware block order
+    int wpb = 4;    // kind 2: waves (= adjacent columns) per block
+    int zrun = 32;  // kind 2: rows marched by one wave
 };
 
 struct FdArgs {
@@ -49,7 +53,9 @@ struct FdArgs {
     float *out;             // shot 0 pointers
     const float *cur;
     const float *prev;
-    long shot_stride;       // floats between shot arenas
+    long shot_stride;       // floats between shots (per-shot arenas: the arena size;
+                            // shots in lanes: 1)
+    long cell_stride;       // floats between cells (per-shot arenas: 1; lanes: Sp)
     int nshot;
     int Nx, ld, ixs, ixe, izs, ize, pml, fs, nx, nz;
     float invdx, invdz, invdxdx, invdzdz, invdtdt, dt;
@@ -57,13 +63,15 @@ struct FdArgs {
     int remap;              // XCD-aware block order (1) or hardware order (0)
     int zfast;              // tile order: z-neighbours adjacent (1) or x-neighbours (0)
     int mx_len, mx_nseg, mz_len, mz_nseg;  // column-march plan
+    int lz_run, lz_nrun, lz_ngrp, lz_nblk; // shots-in-lanes plan
     // source (FD_FWD / FD_RECON)
     const int *src_cell;    // [nshot] flat padded index or -1
     const float *src_wav;   // [nshot][nt]
     int nt, it;
     // pseudo-Hessian (FD_FWD) / imaging (FD_ADJ) every MST steps
     double *acc;            // shot 0, [nx][nz]
-    long acc_stride;
+    long acc_stride;        // doubles between shots
+    long acc_cell;          // doubles between cells
     const float *u;         // FD_ADJ: forward field at `it`, shot 0
     int acc_on;
 };
@@ -94,16 +102,17 @@ struct RcvTable {
     const long *trace_off;  // [n] offset of this trace in the record buffer
     const int *sr;          // [n] sample rate of the entry's shot
 };
-void launch_record(const RcvTable &t, const float *field0, long shot_stride, float *rec, int it,
-                   hipStream_t st);
+void launch_record(const RcvTable &t, const float *field0, long shot_stride, long cell_stride,
+                   float *rec, int it, hipStream_t st);
 struct InjTable {
     int n;                  // unique (shot, cell) pairs
     const int *shot, *cell; // [n]
     const int *beg;         // [n+1] into trace_off (receiver order kept)
     const long *trace_off;  // offset of the trace's sample 0 in the adjoint-source buffer
 };
-void launch_inject(const InjTable &t, float *field0, long shot_stride, const float *K,
-                   const float *adj, int it, float dt, float invdx, float invdz, hipStream_t st);
+void launch_inject(const InjTable &t, float *field0, long shot_stride, long cell_stride,
+                   const float *K, const float *adj, int it, float dt, float invdx, float invdz,
+                   hipStream_t st);
 
 // ---- adjoint source ------------------------------------------------------
 // out = taper(pre - obs) for one shot [ntr][ns]; the cosine tables are built
@@ -113,9 +122,9 @@ void launch_adjsrc(const float *pre, const float *obs, float *out, int ntr, int 
 
 // ---- gradient finalisation / preconditioning ------------------------------
 // g[i] = (float)(scale * acc[i] / (double)(rho*vp) or /(double)((rho*vp)^2))
-void launch_finalize(const double *acc, long acc_stride, float *out, long out_stride, int nshot,
-                     const float *vpw, const float *rhow, long win_stride, int n, double scale,
-                     int squared, hipStream_t st);
+void launch_finalize(const double *acc, long acc_stride, long acc_cell, float *out,
+                     long out_stride, int nshot, const float *vpw, const float *rhow, int n,
+                     double scale, int squared, hipStream_t st);
 // per-shot sum of p (double) -> avg -> g /= 1 + p/(7 avg), p /= same
 void launch_shot_precondition(float *g, float *p, long stride, int nshot, int n, double *scratch,
                               hipStream_t st);

@@ -158,6 +158,18 @@ void fwiedu_set_march(int txm, int spb, int pd);
  * XCD works on one contiguous range of tiles, 0 keeps the hardware's
  * round-robin order.  (The autotuner tries both.) */
 void fwiedu_set_xcd_remap(int on);
+/* Force the shots-in-lanes stencil (fields laid out [x][z][shot], the 64 lanes
+ * of a wavefront are 64 shots of one cell): wpb columns per block, each wave
+ * marching zrun (8 ... 256) rows with the loads of the next pd rows in flight.
+ * lds = 0: x-neighbours loaded from memory (wpb 1, 2, 4, 8; pd 1 ... 3);
+ * lds = 1: x-neighbours exchanged through LDS (wpb 4, 8, 16; pd 1, 2).
+ * Applies to iso_acoustic2d_propagation and sessions; fwiedu_set_tile(0, 0)
+ * returns to the autotuner. */
+void fwiedu_set_lanes(int wpb, int zrun, int pd, int lds);
+/* Field layout of batched launches when the stencil shape is autotuned: -1
+ * (default) shots in lanes from 48 shots per launch up, per-shot arenas below;
+ * 0 always per-shot arenas; 1 always shots in lanes. */
+void fwiedu_set_layout(int layout);
 /* Fraction of free device memory the engine may use (default 0.85). */
 void fwiedu_set_memory_fraction(double frac);
 /* Sample every `every`-th reconstruction-sweep launch (the stencil with no

@@ -306,24 +306,59 @@ def test_c1_subset_parity(L, oracles):
 @pytest.mark.parametrize("fs", [0, 1])
 def test_every_stencil_shape_bit_identical(L, oracles, fs):
     """Every launch shape the autotuner may pick (square tiles of 32/64 rows in
-    both tile orders, column marches of 16/32/64 columns, 1-8 shots per block)
-    gives the same bits as the strict-order oracle."""
+    both tile orders, column marches of 16/32/64 columns, 1-8 shots per block,
+    and the shots-in-lanes layout with 1-16 columns per block, 8-256 rows per
+    wave, x-neighbours from memory or through LDS, its 5 shots padded to 64
+    lanes) gives the same bits as the strict-order oracle."""
     w = synth.workload("tiny", nshot=5, freesurface=fs, shot_depth_cells=1, nx=70)
     say(f"case: stencil shapes tiny fs={fs} 5 shots")
     obs = observed(oracles["f32s"], w, False)
     pre_o, g_o = gradient(oracles["f32s"], w, obs, False)
     shapes = [("tile", tz, spb, zf) for tz in (32, 64) for spb in (1, 2, 4, 8) for zf in (0, 1)]
     shapes += [("march", txm, spb, pd) for txm in (16, 32, 64) for spb in (1, 2, 4) for pd in (1, 2, 4)]
+    shapes += [("lanes", wpb, zrun, (rm, pd, 0)) for wpb in (1, 4, 8) for zrun in (8, 16, 64, 128)
+               for rm in (0, 1) for pd in (1, 2, 3)]
+    shapes += [("lanes", wpb, zrun, (rm, pd, 1)) for wpb in (4, 8, 16) for zrun in (8, 32, 256)
+               for rm in (0, 1) for pd in (1, 2)]
     try:
         for kind, p, spb, zf in shapes:
             if kind == "tile":
                 L.fwiedu_set_tile(p, spb)
                 L.fwiedu_set_tile_order(zf)
-            else:
+            elif kind == "march":
                 L.fwiedu_set_march(p, spb, zf)
+            else:
+                L.fwiedu_set_xcd_remap(zf[0])
+                L.fwiedu_set_lanes(p, spb, zf[1], zf[2])
             pre_g, g_g = gradient(L, w, obs, True)
             assert np.array_equal(g_g, g_o), (kind, p, spb, zf)
             assert np.array_equal(recs(pre_g), recs(pre_o)), (kind, p, spb, zf)
     finally:
         L.fwiedu_set_tile(0, 0)
         L.fwiedu_set_tile_order(0)
+        L.fwiedu_set_xcd_remap(0)
+
+
+def test_lanes_layout_matches_arenas(L):
+    """70 shots: two groups of 64 lanes, the second padded.  Records,
+    per-shot-preconditioned gradients and the misfit of the shots-in-lanes
+    layout are bit-identical to those of per-shot arenas, in one launch and
+    in batches of 33 (33 lanes of 64 used)."""
+    w = synth.workload("tiny", nshot=70, freesurface=1, shot_depth_cells=1, nx=90)
+    say("case: lanes vs arenas tiny fs=1 70 shots")
+    obs = observed(L, w, True)
+    out = {}
+    try:
+        for name, layout, batch in (("arenas", 0, 0), ("lanes", 1, 0), ("lanes33", 1, 33)):
+            L.fwiedu_set_layout(layout)
+            L.fwiedu_set_batch(batch)
+            out[name] = gradient(L, w, obs, True)
+            assert flib.stat("layout") == layout, name
+    finally:
+        L.fwiedu_set_layout(-1)
+        L.fwiedu_set_batch(0)
+    pre_a, g_a = out["arenas"]
+    for name in ("lanes", "lanes33"):
+        pre_l, g_l = out[name]
+        assert np.array_equal(g_l, g_a), name
+        assert np.array_equal(recs(pre_l), recs(pre_a)), name
