@@ -73,6 +73,22 @@ def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
     return times, stats, misfit
 
 
+# HBM bytes of one reconstruction launch of the c1 workload at 120 shots in
+# lanes, per stencil shape (columns per block, rows per wave, rows in flight,
+# XCD order): FETCH_SIZE x 2 + WRITE_SIZE from rocprofv3 counter runs scaled by
+# a 512 MiB copy; the runs and the calibration are in
+# profiles/fd_lanes_recon_c1_traffic.md.
+C1_LANES_TRAFFIC = {(8, 8, 1, 0): 447.5e6, (8, 16, 1, 1): 345.5e6, (4, 8, 1, 0): 518.8e6,
+                    (8, 8, 2, 0): 447.3e6}
+
+
+def measured_traffic(w, st):
+    if w.name != "c1" or w.nshot != 120 or int(st["tile_kind"]) != 2:
+        return None
+    key = (int(st["tile_wpb"]), int(st["tile_zrun"]), int(st["tile_pd"]), int(st["tile_remap"]))
+    return C1_LANES_TRAFFIC.get(key)
+
+
 KERNELS = {0: "fd_kernel (square tiles, per-shot arenas)",
            1: "fd_march_kernel (column march, per-shot arenas)",
            2: "fd_lanes_kernel (shots in lanes)",
@@ -150,7 +166,11 @@ def main():
     ap.add_argument("--plain-shots", type=int, default=4)
     ap.add_argument("--cpu-shots", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-host", action="store_true",
+                    help="skip the host-array (PCIe-inclusive) iso_acoustic2d_propagation timing")
     ap.add_argument("--ktime-every", type=int, default=11)
+    ap.add_argument("--lanes", default="",
+                    help="force the shots-in-lanes stencil shape WPB,ZRUN,PD,XCD (default: autotuned)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -160,6 +180,10 @@ def main():
     if not L.fwiedu_device_ok():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.lanes:
+        wpb, zrun, pd, xcd = (int(v) for v in args.lanes.split(","))
+        L.fwiedu_set_xcd_remap(xcd)
+        L.fwiedu_set_lanes(wpb, zrun, pd, 0)
     over = {"nshot": args.nshot} if args.nshot else {}
     if args.nt:
         over["nt"] = args.nt
@@ -182,6 +206,20 @@ def main():
     bytes_per_launch = cells * (12.0 * S + 28.0)
     avg_ms = st["fd_ms_avg"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
+
+    # the reference boundary as the FWI driver calls it: host models and host
+    # records in, host adjoint sources and host gradient out (PCIe included)
+    host = None
+    if not args.no_host:
+        vps, rhs = Array2D(w.vp_start), Array2D(w.rho_start)
+        pre = obs.header_copy()
+        g = Array2D(np.zeros((w.nx, w.nz), np.float32))
+        t0 = time.perf_counter()
+        flib.propagate(L, w, vps, rhs, obs, 1, pre, g)
+        th = time.perf_counter() - t0
+        host = {"gradient_shots_per_s": w.nshot / th, "ms": th * 1e3,
+                "call": "iso_acoustic2d_propagation(problem=1) on host arrays, 1 call"}
+        log(f"host-boundary gradient: {w.nshot / th:.2f} shots/s")
 
     # plain path: one shot per launch
     plain = None
@@ -226,8 +264,13 @@ def main():
         "misfit": misfit,
         "plain": plain,
         "batched_over_plain": (shots_per_s / plain["gradient_shots_per_s"]) if plain else None,
+        "host_boundary": host,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": measured_traffic(w, st),
+                     "traffic_source": "profiles/fd_lanes_recon_c1_traffic.md (counter runs of the "
+                                       "same stencil shape; null when the autotuner picked a shape "
+                                       "not measured there)",
                      "kernel": KERNELS[int(st["tile_kind"])].split(" ")[0] +
                                "<FD_RECON> (reconstruction sweep)",
                      "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_ms * 1e3,

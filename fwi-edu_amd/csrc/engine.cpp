@@ -290,7 +290,7 @@ FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const flo
         {
             launch_fd(FD_RECON, a, t, c.st);
             HIPCHECK(hipEventRecord(e0, c.st));
-            for (int r = 0; r < 3; ++r) launch_fd(FD_RECON, a, t, c.st);
+            for (int r = 0; r < 8; ++r) launch_fd(FD_RECON, a, t, c.st);
             HIPCHECK(hipEventRecord(e1, c.st));
             HIPCHECK(hipEventSynchronize(e1));
             float ms = 0.f;
