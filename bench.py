@@ -169,6 +169,8 @@ def main():
     ap.add_argument("--no-host", action="store_true",
                     help="skip the host-array (PCIe-inclusive) iso_acoustic2d_propagation timing")
     ap.add_argument("--ktime-every", type=int, default=11)
+    ap.add_argument("--ckpt-bits", type=int, default=32, choices=(32, 16),
+                    help="checkpoint storage: FP32 or FP16 with a per-snapshot scale")
     ap.add_argument("--lanes", default="",
                     help="force the shots-in-lanes stencil shape WPB,ZRUN,PD,XCD (default: autotuned)")
     args = ap.parse_args()
@@ -180,6 +182,7 @@ def main():
     if not L.fwiedu_device_ok():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
+    L.fwiedu_set_checkpoint_bits(args.ckpt_bits)
     if args.lanes:
         wpb, zrun, pd, xcd = (int(v) for v in args.lanes.split(","))
         L.fwiedu_set_xcd_remap(xcd)
@@ -258,6 +261,7 @@ def main():
                                f"{w.freesurface}, checkpointed adjoint gradient, precondition "
                                f"{w.precondition}, model and records resident in HBM",
                    "shots_per_launch": S, "batch": int(st["batch"]),
+                   "checkpoint_storage": f"fp{args.ckpt_bits}",
                    "stencil": stencil_shape(st)},
         "gcell_updates_per_s": shots_per_s * cu / 1e9,
         "cell_updates_per_gradient_shot": cu,

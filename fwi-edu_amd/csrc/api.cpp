@@ -529,6 +529,7 @@ extern "C" double fwiedu_stat(const char *name)
     if (n == "tile_wpb") return (double)s.tile_wpb;
     if (n == "tile_zrun") return (double)s.tile_zrun;
     if (n == "layout") return (double)s.layout;
+    if (n == "ckpt_bits") return (double)s.ckpt_bits;
     if (n == "h2d_bytes") return s.h2d_bytes;
     if (n == "d2h_bytes") return s.d2h_bytes;
     if (n == "device_bytes") {
@@ -553,7 +554,8 @@ extern "C" void fwiedu_release(void)
     sync();
     DevBuf *all[] = {&c.model[0], &c.model[1], &c.setup, &c.coef,    &c.arena,   &c.acc,
                      &c.rec,      &c.obs,      &c.src,   &c.tables,  &c.taper,   &c.shot_out,
-                     &c.gsum,     &c.scratch,  &c.misc[0], &c.misc[1], &c.misc[2], &c.misc[3]};
+                     &c.gsum,     &c.scratch,  &c.misc[0], &c.misc[1], &c.misc[2], &c.misc[3],
+                     &c.ck16,     &c.ckaux};
     for (DevBuf *d : all) d->release();
 }
 

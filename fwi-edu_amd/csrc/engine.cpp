@@ -353,14 +353,19 @@ struct BatchPlan {
     int nbuf = 0;               // fields per shot arena
     long shot_stride = 0;       // floats
     // arena field indices: adjoint/forward work, the zero field, reconstruction
-    // work, then checkpoint slots and the imaging ring (gradient batches only)
+    // work, then (gradient batches only) the checkpoint slots and the imaging
+    // ring; with FP16 checkpoints the slots live in a separate half-precision
+    // buffer and three FP32 fields R0, R1 (restart pair) and RU (imaging
+    // field at a checkpoint time) take their place
     enum { A0 = 0, A1, A2, ZERO, F0, F1, F2, CK0 };
+    enum { R0 = CK0, R1, RU, RING16 };
     int ring0 = 0;
 };
 
-static long fields_per_shot(const Schedule &sc, bool adjoint)
+static long fields_per_shot(const Schedule &sc, bool adjoint, bool ck16)
 {
-    return adjoint ? (BatchPlan::CK0 + sc.nck + sc.nring) : 4;
+    if (!adjoint) return 4;
+    return ck16 ? (BatchPlan::RING16 + sc.nring) : (BatchPlan::CK0 + sc.nck + sc.nring);
 }
 
 struct Tables {
@@ -565,12 +570,18 @@ struct BatchState {
     int S = 0, nt = 0;
     bool adjoint = false;
     bool lanes = false;
+    bool ck16 = false;         // FP16 checkpoint slots
     int Sp = 0;                // shots the arena holds (S, or S padded to 64)
     Schedule *sc = nullptr;
     long KS = 0, SS = 0, CS = 1;  // field, shot and cell strides (floats)
     long ASS = 0, ACS = 1;        // accumulator shot and cell strides (doubles)
     long arena_floats = 0;
     float *arena = nullptr;
+    __half *h16 = nullptr;        // FP16 slots: slot k of shot s, cell i at k*KS16 + s*SS16 + i*CS16
+    long KS16 = 0, SS16 = 0, CS16 = 1;
+    float *ckinv = nullptr;       // [nck][Sp] inverse scales
+    unsigned *ckmx = nullptr;     // [Sp] scratch
+    int ring0 = 0;                // first imaging-ring field
     int *cell_dev = nullptr;
     float *wav_dev = nullptr;
     Tables tb;
@@ -586,7 +597,7 @@ struct BatchState {
 
 static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
                           const std::vector<Record2D *> &recs, int nt, bool adjoint, Schedule &sc,
-                          bool lanes)
+                          bool lanes, bool ck16)
 {
     Ctx &c = g_ctx;
     b.g = g;
@@ -596,8 +607,10 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
     b.adjoint = adjoint;
     b.sc = &sc;
     b.lanes = lanes;
+    b.ck16 = ck16 && adjoint && sc.nck > 0;
+    b.ring0 = b.ck16 ? BatchPlan::RING16 : BatchPlan::CK0 + sc.nck;
     const int S = b.S;
-    const long nf = fields_per_shot(sc, adjoint);
+    const long nf = fields_per_shot(sc, adjoint, b.ck16);
     b.nacc = (long)g.nx * g.nz;
     if (lanes) {
         b.Sp = (S + 63) / 64 * 64;
@@ -617,6 +630,21 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
     b.arena_floats = nf * g.fsz * b.Sp;
     b.arena = c.arena.as<float>((size_t)b.arena_floats);
     HIPCHECK(hipMemsetAsync(b.arena, 0, (size_t)b.arena_floats * sizeof(float), c.st));
+    if (b.ck16) {
+        b.h16 = c.ck16.as<__half>((size_t)sc.nck * g.fsz * b.Sp);
+        if (lanes) {
+            b.KS16 = g.fsz * b.Sp;
+            b.SS16 = 1;
+            b.CS16 = b.Sp;
+        } else {
+            b.KS16 = g.fsz;
+            b.SS16 = (long)sc.nck * g.fsz;
+            b.CS16 = 1;
+        }
+        char *aux = c.ckaux.as<char>(((size_t)sc.nck + 1) * b.Sp * 4 + 256);
+        b.ckinv = reinterpret_cast<float *>(aux);
+        b.ckmx = reinterpret_cast<unsigned *>(aux + (size_t)sc.nck * b.Sp * 4);
+    }
 
     std::vector<int> cells(b.Sp, -1);
     std::vector<float> wav((size_t)S * nt);
@@ -663,6 +691,19 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
     c.stats.tile_wpb = b.tile.wpb;
     c.stats.tile_zrun = b.tile.zrun;
     c.stats.layout = lanes ? 1 : 0;
+    c.stats.ckpt_bits = b.ck16 ? 16 : 32;
+}
+
+// FP16 checkpoint slot k <-> FP32 field f of the batch.
+static void ck_store(BatchState &b, int k, const float *f)
+{
+    launch_ck_pack(f, b.SS, b.CS, b.h16 + (long)k * b.KS16, b.SS16, b.CS16, b.g.fsz, b.S,
+                   b.lanes ? 1 : 0, b.ckmx, b.ckinv + (long)k * b.Sp, g_ctx.st);
+}
+static void ck_load(BatchState &b, int k, float *f)
+{
+    launch_ck_unpack(b.h16 + (long)k * b.KS16, b.SS16, b.CS16, f, b.SS, b.CS, b.g.fsz, b.S,
+                     b.lanes ? 1 : 0, b.ckinv + (long)k * b.Sp, g_ctx.st);
 }
 
 // propagation.c:539-608.  With save_ckpt the fields u(kd-1), u(kd) land in
@@ -683,7 +724,8 @@ static void forward_pass(BatchState &b, bool save_ckpt, bool precond)
     const float *cur = zero, *prev = zero;
     for (int it = 0; it < b.nt; ++it) {
         float *out;
-        if (save_ckpt && it + 1 < b.nt && sc.ckslot[it + 1] >= 0)
+        const bool ck = save_ckpt && it + 1 < b.nt && sc.ckslot[it + 1] >= 0;
+        if (ck && !b.ck16)
             out = b.F(BatchPlan::CK0 + sc.ckslot[it + 1]);
         else
             out = pick_free(W, cur, prev);
@@ -694,6 +736,7 @@ static void forward_pass(BatchState &b, bool save_ckpt, bool precond)
         a.it = it;
         a.acc_on = (precond && it % MST == 0) ? 1 : 0;
         fd_launch(FD_FWD, a, b.cells_per_launch, b.tile);
+        if (ck && b.ck16) ck_store(b, sc.ckslot[it + 1], out);
         prev = cur;
         cur = out;
     }
@@ -714,10 +757,11 @@ static void adjoint_pass(BatchState &b)
             where[it] = b.F(BatchPlan::ZERO);
         } else if (sc.ckslot[it] >= 0) {
             stored[it] = 1;
-            where[it] = b.F(BatchPlan::CK0 + sc.ckslot[it]);
+            // FP16 slots are restored into R0, R1 or RU when they are read
+            where[it] = b.ck16 ? nullptr : b.F(BatchPlan::CK0 + sc.ckslot[it]);
         }
     }
-    const int ring0 = BatchPlan::CK0 + sc.nck;
+    const int ring0 = b.ring0;
     float *FW[3] = {b.F(BatchPlan::F0), b.F(BatchPlan::F1), b.F(BatchPlan::F2)};
     float *AW[3] = {b.F(BatchPlan::A0), b.F(BatchPlan::A1), b.F(BatchPlan::A2)};
     // the adjoint wavefield starts from zero (propagation.c:504-506); only the
@@ -745,6 +789,12 @@ static void adjoint_pass(BatchState &b)
             const int s0 = (it / sc.d) * sc.d;
             const float *fprev = (s0 == 0) ? b.F(BatchPlan::ZERO) : where[s0 - 1];
             const float *fcur = where[s0];
+            if (b.ck16 && s0 > 0) {
+                ck_load(b, sc.ckslot[s0 - 1], b.F(BatchPlan::R0));
+                ck_load(b, sc.ckslot[s0], b.F(BatchPlan::R1));
+                fprev = b.F(BatchPlan::R0);
+                fcur = b.F(BatchPlan::R1);
+            }
             for (int itt = s0; itt < it; ++itt) {
                 const int t = itt + 1;
                 float *fout = (t % MST == 0) ? b.F(ring0 + (t - s0) / MST) : pick_free(FW, fcur, fprev);
@@ -768,6 +818,11 @@ static void adjoint_pass(BatchState &b)
         ad.it = it;
         ad.acc_on = (it % MST == 0) ? 1 : 0;
         ad.u = (it % MST == 0) ? where[it] : nullptr;
+        if (ad.acc_on && !ad.u) {
+            // an FP16 checkpoint time: restore the forward field for imaging
+            ck_load(b, sc.ckslot[it], b.F(BatchPlan::RU));
+            ad.u = b.F(BatchPlan::RU);
+        }
         fd_launch(FD_ADJ, ad, b.cells_per_launch, b.tile);
         prev = cur;
         cur = out;
@@ -777,10 +832,11 @@ static void adjoint_pass(BatchState &b)
 // ---------------------------------------------------------------------------
 // batch size
 
-static size_t per_shot_bytes(const Geom &g, const Schedule &sc, bool adjoint, long rec_floats,
-                             int nt)
+static size_t per_shot_bytes(const Geom &g, const Schedule &sc, bool adjoint, bool ck16,
+                             long rec_floats, int nt)
 {
-    size_t b = (size_t)fields_per_shot(sc, adjoint) * g.fsz * sizeof(float);
+    size_t b = (size_t)fields_per_shot(sc, adjoint, ck16) * g.fsz * sizeof(float);
+    if (adjoint && ck16) b += (size_t)sc.nck * g.fsz * sizeof(__half);
     b += (size_t)2 * rec_floats * sizeof(float) + (size_t)nt * sizeof(float);
     if (adjoint) b += (size_t)g.nx * g.nz * (sizeof(double) + 2 * sizeof(float));
     return b;
@@ -801,7 +857,7 @@ static int choose_batch(int n, size_t per_shot, bool &lanes)
         HIPCHECK(hipMemGetInfo(&fr, &tot));
         // buffers the engine already holds are reused
         size_t held = c.arena.cap + c.acc.cap + c.rec.cap + c.obs.cap + c.src.cap +
-                      c.misc[0].cap + c.misc[1].cap;
+                      c.misc[0].cap + c.misc[1].cap + c.ck16.cap;
         double budget = ((double)fr + (double)held) * c.memfrac;
         Smax = std::max<long>(1, (long)(budget / (double)per_shot));
     }
@@ -925,8 +981,9 @@ static void propagate_core(const Call &P)
         long recmax = 0;
         for (int s : grp) recmax = std::max<long>(recmax, (long)P.pre[s].ntr * P.pre[s].ns);
         bool lanes = false;
-        const int S =
-            choose_batch((int)grp.size(), per_shot_bytes(g, sc, problem == 1, recmax, nt), lanes);
+        const bool ck16 = c.ckpt_bits == 16;
+        const int S = choose_batch(
+            (int)grp.size(), per_shot_bytes(g, sc, problem == 1, ck16, recmax, nt), lanes);
         c.stats.batch = S;
         c.stats.layout = lanes ? 1 : 0;
         for (size_t b0 = 0; b0 < grp.size(); b0 += S) {
@@ -940,7 +997,7 @@ static void propagate_core(const Call &P)
                 recs[k] = pr;
             }
             BatchState b;
-            prepare_batch(b, g, coef, recs, nt, problem == 1, sc, lanes);
+            prepare_batch(b, g, coef, recs, nt, problem == 1, sc, lanes, ck16);
             float *obsd = nullptr;
             if (problem != 0) {
                 obsd = c.obs.as<float>(b.rec_total + 64);
@@ -1214,7 +1271,8 @@ extern "C" void iso_acoustic2d_propagation_engine(Record2D *record2d, Acoustic2d
     if (c.tile_req.tz > 0 && c.tile_req.kind >= 2)
         fail("<error>: iso_acoustic2d_propagation_engine runs one shot; the shots-in-lanes "
              "stencil needs a batch");
-    prepare_batch(b, g, coef, recs, nt, mode != 0, sc, false);
+    // the host checkpoint of the reference is FP32
+    prepare_batch(b, g, coef, recs, nt, mode != 0, sc, false, false);
     const long nacc = b.nacc;
     if (mode == 0 || mode == 1) {
         forward_pass(b, mode == 1, mode == 1 && precondition == 1);

@@ -47,7 +47,7 @@ struct Stats {
     double device_bytes = 0;
     double h2d_bytes = 0, d2h_bytes = 0;
     int tile_tz = 0, tile_spb = 0, tile_kind = 0, tile_txm = 0, tile_zfast = 0, tile_pd = 0, tile_remap = 0;
-    int tile_wpb = 0, tile_zrun = 0, layout = 0;
+    int tile_wpb = 0, tile_zrun = 0, layout = 0, ckpt_bits = 32;
 };
 
 struct Ctx {
@@ -80,6 +80,8 @@ struct Ctx {
     DevBuf gsum;                  // double sums
     DevBuf scratch;               // reductions
     DevBuf misc[4];
+    DevBuf ck16;                  // FP16 checkpoint slots
+    DevBuf ckaux;                 // per-slot inverse scales and max scratch
 };
 
 Ctx &ctx();

@@ -6,6 +6,8 @@
 // of the same sequence of adds and multiplies as a strict C evaluation of the
 // reference.  The stencil is HBM-bound (about 1.3 flop per byte), so the fused
 // multiply-adds would buy nothing.
+#include <hip/hip_fp16.h>
+
 #include "kernels.hpp"
 
 #include <math.h>
@@ -1372,6 +1374,125 @@ __global__ void freesurface_kernel(float *p, int Nx, int ld, int pml)
 void launch_freesurface(float *p, int Nx, int ld, int pml, hipStream_t st)
 {
     freesurface_kernel<<<(Nx + 63) / 64, 64, 0, st>>>(p, Nx, ld, pml);
+}
+
+// ---------------------------------------------------------------------------
+// FP16 checkpoint storage with a per-snapshot scale.
+//
+// A checkpoint field of shot s is stored as half(u * 2^k_s), where 2^k_s is the
+// largest power of two with max|u_s| * 2^k_s <= 65504 (k_s = 0 for an all-zero
+// field).  A power-of-two scale is exact in FP32, so the only rounding is the
+// conversion to FP16 (round to nearest even), and restoring is exact:
+// u' = float(h) * 2^-k_s.  Shot s, cell i of a field is at s*SS + i*CS
+// (per-shot arenas: SS = arena, CS = 1; shots in lanes: SS = 1, CS = Sp); the
+// FP16 slot uses strides SS16, CS16 of the same shape.
+
+__global__ void fill_u32_kernel(unsigned *p, int n, unsigned v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void launch_fill_u32(unsigned *p, int n, unsigned v, hipStream_t st)
+{
+    if (n > 0) fill_u32_kernel<<<(n + 255) / 256, 256, 0, st>>>(p, n, v);
+}
+
+// max |u| per shot, as the bit pattern of a non-negative float (ordered like
+// the float), into mx[s] (cleared to 0 beforehand).  lanes: a wave reads one
+// cell of 64 shots; otherwise block y is one shot and the block strides cells.
+__global__ void ck_maxabs_kernel(const float *f, long SS, long CS, long ncell, int nshot, int lanes,
+                                 unsigned *mx)
+{
+    float m = 0.0f;
+    int s;
+    if (lanes) {
+        s = blockIdx.y * 64 + (threadIdx.x & 63);
+        const long w0 = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        const long nw = (long)gridDim.x * (blockDim.x >> 6);
+        for (long i = w0; i < ncell; i += nw) m = fmaxf(m, fabsf(f[s * SS + i * CS]));
+    } else {
+        s = blockIdx.y;
+        for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < ncell;
+             i += (long)gridDim.x * blockDim.x)
+            m = fmaxf(m, fabsf(f[s * SS + i * CS]));
+        // wave maximum, then one atomic per wave
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+        if ((threadIdx.x & 63) != 0) return;
+    }
+    if (s < nshot) atomicMax(&mx[s], __float_as_uint(m));
+}
+
+__device__ __forceinline__ float ck_scale_of(unsigned bits)
+{
+    const float m = __uint_as_float(bits);
+    if (!(m > 0.0f)) return 1.0f;
+    int e;
+    (void)frexpf(65504.0f / m, &e);  // 65504/m = f * 2^e, f in [0.5, 1)
+    float sc = ldexpf(1.0f, e - 1);
+    if (m * sc > 65504.0f) sc *= 0.5f;  // rounding of the quotient at a power of two
+    return sc;
+}
+
+__global__ void ck_pack_kernel(const float *f, long SS, long CS, __half *h, long SS16, long CS16,
+                               long ncell, int nshot, int lanes, const unsigned *mx, float *inv)
+{
+    long i;
+    int s;
+    if (lanes) {
+        s = blockIdx.y * 64 + (threadIdx.x & 63);
+        i = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    } else {
+        s = blockIdx.y;
+        i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    }
+    if (i >= ncell || s >= nshot) return;
+    const float sc = ck_scale_of(mx[s]);
+    h[s * SS16 + i * CS16] = __float2half_rn(f[s * SS + i * CS] * sc);
+    if (i == 0) inv[s] = 1.0f / sc;  // a power of two: exact
+}
+
+__global__ void ck_unpack_kernel(const __half *h, long SS16, long CS16, float *f, long SS, long CS,
+                                 long ncell, int nshot, int lanes, const float *inv)
+{
+    long i;
+    int s;
+    if (lanes) {
+        s = blockIdx.y * 64 + (threadIdx.x & 63);
+        i = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    } else {
+        s = blockIdx.y;
+        i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    }
+    if (i >= ncell || s >= nshot) return;
+    f[s * SS + i * CS] = __half2float(h[s * SS16 + i * CS16]) * inv[s];
+}
+
+static dim3 ck_grid(long ncell, int nshot, int lanes, int reduce)
+{
+    if (lanes) {
+        const long nb = reduce ? std::min<long>((ncell + 3) / 4, 4096) : (ncell + 3) / 4;
+        return dim3((unsigned)nb, (unsigned)((nshot + 63) / 64));
+    }
+    const long nb = reduce ? std::min<long>((ncell + 255) / 256, 512) : (ncell + 255) / 256;
+    return dim3((unsigned)nb, (unsigned)nshot);
+}
+
+void launch_ck_pack(const float *f, long SS, long CS, __half *h, long SS16, long CS16, long ncell,
+                    int nshot, int lanes, unsigned *mx, float *inv, hipStream_t st)
+{
+    launch_fill_u32(mx, nshot, 0u, st);
+    ck_maxabs_kernel<<<ck_grid(ncell, nshot, lanes, 1), 256, 0, st>>>(f, SS, CS, ncell, nshot,
+                                                                        lanes, mx);
+    ck_pack_kernel<<<ck_grid(ncell, nshot, lanes, 0), 256, 0, st>>>(f, SS, CS, h, SS16, CS16,
+                                                                      ncell, nshot, lanes, mx, inv);
+}
+
+void launch_ck_unpack(const __half *h, long SS16, long CS16, float *f, long SS, long CS,
+                      long ncell, int nshot, int lanes, const float *inv, hipStream_t st)
+{
+    ck_unpack_kernel<<<ck_grid(ncell, nshot, lanes, 0), 256, 0, st>>>(h, SS16, CS16, f, SS, CS,
+                                                                        ncell, nshot, lanes, inv);
 }
 
 }  // namespace fwiedu

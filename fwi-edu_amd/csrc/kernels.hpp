@@ -7,6 +7,7 @@
 // shot s sits at  base + s * shot_stride.
 #pragma once
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -150,5 +151,16 @@ size_t reduce_scratch_doubles();
 
 // ---- small helpers -------------------------------------------------------
 void launch_freesurface(float *p, int Nx, int ld, int pml, hipStream_t st);
+void launch_fill_u32(unsigned *p, int n, unsigned v, hipStream_t st);
+
+// ---- FP16 checkpoints ------------------------------------------------------
+// Pack the FP32 field f of nshot shots (strides SS, CS, ncell cells each) into
+// FP16 h (strides SS16, CS16) with a per-shot power-of-two scale; inv[s] gets
+// the inverse scale.  mx: nshot words of scratch.  lanes selects the thread
+// mapping of the layout (shot fastest or cell fastest).
+void launch_ck_pack(const float *f, long SS, long CS, __half *h, long SS16, long CS16, long ncell,
+                    int nshot, int lanes, unsigned *mx, float *inv, hipStream_t st);
+void launch_ck_unpack(const __half *h, long SS16, long CS16, float *f, long SS, long CS,
+                      long ncell, int nshot, int lanes, const float *inv, hipStream_t st);
 
 }  // namespace fwiedu

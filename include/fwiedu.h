@@ -175,7 +175,13 @@ void fwiedu_set_memory_fraction(double frac);
 /* Sample every `every`-th reconstruction-sweep launch (the stencil with no
  * accumulation epilogue) with HIP events on the library's stream (0 = off). */
 void fwiedu_set_kernel_timing(int every);
-/* Checkpoint storage precision: 32 (default) or 16 (FP16 with a per-snapshot scale). */
+/* Checkpoint storage precision of iso_acoustic2d_propagation and sessions:
+ * 32 (default) or 16.  With 16, each checkpointed field u(kd-1), u(kd) of each
+ * shot is stored as FP16 of u * 2^k, 2^k the largest power of two that keeps
+ * max|u| * 2^k <= 65504, and restored exactly as float(h) * 2^-k; arithmetic
+ * stays FP32.  Records are unchanged, the gradient sees the rounding of the
+ * restarted segments.  The per-shot engine with a host Acoustic2dCheckPoint
+ * (the reference's FP32 structure) always stores FP32. */
 void fwiedu_set_checkpoint_bits(int bits);
 /* Named counters of the last calls: "fd_launches", "fd_timed", "fd_ms_avg",
  * "fd_cells_per_launch", "fd_shots_per_launch", "batch", "device_bytes" ...
