@@ -488,6 +488,13 @@ extern "C" void fwiedu_set_lanes(int wpb, int zrun, int pd, int lds)
     c.tile_req.zrun = zrun;
     c.tile_req.pd = pd;
 }
+extern "C" void fwiedu_set_graphs(int on)
+{
+    Ctx &c = ctx();
+    c.graphs = on ? 1 : 0;
+    if (!on && c.ready) drop_graphs();
+}
+
 extern "C" void fwiedu_set_layout(int layout)
 {
     if (layout < -1 || layout > 1)
@@ -530,6 +537,8 @@ extern "C" double fwiedu_stat(const char *name)
     if (n == "tile_zrun") return (double)s.tile_zrun;
     if (n == "layout") return (double)s.layout;
     if (n == "ckpt_bits") return (double)s.ckpt_bits;
+    if (n == "graphs_built") return (double)s.graphs_built;
+    if (n == "graph_launches") return (double)s.graph_launches;
     if (n == "h2d_bytes") return s.h2d_bytes;
     if (n == "d2h_bytes") return s.d2h_bytes;
     if (n == "device_bytes") {
@@ -552,6 +561,7 @@ extern "C" void fwiedu_release(void)
     Ctx &c = ctx();
     if (!c.ready) return;
     sync();
+    drop_graphs();
     DevBuf *all[] = {&c.model[0], &c.model[1], &c.setup, &c.coef,    &c.arena,   &c.acc,
                      &c.rec,      &c.obs,      &c.src,   &c.tables,  &c.taper,   &c.shot_out,
                      &c.gsum,     &c.scratch,  &c.misc[0], &c.misc[1], &c.misc[2], &c.misc[3],

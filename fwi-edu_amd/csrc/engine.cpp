@@ -157,7 +157,8 @@ void fd_launch(int mode, const FdArgs &a, long cells, FdTile tile)
 {
     Ctx &c = g_ctx;
     // sampled: the reconstruction sweep, the stencil with no accumulation epilogue
-    const bool timed = c.ktiming > 0 && mode == FD_RECON && (c.recon_launches++ % c.ktiming) == 0;
+    const bool timed = c.ktiming > 0 && !c.capturing && mode == FD_RECON &&
+                       (c.recon_launches++ % c.ktiming) == 0;
     if (timed) {
         if (c.ev_used + 2 > 4096) collect_timing();
         while (c.ev.size() < c.ev_used + 2) {
@@ -708,7 +709,8 @@ static void ck_load(BatchState &b, int k, float *f)
 
 // propagation.c:539-608.  With save_ckpt the fields u(kd-1), u(kd) land in
 // their checkpoint slots; with precond the pseudo-Hessian accumulates in acc.
-static void forward_pass(BatchState &b, bool save_ckpt, bool precond)
+// Kernel launches only, so a HIP graph can capture it.
+static void forward_loop(BatchState &b, bool save_ckpt, bool precond)
 {
     Ctx &c = g_ctx;
     const Schedule &sc = *b.sc;
@@ -742,8 +744,24 @@ static void forward_pass(BatchState &b, bool save_ckpt, bool precond)
     }
 }
 
+// The adjoint wavefield starts from zero (propagation.c:504-506); only the
+// three work fields of every arena are cleared, the checkpoints stay.
+static void adjoint_clear(BatchState &b)
+{
+    Ctx &c = g_ctx;
+    if (b.lanes)
+        HIPCHECK(hipMemsetAsync(b.arena, 0, 3 * b.KS * sizeof(float), c.st));
+    else
+        for (int s = 0; s < b.S; ++s)
+            HIPCHECK(
+                hipMemsetAsync(b.arena + (long)s * b.SS, 0, 3 * b.g.fsz * sizeof(float), c.st));
+    HIPCHECK(hipMemsetAsync(b.acc, 0, (size_t)b.Sp * b.nacc * sizeof(double), c.st));
+}
+
 // propagation.c:633-744.  b.rec holds the adjoint source [ntr][nt] per shot.
-static void adjoint_pass(BatchState &b)
+// Kernel launches only (the clearing is adjoint_clear), so a HIP graph can
+// capture it.
+static void adjoint_loop(BatchState &b)
 {
     Ctx &c = g_ctx;
     const Schedule &sc = *b.sc;
@@ -764,14 +782,6 @@ static void adjoint_pass(BatchState &b)
     const int ring0 = b.ring0;
     float *FW[3] = {b.F(BatchPlan::F0), b.F(BatchPlan::F1), b.F(BatchPlan::F2)};
     float *AW[3] = {b.F(BatchPlan::A0), b.F(BatchPlan::A1), b.F(BatchPlan::A2)};
-    // the adjoint wavefield starts from zero (propagation.c:504-506); only the
-    // three work fields of every arena are cleared, the checkpoints stay
-    if (b.lanes)
-        HIPCHECK(hipMemsetAsync(b.arena, 0, 3 * b.KS * sizeof(float), c.st));
-    else
-        for (int s = 0; s < b.S; ++s)
-            HIPCHECK(hipMemsetAsync(b.arena + (long)s * b.SS, 0, 3 * g.fsz * sizeof(float), c.st));
-    HIPCHECK(hipMemsetAsync(b.acc, 0, (size_t)b.Sp * b.nacc * sizeof(double), c.st));
     FdArgs fr = base_args(g, b.coef, b.S, b.SS, b.CS);
     fr.src_cell = b.cell_dev;
     fr.src_wav = b.wav_dev;
@@ -827,6 +837,104 @@ static void adjoint_pass(BatchState &b)
         prev = cur;
         cur = out;
     }
+}
+
+// ---------------------------------------------------------------------------
+// HIP graphs of the time loops.
+//
+// A time loop of a batch (forward: nt record + stencil launches; adjoint:
+// recomputation, injection and adjoint launches) depends only on the batch's
+// device pointers, sizes, schedule and launch shape, so once captured it can be
+// replayed with one hipGraphLaunch whenever the same batch comes again (the
+// next gradient of a session, the next FWI iteration, a serving job of the
+// same shape).  The key holds every value the captured kernel arguments
+// depend on; buffer contents (models, coefficients, wavelets, records) are
+// read at replay.
+
+static void key_push_geom(std::vector<long> &k, const Geom &g)
+{
+    const float fl[] = {g.dx, g.dz, g.dt};
+    uint32_t bits[3];
+    memcpy(bits, fl, sizeof(bits));
+    for (long v : {(long)g.nx, (long)g.nz, (long)g.pml, (long)g.fs, (long)g.Nx, (long)g.ld,
+                   (long)bits[0], (long)bits[1], (long)bits[2]})
+        k.push_back(v);
+}
+
+static std::vector<long> batch_key(int pass, const BatchState &b, bool f1, bool f2)
+{
+    std::vector<long> k = {pass, (long)f1, (long)f2, (long)b.nt, (long)b.S, (long)b.Sp,
+                           (long)b.lanes, (long)b.ck16, b.sc ? (long)b.sc->d : 0L,
+                           b.tb.rcv.n, b.tb.inj.n};
+    key_push_geom(k, b.g);
+    for (const void *p : {(const void *)b.arena, (const void *)b.coef, (const void *)b.cell_dev,
+                          (const void *)b.wav_dev, (const void *)b.rec, (const void *)b.acc,
+                          (const void *)b.h16, (const void *)b.ckinv, (const void *)b.ckmx,
+                          (const void *)b.tb.rcv.trace_off, (const void *)b.tb.rcv.shot,
+                          (const void *)b.tb.inj.trace_off, (const void *)b.tb.inj.shot})
+        k.push_back((long)(uintptr_t)p);
+    const FdTile &t = b.tile;
+    for (long v : {(long)t.kind, (long)t.tz, (long)t.spb, (long)t.txm, (long)t.zfast, (long)t.pd,
+                   (long)t.remap, (long)t.wpb, (long)t.zrun})
+        k.push_back(v);
+    return k;
+}
+
+// Run body on the library stream, through a cached graph when graphs are on.
+template <class F>
+static void run_loop(const std::vector<long> &key, F body)
+{
+    Ctx &c = g_ctx;
+    if (!c.graphs) {
+        body();
+        return;
+    }
+    auto it = c.graph_cache.find(key);
+    if (it == c.graph_cache.end()) {
+        if (c.graph_cache.size() >= 16) {
+            for (auto &e : c.graph_cache) (void)hipGraphExecDestroy(e.second.exec);
+            c.graph_cache.clear();
+        }
+        const Stats before = c.stats;
+        hipGraph_t gr = nullptr;
+        c.capturing = true;
+        HIPCHECK(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
+        body();
+        HIPCHECK(hipStreamEndCapture(c.st, &gr));
+        c.capturing = false;
+        GraphEntry e;
+        HIPCHECK(hipGraphInstantiate(&e.exec, gr, nullptr, nullptr, 0));
+        HIPCHECK(hipGraphDestroy(gr));
+        e.fd_launches = c.stats.fd_launches - before.fd_launches;
+        e.fd_cells = c.stats.fd_cells - before.fd_cells;
+        c.stats.graphs_built++;
+        it = c.graph_cache.emplace(key, e).first;
+        // the launches were counted while capturing; the replay below runs them
+        c.stats.fd_launches = before.fd_launches;
+        c.stats.fd_cells = before.fd_cells;
+    }
+    HIPCHECK(hipGraphLaunch(it->second.exec, c.st));
+    c.stats.fd_launches += it->second.fd_launches;
+    c.stats.fd_cells += it->second.fd_cells;
+    c.stats.graph_launches++;
+}
+
+void drop_graphs()
+{
+    Ctx &c = g_ctx;
+    for (auto &e : c.graph_cache) (void)hipGraphExecDestroy(e.second.exec);
+    c.graph_cache.clear();
+}
+
+static void forward_pass(BatchState &b, bool save_ckpt, bool precond)
+{
+    run_loop(batch_key(0, b, save_ckpt, precond), [&] { forward_loop(b, save_ckpt, precond); });
+}
+
+static void adjoint_pass(BatchState &b)
+{
+    adjoint_clear(b);
+    run_loop(batch_key(1, b, false, false), [&] { adjoint_loop(b); });
 }
 
 // ---------------------------------------------------------------------------

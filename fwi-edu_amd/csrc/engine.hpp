@@ -48,6 +48,13 @@ struct Stats {
     double h2d_bytes = 0, d2h_bytes = 0;
     int tile_tz = 0, tile_spb = 0, tile_kind = 0, tile_txm = 0, tile_zfast = 0, tile_pd = 0, tile_remap = 0;
     int tile_wpb = 0, tile_zrun = 0, layout = 0, ckpt_bits = 32;
+    long graphs_built = 0, graph_launches = 0;
+};
+
+struct GraphEntry {
+    hipGraphExec_t exec = nullptr;
+    long fd_launches = 0;     // stencil launches and cells one replay runs
+    long fd_cells = 0;
 };
 
 struct Ctx {
@@ -61,6 +68,9 @@ struct Ctx {
     int ckpt_bits = 32;
     FdTile tile_req = {0, 1};     // tz == 0: autotune
     int layout_req = -1;          // -1 automatic, 0 per-shot arenas, 1 shots in lanes
+    int graphs = 0;               // replay time loops from captured HIP graphs
+    bool capturing = false;
+    std::map<std::vector<long>, GraphEntry> graph_cache;
     std::map<std::tuple<int, int, int, int, int, int, int, int, long>, FdTile> tuned;
     Stats stats;
     std::vector<hipEvent_t> ev;   // pool of events for sampled kernel timing
@@ -108,5 +118,6 @@ void adjoint_source_device(const Record2D *r, float **vp_full, float freq, int f
 void fd_launch(int mode, const FdArgs &a, long cells, FdTile tile);
 FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const float *prev);
 void collect_timing();
+void drop_graphs();
 
 }  // namespace fwiedu

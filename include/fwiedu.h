@@ -166,6 +166,12 @@ void fwiedu_set_xcd_remap(int on);
  * Applies to iso_acoustic2d_propagation and sessions; fwiedu_set_tile(0, 0)
  * returns to the autotuner. */
 void fwiedu_set_lanes(int wpb, int zrun, int pd, int lds);
+/* 1: run each batch's forward and adjoint time loops as HIP graphs, captured
+ * on first use and replayed whenever the same batch (same device buffers,
+ * sizes, schedule and launch shape) comes again: the next gradient of a
+ * session, the next FWI iteration.  0 (default): launch kernel by kernel.
+ * Sampled kernel timing skips captured launches. */
+void fwiedu_set_graphs(int on);
 /* Field layout of batched launches when the stencil shape is autotuned: -1
  * (default) shots in lanes from 48 shots per launch up, per-shot arenas below;
  * 0 always per-shot arenas; 1 always shots in lanes. */

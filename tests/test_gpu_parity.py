@@ -362,3 +362,38 @@ def test_lanes_layout_matches_arenas(L):
         pre_l, g_l = out[name]
         assert np.array_equal(g_l, g_a), name
         assert np.array_equal(recs(pre_l), recs(pre_a)), name
+
+
+def test_graph_replays_match_eager(L):
+    """Time loops replayed from captured HIP graphs give the eager result bit
+    for bit on every replay, with eager work and synchronisations between the
+    replays, for per-shot arenas (one shot per launch and 4 per launch) and for
+    shots in lanes, FP32 and FP16 checkpoints."""
+    w = synth.workload("tiny", nshot=4, freesurface=1, shot_depth_cells=1)
+    w2 = synth.workload("tiny", nshot=2, freesurface=0)
+    say("case: graph replays tiny fs=1 4 shots")
+    obs = observed(L, w, True)
+    obs2 = observed(L, w2, True)
+    try:
+        for batch, layout, bits in ((1, 0, 32), (0, 0, 32), (0, 1, 32), (0, 1, 16)):
+            L.fwiedu_set_batch(batch)
+            L.fwiedu_set_layout(layout)
+            L.fwiedu_set_checkpoint_bits(bits)
+            L.fwiedu_set_graphs(0)
+            pre_e, g_e = gradient(L, w, obs, True)
+            L.fwiedu_set_graphs(1)
+            L.fwiedu_reset_stats()
+            for rep in range(3):
+                pre_r, g_r = gradient(L, w, obs, True)
+                assert np.array_equal(g_r, g_e), (batch, layout, bits, rep)
+                assert np.array_equal(recs(pre_r), recs(pre_e)), (batch, layout, bits, rep)
+                # eager work of another shape between the replays
+                L.fwiedu_set_graphs(0)
+                gradient(L, w2, obs2, True)
+                L.fwiedu_set_graphs(1)
+            assert flib.stat("graph_launches") >= 6
+    finally:
+        L.fwiedu_set_graphs(0)
+        L.fwiedu_set_batch(0)
+        L.fwiedu_set_layout(-1)
+        L.fwiedu_set_checkpoint_bits(32)
