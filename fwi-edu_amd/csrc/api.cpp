@@ -561,12 +561,7 @@ extern "C" void fwiedu_release(void)
     Ctx &c = ctx();
     if (!c.ready) return;
     sync();
-    drop_graphs();
-    DevBuf *all[] = {&c.model[0], &c.model[1], &c.setup, &c.coef,    &c.arena,   &c.acc,
-                     &c.rec,      &c.obs,      &c.src,   &c.tables,  &c.taper,   &c.shot_out,
-                     &c.gsum,     &c.scratch,  &c.misc[0], &c.misc[1], &c.misc[2], &c.misc[3],
-                     &c.ck16,     &c.ckaux};
-    for (DevBuf *d : all) d->release();
+    release_buffers(c);
 }
 
 extern "C" int fwiedu_device_ok(void)

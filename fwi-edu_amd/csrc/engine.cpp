@@ -49,12 +49,16 @@ void DevBuf::release()
     cap = 0;
 }
 
-static Ctx g_ctx;
-Ctx &ctx() { return g_ctx; }
+// The engine state of the calling thread: the process-wide default context,
+// or the one a serving worker bound to its thread.
+static Ctx g_default;
+static thread_local Ctx *t_bound = nullptr;
+Ctx &ctx() { return t_bound ? *t_bound : g_default; }
+void bind_ctx(Ctx *c) { t_bound = c; }
 
 void ensure_ready()
 {
-    Ctx &c = g_ctx;
+    Ctx &c = ctx();
     if (c.ready) return;
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
@@ -64,23 +68,47 @@ void ensure_ready()
     c.ready = true;
 }
 
-void sync() { HIPCHECK(hipStreamSynchronize(g_ctx.st)); }
+static std::vector<DevBuf *> all_buffers(Ctx &c)
+{
+    return {&c.model[0], &c.model[1], &c.setup,   &c.coef,    &c.arena,   &c.acc,
+            &c.rec,      &c.obs,      &c.src,     &c.tables,  &c.taper,   &c.shot_out,
+            &c.gsum,     &c.scratch,  &c.misc[0], &c.misc[1], &c.misc[2], &c.misc[3],
+            &c.ck16,     &c.ckaux};
+}
+
+size_t held_bytes(const Ctx &c)
+{
+    size_t n = 0;
+    for (DevBuf *d : all_buffers(const_cast<Ctx &>(c))) n += d->cap;
+    return n;
+}
+
+// Free every device buffer of c (its stream must be idle); graphs that
+// captured them go too.
+void release_buffers(Ctx &c)
+{
+    for (auto &e : c.graph_cache) (void)hipGraphExecDestroy(e.second.exec);
+    c.graph_cache.clear();
+    for (DevBuf *d : all_buffers(c)) d->release();
+}
+
+void sync() { HIPCHECK(hipStreamSynchronize(ctx().st)); }
 
 // Host-to-device copies complete before returning: sources are often
 // temporaries, and pageable memory gives no asynchrony to keep.
 void h2d(void *dst, const void *src, size_t bytes)
 {
     if (!bytes) return;
-    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_ctx.st));
-    HIPCHECK(hipStreamSynchronize(g_ctx.st));
-    g_ctx.stats.h2d_bytes += (double)bytes;
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx().st));
+    HIPCHECK(hipStreamSynchronize(ctx().st));
+    ctx().stats.h2d_bytes += (double)bytes;
 }
 
 void d2h(void *dst, const void *src, size_t bytes)
 {
     if (!bytes) return;
-    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_ctx.st));
-    g_ctx.stats.d2h_bytes += (double)bytes;
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx().st));
+    ctx().stats.d2h_bytes += (double)bytes;
 }
 
 Geom make_geom(int nx, int nz, int pml, int fs, float dx, float dz, float dt)
@@ -133,7 +161,7 @@ float meal_lnR(int nl, float pd)
 // arithmetic runs on the device.
 void build_coefs(const Geom &g, const float *vpw, const float *rhow, float fpeak, float *coef)
 {
-    Ctx &c = g_ctx;
+    Ctx &c = ctx();
     const int nl = g.pml;
     float *base = c.setup.as<float>(4 * g.fsz + nl + 1 + 64);
     float *vp = base, *rho = base + g.fsz, *D = base + 2 * g.fsz, *Beta = base + 3 * g.fsz;
@@ -155,7 +183,7 @@ void build_coefs(const Geom &g, const float *vpw, const float *rhow, float fpeak
 
 void fd_launch(int mode, const FdArgs &a, long cells, FdTile tile)
 {
-    Ctx &c = g_ctx;
+    Ctx &c = ctx();
     // sampled: the reconstruction sweep, the stencil with no accumulation epilogue
     const bool timed = c.ktiming > 0 && !c.capturing && mode == FD_RECON &&
                        (c.recon_launches++ % c.ktiming) == 0;
@@ -181,7 +209,7 @@ void fd_launch(int mode, const FdArgs &a, long cells, FdTile tile)
 
 void collect_timing()
 {
-    Ctx &c = g_ctx;
+    Ctx &c = ctx();
     if (c.ev_cells.empty()) return;
     sync();
     for (auto &pc : c.ev_cells) {
@@ -205,7 +233,7 @@ void collect_timing()
 
 FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const float *prev)
 {
-    Ctx &c = g_ctx;
+    Ctx &c = ctx();
     const bool lanes = base.cell_stride > 1;
     if (c.tile_req.tz > 0) {
         if ((c.tile_req.kind >= 2) != lanes)
@@ -379,7 +407,7 @@ struct Tables {
 static Tables build_tables(const std::vector<ShotIn> &shots, const Geom &g, std::vector<long> &roff,
                            long &rec_total)
 {
-    Ctx &c = g_ctx;
+    Ctx &c = ctx();
     const int S = (int)shots.size();
     roff.assign(S, 0);
     rec_total = 0;
@@ -519,7 +547,7 @@ static std::vector<float> taper_table(int &L, int n)
 void adjoint_source_device(const Record2D *r, float **vp_full, float freq, int filter,
                            const float *pre_dev, const float *obs_dev, float *out_dev)
 {
-    Ctx &c = g_ctx;
+    Ctx &c = ctx();
     if (filter == 0 || filter == 1)
         fail("<error>: compute_adjoint_source2d: filter %d (FFT low/band-pass) is not available "
              "in this build; use filter=2",
@@ -600,7 +628,7 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
                           const std::vector<Record2D *> &recs, int nt, bool adjoint, Schedule &sc,
                           bool lanes, bool ck16)
 {
-    Ctx &c = g_ctx;
+    Ctx &c = ctx();
     b.g = g;
     b.coef = coef;
     b.S = (int)recs.size();
@@ -699,12 +727,12 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
 static void ck_store(BatchState &b, int k, const float *f)
 {
     launch_ck_pack(f, b.SS, b.CS, b.h16 + (long)k * b.KS16, b.SS16, b.CS16, b.g.fsz, b.S,
-                   b.lanes ? 1 : 0, b.ckmx, b.ckinv + (long)k * b.Sp, g_ctx.st);
+                   b.lanes ? 1 : 0, b.ckmx, b.ckinv + (long)k * b.Sp, ctx().st);
 }
 static void ck_load(BatchState &b, int k, float *f)
 {
     launch_ck_unpack(b.h16 + (long)k * b.KS16, b.SS16, b.CS16, f, b.SS, b.CS, b.g.fsz, b.S,
-                     b.lanes ? 1 : 0, b.ckinv + (long)k * b.Sp, g_ctx.st);
+                     b.lanes ? 1 : 0, b.ckinv + (long)k * b.Sp, ctx().st);
 }
 
 // propagation.c:539-608.  With save_ckpt the fields u(kd-1), u(kd) land in
@@ -712,7 +740,7 @@ static void ck_load(BatchState &b, int k, float *f)
 // Kernel launches only, so a HIP graph can capture it.
 static void forward_loop(BatchState &b, bool save_ckpt, bool precond)
 {
-    Ctx &c = g_ctx;
+    Ctx &c = ctx();
     const Schedule &sc = *b.sc;
     FdArgs a = base_args(b.g, b.coef, b.S, b.SS, b.CS);
     a.src_cell = b.cell_dev;
@@ -748,7 +776,7 @@ static void forward_loop(BatchState &b, bool save_ckpt, bool precond)
 // three work fields of every arena are cleared, the checkpoints stay.
 static void adjoint_clear(BatchState &b)
 {
-    Ctx &c = g_ctx;
+    Ctx &c = ctx();
     if (b.lanes)
         HIPCHECK(hipMemsetAsync(b.arena, 0, 3 * b.KS * sizeof(float), c.st));
     else
@@ -763,7 +791,7 @@ static void adjoint_clear(BatchState &b)
 // capture it.
 static void adjoint_loop(BatchState &b)
 {
-    Ctx &c = g_ctx;
+    Ctx &c = ctx();
     const Schedule &sc = *b.sc;
     const Geom &g = b.g;
     const int nt = b.nt;
@@ -884,7 +912,7 @@ static std::vector<long> batch_key(int pass, const BatchState &b, bool f1, bool 
 template <class F>
 static void run_loop(const std::vector<long> &key, F body)
 {
-    Ctx &c = g_ctx;
+    Ctx &c = ctx();
     if (!c.graphs) {
         body();
         return;
@@ -921,7 +949,7 @@ static void run_loop(const std::vector<long> &key, F body)
 
 void drop_graphs()
 {
-    Ctx &c = g_ctx;
+    Ctx &c = ctx();
     for (auto &e : c.graph_cache) (void)hipGraphExecDestroy(e.second.exec);
     c.graph_cache.clear();
 }
@@ -956,17 +984,23 @@ static size_t per_shot_bytes(const Geom &g, const Schedule &sc, bool adjoint, bo
 // shape is forced.
 static int choose_batch(int n, size_t per_shot, bool &lanes)
 {
-    Ctx &c = g_ctx;
+    Ctx &c = ctx();
     long Smax;
     if (c.batch_req > 0) {
         Smax = c.batch_req;
     } else {
-        size_t fr = 0, tot = 0;
-        HIPCHECK(hipMemGetInfo(&fr, &tot));
-        // buffers the engine already holds are reused
-        size_t held = c.arena.cap + c.acc.cap + c.rec.cap + c.obs.cap + c.src.cap +
-                      c.misc[0].cap + c.misc[1].cap + c.ck16.cap;
-        double budget = ((double)fr + (double)held) * c.memfrac;
+        double budget;
+        if (c.mem_cap > 0) {
+            // a serving worker: the batch buffers must fit its reservation
+            budget = (double)c.mem_cap;
+        } else {
+            size_t fr = 0, tot = 0;
+            HIPCHECK(hipMemGetInfo(&fr, &tot));
+            // buffers the engine already holds are reused
+            size_t held = c.arena.cap + c.acc.cap + c.rec.cap + c.obs.cap + c.src.cap +
+                          c.misc[0].cap + c.misc[1].cap + c.ck16.cap;
+            budget = ((double)fr + (double)held) * c.memfrac;
+        }
         Smax = std::max<long>(1, (long)(budget / (double)per_shot));
     }
     long S = std::min<long>(n, Smax);
@@ -1021,7 +1055,7 @@ static std::vector<std::vector<int>> group_shots(const Record2D *obs, int Nshot,
     return groups;
 }
 
-static void check_common(int nz, int nx, float dx, float dz, float dt, int nt, int pml, int fs)
+void check_common(int nz, int nx, float dx, float dz, float dt, int nt, int pml, int fs)
 {
     if (nz < 4 || nx < 1) fail("<error>: model too small (nz=%d, nx=%d)", nz, nx);
     if (nt < 1) fail("<error>: nt must be positive (nt=%d)", nt);
@@ -1032,27 +1066,36 @@ static void check_common(int nz, int nx, float dx, float dz, float dt, int nt, i
 }
 
 
-// The shot loop of iso_acoustic2d_propagation (propagation.c:854-1035) with
-// the choice of where the observed records come from and whether the
-// predicted / adjoint-source records go back to the host.
-struct Call {
-    float **vp, **rho;          // host models (vp also feeds the adjoint-source taper)
-    Record2D *obs, *pre;
-    int Nshot, nz, nx;
-    float dx, dz, dt;
-    int nt, pml, fs;
-    float freq;
-    int filter, precondition, problem;
-    float **grad;               // host gradient (problem 1), may be null with grad_dev
-    const float *obs_dev = nullptr;   // all observed records on the device (session)
-    const long *obs_off = nullptr;    // per-shot offsets into obs_dev
-    bool records_to_host = true;
-    float *grad_dev = nullptr;        // device gradient output (session)
-    double *misfit = nullptr;         // 0.5 * sum of squared adjoint sources (problem 1/2)
-    bool models_on_device = false;    // c.model[] already hold vp, rho
-};
+// Device bytes a call of propagate_core needs: `fixed` for the whole call and
+// `per_shot` for every shot advanced in the same launch (the batcher's own
+// accounting plus the per-call buffers), so that a serving worker can reserve
+// fixed + per_shot * S and run batches of S shots inside it.
+void call_footprint(const Call &P, bool ck16, size_t *fixed, size_t *per_shot)
+{
+    const long nmod = (long)P.nx * P.nz;
+    long win_max = 0, recmax = 0, rec_total = 0, ntr_total = 0;
+    for (int s = 0; s < P.Nshot; ++s) {
+        win_max = std::max<long>(win_max, (long)(P.obs[s].ix_max - P.obs[s].ix_min + 1));
+        recmax = std::max<long>(recmax, (long)P.obs[s].ntr * P.obs[s].ns);
+        rec_total += (long)P.obs[s].ntr * P.obs[s].ns;
+        ntr_total += P.obs[s].ntr;
+    }
+    Geom g = make_geom((int)win_max, P.nz, P.pml, P.fs, P.dx, P.dz, P.dt);
+    Schedule sc(P.nt);
+    const bool grad = P.problem == 1;
+    size_t f = (size_t)2 * nmod * 4;                                     // models
+    f += (size_t)(CO_N + 4) * g.fsz * 4;                               // coefficients + setup
+    f += (size_t)rec_total * 4 + (size_t)ntr_total * 32;               // observed records, tables
+    if (grad) {
+        f += (size_t)2 * P.Nshot * win_max * P.nz * 4;                 // per-shot outputs
+        f += (size_t)2 * nmod * (8 + 4) + (size_t)2 * nmod * 4;        // sums, float copies
+    }
+    f += (size_t)64 << 20;                                             // reductions, slack
+    *fixed = f;
+    *per_shot = per_shot_bytes(g, sc, grad, ck16, recmax, P.nt) + (size_t)P.nt * 4 + 4096;
+}
 
-static void propagate_core(const Call &P)
+void propagate_core(const Call &P)
 {
     Ctx &c = ctx();
     const int Nshot = P.Nshot, nz = P.nz, nx = P.nx, nt = P.nt, problem = P.problem;

@@ -69,6 +69,7 @@ struct Ctx {
     FdTile tile_req = {0, 1};     // tz == 0: autotune
     int layout_req = -1;          // -1 automatic, 0 per-shot arenas, 1 shots in lanes
     int graphs = 0;               // replay time loops from captured HIP graphs
+    size_t mem_cap = 0;           // serving worker: device bytes its batches may use (0: free memory)
     bool capturing = false;
     std::map<std::vector<long>, GraphEntry> graph_cache;
     std::map<std::tuple<int, int, int, int, int, int, int, int, long>, FdTile> tuned;
@@ -95,6 +96,7 @@ struct Ctx {
 };
 
 Ctx &ctx();
+void bind_ctx(Ctx *c);  // make c the calling thread's engine context (nullptr: the default)
 void ensure_ready();
 void sync();
 void h2d(void *dst, const void *src, size_t bytes);
@@ -119,5 +121,31 @@ void fd_launch(int mode, const FdArgs &a, long cells, FdTile tile);
 FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const float *prev);
 void collect_timing();
 void drop_graphs();
+
+// The shot loop of iso_acoustic2d_propagation (propagation.c:854-1035) with
+// the choice of where the observed records come from and whether the
+// predicted / adjoint-source records go back to the host.
+struct Call {
+    float **vp, **rho;          // host models (vp also feeds the adjoint-source taper)
+    Record2D *obs, *pre;
+    int Nshot, nz, nx;
+    float dx, dz, dt;
+    int nt, pml, fs;
+    float freq;
+    int filter, precondition, problem;
+    float **grad;               // host gradient (problem 1), may be null with grad_dev
+    const float *obs_dev = nullptr;   // all observed records on the device (session)
+    const long *obs_off = nullptr;    // per-shot offsets into obs_dev
+    bool records_to_host = true;
+    float *grad_dev = nullptr;        // device gradient output (session)
+    double *misfit = nullptr;         // 0.5 * sum of squared adjoint sources (problem 1/2)
+    bool models_on_device = false;    // c.model[] already hold vp, rho
+};
+
+void propagate_core(const Call &P);
+void call_footprint(const Call &P, bool ck16, size_t *fixed, size_t *per_shot);
+void check_common(int nz, int nx, float dx, float dz, float dt, int nt, int pml, int fs);
+size_t held_bytes(const Ctx &c);
+void release_buffers(Ctx &c);
 
 }  // namespace fwiedu

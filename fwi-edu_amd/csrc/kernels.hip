@@ -627,7 +627,6 @@ __global__ __launch_bounds__(512) void fd_lanes_kernel(FdArgs a)
     for (int p = 0; p < PD; ++p) load_row(min(iz0 + p, iz1), nx_[p]);
 #pragma unroll 2
     for (int iz = iz0; iz <= iz1; ++iz) {
-        const long i = colb + iz;
         float r[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) r[k] = nx_[0][k];

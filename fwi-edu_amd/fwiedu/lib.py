@@ -114,6 +114,8 @@ EXPORTED = [
     "fwiedu_session_destroy", "fwiedu_set_batch", "fwiedu_set_tile", "fwiedu_set_march", "fwiedu_set_tile_order", "fwiedu_set_xcd_remap", "fwiedu_set_lanes", "fwiedu_set_layout", "fwiedu_set_graphs", "fwiedu_set_memory_fraction", "fwiedu_set_kernel_timing",
     "fwiedu_set_checkpoint_bits", "fwiedu_stat", "fwiedu_reset_stats", "fwiedu_release",
     "fwiedu_device_ok", "fwiedu_version",
+    "fwiedu_server_create", "fwiedu_server_submit", "fwiedu_server_done", "fwiedu_server_wait",
+    "fwiedu_server_stat", "fwiedu_server_destroy",
 ]
 
 

@@ -207,6 +207,47 @@ void fwiedu_session_set_model(void *session, float **vp, float **rho);
  * the processed residual).  grad may be NULL (the gradient stays on the device). */
 float fwiedu_session_gradient(void *session, float **grad);
 void fwiedu_session_destroy(void *session);
+
+/* ---- Serving loop (no counterpart in the reference) ------------------------
+ * A job is one iso_acoustic2d_propagation call of problem 0 (modeling: records
+ * into pre) or 1 (gradient: adjoint sources into pre, gradient into grad),
+ * filter 2.  The caller keeps every array of a job alive until
+ * fwiedu_server_wait returns for it. */
+typedef struct FwieduJob {
+    int problem;
+    float **vp, **rho;   /* [nx][nz] row tables over one block, as the reference */
+    Record2D *obs;       /* geometry, and the observed traces of a gradient job */
+    Record2D *pre;       /* output traces */
+    float **grad;        /* output gradient (problem 1) */
+    int nshot, nz, nx;
+    float dx, dz, dt;
+    int nt, pml, freesurface;
+    float freq;
+    int precondition;
+} FwieduJob;
+/* Start a server with `slots` workers (1-16), each a host thread with its own
+ * HIP stream and device buffers, sharing mem_fraction of the free device
+ * memory.  Jobs are admitted in arrival order as a worker and their device
+ * footprint become free (a later, smaller job may pass a waiting one during
+ * its first second of waiting); the kernels of running jobs overlap on the
+ * device.  graphs: replay the time loops of repeated job shapes from HIP
+ * graphs.  ckpt_bits: checkpoint storage of gradient jobs (32 or 16). */
+void *fwiedu_server_create(int slots, double mem_fraction, int graphs, int ckpt_bits);
+/* Queue a job; returns its id.  Invalid jobs end the process like err(). */
+long fwiedu_server_submit(void *server, const FwieduJob *job);
+/* 1 once the job has completed. */
+int fwiedu_server_done(void *server, long id);
+/* Block until the job completes; latency_s = submit to completion, queue_s =
+ * submit to start, misfit = 0.5 * sum of squared adjoint sources (gradient
+ * jobs).  Any pointer may be NULL.  Returns 0, or -1 for an unknown id. */
+int fwiedu_server_wait(void *server, long id, double *latency_s, double *queue_s,
+                       double *misfit);
+/* "completed", "queued", "peak_concurrency" (most jobs running at once),
+ * "budget_bytes", "p50_s", "p99_s", "max_s" (latencies of completed jobs). */
+double fwiedu_server_stat(void *server, const char *name);
+/* Finish the queued jobs, stop the workers, free their device memory. */
+void fwiedu_server_destroy(void *server);
+
 /* Free all device memory held by the library. */
 void fwiedu_release(void);
 /* 1 if a HIP device is present and usable. */

@@ -21,7 +21,7 @@ HEADER = os.path.join(ROOT, "include", "fwiedu.h")
 def header_functions():
     txt = open(HEADER).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    names = re.findall(r"^\s*(?:const\s+)?(?:void|int|float|double|char)\s*\**\s*(\w+)\s*\(", txt,
+    names = re.findall(r"^\s*(?:const\s+)?(?:void|int|long|float|double|char)\s*\**\s*(\w+)\s*\(", txt,
                        flags=re.M)
     return sorted(set(names))
 
