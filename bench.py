@@ -128,6 +128,37 @@ def copy_calibration(mb):
     log(f"copy calibration: 3 x {mb} MiB device-to-device")
 
 
+def serving_leg(args):
+    """Continuous-batching serving loop: a seeded Poisson stream of modeling /
+    gradient requests (70/30) on random-layer models, grids uniform in 128-1024
+    per dimension at 10 m, nt uniform in 1000-4000, 1-64 shots, submitted at
+    their arrival times to a server with `--serving-slots` workers."""
+    from fwiedu import serving
+    t0 = time.perf_counter()
+    stream = serving.request_stream(args.serving, args.serving_rate, 303)
+    log(f"serving: {args.serving} requests built in {time.perf_counter() - t0:.1f} s")
+    srv = serving.Server(slots=args.serving_slots, mem_fraction=0.8, graphs=True)
+    try:
+        jobs, wall = serving.run_stream(srv, stream, log=log)
+        out = {"requests": args.serving, "rate_per_s": args.serving_rate,
+               "slots": args.serving_slots,
+               "gradient_jobs": sum(1 for j in jobs if j.problem == 1),
+               "shots": sum(j.shots() for j in jobs), "wall_s": wall,
+               "shots_per_s": sum(j.shots() for j in jobs) / wall,
+               "jobs_per_s": len(jobs) / wall,
+               "p50_latency_s": srv.stat("p50_s"), "p99_latency_s": srv.stat("p99_s"),
+               "max_latency_s": srv.stat("max_s"),
+               "mean_queue_s": float(np.mean([j.queue for j in jobs])),
+               "peak_concurrent_jobs": int(srv.stat("peak_concurrency")),
+               "stream": "seed 303; grids 128-1024 x 128-1024 at 10 m, dt 0.8 ms, Ricker 10 Hz; "
+                         "nt 1000-4000; 1-64 shots; 30 % gradient jobs"}
+    finally:
+        srv.close()
+    log(f"serving: p50 {out['p50_latency_s']:.3f} s, p99 {out['p99_latency_s']:.3f} s, "
+        f"{out['shots_per_s']:.1f} shots/s")
+    return out
+
+
 def cpu_baseline(w, nshots):
     """The FP32 oracle (reference algorithm and compiler flags, OpenMP over all
     host threads) on `nshots` gradient shots of the same workload."""
@@ -169,6 +200,10 @@ def main():
     ap.add_argument("--no-host", action="store_true",
                     help="skip the host-array (PCIe-inclusive) iso_acoustic2d_propagation timing")
     ap.add_argument("--ktime-every", type=int, default=11)
+    ap.add_argument("--serving", type=int, default=48,
+                    help="requests of the serving leg (0 = skip; 2000 = the full stream)")
+    ap.add_argument("--serving-rate", type=float, default=3.0, help="Poisson arrivals per second")
+    ap.add_argument("--serving-slots", type=int, default=4)
     ap.add_argument("--ckpt-bits", type=int, default=32, choices=(32, 16),
                     help="checkpoint storage: FP32 or FP16 with a per-snapshot scale")
     ap.add_argument("--lanes", default="",
@@ -237,6 +272,8 @@ def main():
                  "sample": f"{args.plain_shots} shots, 1 step"}
         log(f"plain: {ps:.2f} shots/s")
 
+    serve = serving_leg(args) if args.serving > 0 else None
+
     if args.copy_mb:
         copy_calibration(args.copy_mb)
     cpu = None if args.no_cpu else cpu_baseline(w, args.cpu_shots)
@@ -269,6 +306,7 @@ def main():
         "plain": plain,
         "batched_over_plain": (shots_per_s / plain["gradient_shots_per_s"]) if plain else None,
         "host_boundary": host,
+        "serving": serve,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": measured_traffic(w, st),
