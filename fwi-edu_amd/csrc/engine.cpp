@@ -73,7 +73,7 @@ static std::vector<DevBuf *> all_buffers(Ctx &c)
     return {&c.model[0], &c.model[1], &c.setup,   &c.coef,    &c.arena,   &c.acc,
             &c.rec,      &c.obs,      &c.src,     &c.tables,  &c.taper,   &c.shot_out,
             &c.gsum,     &c.scratch,  &c.misc[0], &c.misc[1], &c.misc[2], &c.misc[3],
-            &c.ck16,     &c.ckaux};
+            &c.ck16,     &c.ckaux,    &c.fftspec, &c.fftwin,  &c.ffttmp};
 }
 
 size_t held_bytes(const Ctx &c)
@@ -89,6 +89,7 @@ void release_buffers(Ctx &c)
 {
     for (auto &e : c.graph_cache) (void)hipGraphExecDestroy(e.second.exec);
     c.graph_cache.clear();
+    release_fft(c);
     for (DevBuf *d : all_buffers(c)) d->release();
 }
 
@@ -548,10 +549,6 @@ void adjoint_source_device(const Record2D *r, float **vp_full, float freq, int f
                            const float *pre_dev, const float *obs_dev, float *out_dev)
 {
     Ctx &c = ctx();
-    if (filter == 0 || filter == 1)
-        fail("<error>: compute_adjoint_source2d: filter %d (FFT low/band-pass) is not available "
-             "in this build; use filter=2",
-             filter);
     const int ntr = r->ntr, ns = r->ns;
     double vsum = 0.0;
     for (int itr = 0; itr < ntr; ++itr) vsum += vp_full[r->igx[itr]][r->igz[itr]];
@@ -562,15 +559,25 @@ void adjoint_source_device(const Record2D *r, float **vp_full, float freq, int f
     int Lx = (int)roundf((2.0f / kpass) / r->dtr) + 1;
     std::vector<float> cT = taper_table(Lt, ns), cX = taper_table(Lx, ntr);
     float *tb = c.taper.as<float>(cT.size() + cX.size() + 64);
-    if (cT.empty() && cX.empty()) {
-        launch_adjsrc(pre_dev, obs_dev, out_dev, ntr, ns, tb, -1, tb, -1, c.st);
-        sync();
-        return;
+    if (!cT.empty()) h2d(tb, cT.data(), cT.size() * sizeof(float));
+    if (!cX.empty()) h2d(tb + cT.size(), cX.data(), cX.size() * sizeof(float));
+    const float *tT = tb, *tX = tb + cT.size();
+    if (filter == 0 || filter == 1) {
+        // optimization.c:368-415: taper in t, filter in t (freq, 2 freq), taper
+        // in x, transpose, filter in x (kpass, 2 kpass), transpose back
+        launch_adjsrc(pre_dev, obs_dev, out_dev, ntr, ns, tT, Lt, tX, -1, c.st);
+        filter_traces_device(filter, out_dev, ns, ntr, freq, 2.0f * freq, r->dt);
+        launch_taper(out_dev, ntr, ns, tT, -1, tX, Lx, c.st);
+        float *tr = c.ffttmp.as<float>((size_t)ntr * ns + 64);
+        launch_transpose(out_dev, tr, ntr, ns, c.st);
+        filter_traces_device(filter, tr, ntr, ns, kpass, 2.0f * kpass, r->dtr);
+        launch_transpose(tr, out_dev, ns, ntr, c.st);
+        // optimization.c:417-418: the final taper in t and x
+        launch_taper(out_dev, ntr, ns, tT, Lt, tX, Lx, c.st);
+    } else {
+        launch_adjsrc(pre_dev, obs_dev, out_dev, ntr, ns, tT, Lt, tX, Lx, c.st);
     }
-    h2d(tb, cT.data(), cT.size() * sizeof(float));
-    h2d(tb + cT.size(), cX.data(), cX.size() * sizeof(float));
-    launch_adjsrc(pre_dev, obs_dev, out_dev, ntr, ns, tb, Lt, tb + cT.size(), Lx, c.st);
-    // the taper table must stay valid until the kernel ran
+    // the taper table must stay valid until the kernels ran
     sync();
 }
 

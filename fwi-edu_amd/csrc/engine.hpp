@@ -93,6 +93,8 @@ struct Ctx {
     DevBuf misc[4];
     DevBuf ck16;                  // FP16 checkpoint slots
     DevBuf ckaux;                 // per-slot inverse scales and max scratch
+    DevBuf fftspec, fftwin, ffttmp;   // trace filters
+    std::map<std::pair<int, int>, std::pair<void *, void *>> fft_plans;  // (n, batch) -> R2C, C2R
 };
 
 Ctx &ctx();
@@ -147,5 +149,11 @@ void call_footprint(const Call &P, bool ck16, size_t *fixed, size_t *per_shot);
 void check_common(int nz, int nx, float dx, float dz, float dt, int nt, int pml, int fs);
 size_t held_bytes(const Ctx &c);
 void release_buffers(Ctx &c);
+
+// FFT band filters (filters.cpp): kind 0 low-pass, 1 band-pass; x is
+// [batch][n] on the device, filtered in place.
+std::vector<float> filter_window(int kind, int n, float fpass, float fcut, float dt);
+void filter_traces_device(int kind, float *x, int n, int batch, float fpass, float fcut, float dt);
+void release_fft(Ctx &c);
 
 }  // namespace fwiedu

@@ -1076,6 +1076,88 @@ void launch_adjsrc(const float *pre, const float *obs, float *out, int ntr, int 
                                                                 coefX, Lx);
 }
 
+// In-place cosine tapers of [ntr][ns] traces (taper_traces common.c:796-895):
+// time taper first, then the offset taper, each skipped when its length is < 0.
+__global__ void taper_kernel(float *x, int ntr, int ns, const float *cT, int Lt, const float *cX,
+                             int Lx)
+{
+    const long n = (long)ntr * ns;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int itr = (int)(e / ns), it = (int)(e - (long)itr * ns);
+    float r = x[e];
+    if (Lt >= 0) {
+        if (it <= Lt) r *= cT[Lt - it];
+        if (it >= ns - Lt - 1) r *= cT[it - (ns - Lt - 1)];
+    }
+    if (Lx >= 0) {
+        if (itr <= Lx) r *= cX[Lx - itr];
+        if (itr >= ntr - Lx - 1) r *= cX[itr - (ntr - Lx - 1)];
+    }
+    x[e] = r;
+}
+
+void launch_taper(float *x, int ntr, int ns, const float *coefT, int Lt, const float *coefX,
+                  int Lx, hipStream_t st)
+{
+    const long n = (long)ntr * ns;
+    if (n <= 0 || (Lt < 0 && Lx < 0)) return;
+    taper_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(x, ntr, ns, coefT, Lt, coefX, Lx);
+}
+
+// Spectrum [batch][nb] (interleaved complex) times a real window of nb bins.
+__global__ void spec_window_kernel(float2 *spec, long total, int nb, const float *win)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const float w = win[e % nb];
+    float2 v = spec[e];
+    v.x *= w;
+    v.y *= w;
+    spec[e] = v;
+}
+
+void launch_spec_window(float2 *spec, long total, int nb, const float *win, hipStream_t st)
+{
+    if (total > 0)
+        spec_window_kernel<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(spec, total, nb, win);
+}
+
+// x[e] = x[e] / d (the reference divides the unnormalised inverse transform).
+__global__ void div_kernel(float *x, long n, float d)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) x[e] = x[e] / d;
+}
+
+void launch_div(float *x, long n, float d, hipStream_t st)
+{
+    if (n > 0) div_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(x, n, d);
+}
+
+// out[c][r] = in[r][c] for an rows x cols matrix, through 32 x 32 LDS tiles.
+__global__ void transpose_kernel(const float *in, float *out, int rows, int cols)
+{
+    __shared__ float t[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+        const int r = r0 + k, c = c0 + threadIdx.x;
+        if (r < rows && c < cols) t[k][threadIdx.x] = in[(long)r * cols + c];
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+        const int c = c0 + k, r = r0 + threadIdx.x;
+        if (r < rows && c < cols) out[(long)c * rows + r] = t[threadIdx.x][k];
+    }
+}
+
+void launch_transpose(const float *in, float *out, int rows, int cols, hipStream_t st)
+{
+    if (rows <= 0 || cols <= 0) return;
+    dim3 grid((cols + 31) / 32, (rows + 31) / 32), block(32, 8);
+    transpose_kernel<<<grid, block, 0, st>>>(in, out, rows, cols);
+}
+
 // ---------------------------------------------------------------------------
 // Gradient finalisation and preconditioning (propagation.c:610-623, 732-743,
 // 953-993; optimization.c:13-97)

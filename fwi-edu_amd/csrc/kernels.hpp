@@ -149,6 +149,13 @@ void launch_histogram(const float *g, long n, float mn, float interval, int *his
 void launch_scale(const float *in, float *out, long n, float s, hipStream_t st);
 size_t reduce_scratch_doubles();
 
+// ---- trace filtering (FFT band filters of the adjoint source) ---------------
+void launch_taper(float *x, int ntr, int ns, const float *coefT, int Lt, const float *coefX,
+                  int Lx, hipStream_t st);
+void launch_spec_window(float2 *spec, long total, int nb, const float *win, hipStream_t st);
+void launch_div(float *x, long n, float d, hipStream_t st);
+void launch_transpose(const float *in, float *out, int rows, int cols, hipStream_t st);
+
 // ---- small helpers -------------------------------------------------------
 void launch_freesurface(float *p, int Nx, int ld, int pml, hipStream_t st);
 void launch_fill_u32(unsigned *p, int n, unsigned v, hipStream_t st);
