@@ -73,7 +73,7 @@ static std::vector<DevBuf *> all_buffers(Ctx &c)
     return {&c.model[0], &c.model[1], &c.setup,   &c.coef,    &c.arena,   &c.acc,
             &c.rec,      &c.obs,      &c.src,     &c.tables,  &c.taper,   &c.shot_out,
             &c.gsum,     &c.scratch,  &c.misc[0], &c.misc[1], &c.misc[2], &c.misc[3],
-            &c.ck16,     &c.ckaux,    &c.fftspec, &c.fftwin,  &c.ffttmp};
+            &c.ck16,     &c.ckaux,    &c.fftspec, &c.fftwin,  &c.ffttmp,  &c.adjfull};
 }
 
 size_t held_bytes(const Ctx &c)
@@ -400,29 +400,35 @@ static long fields_per_shot(const Schedule &sc, bool adjoint, bool ck16)
 
 struct Tables {
     RcvTable rcv;
-    InjTable inj;
+    InjTable inj;       // adjoint sources at the recording rate ([ntr][ns] per shot)
+    InjTable inj_full;  // adjoint sources interpolated to every time step ([ntr][nt] per shot)
 };
 
 // Receiver tables of the batch.  trace offsets index the batch record buffer
 // (shot-major, [ntr][ns] per shot).
-static Tables build_tables(const std::vector<ShotIn> &shots, const Geom &g, std::vector<long> &roff,
-                           long &rec_total)
+static Tables build_tables(const std::vector<ShotIn> &shots, const Geom &g, int nt,
+                           std::vector<long> &roff, long &rec_total, std::vector<long> &foff,
+                           long &full_total)
 {
     Ctx &c = ctx();
     const int S = (int)shots.size();
     roff.assign(S, 0);
+    foff.assign(S, 0);
     rec_total = 0;
+    full_total = 0;
     int nent = 0;
     for (int s = 0; s < S; ++s) {
         roff[s] = rec_total;
+        foff[s] = full_total;
         rec_total += (long)shots[s].pre->ntr * shots[s].pre->ns;
+        full_total += (long)shots[s].pre->ntr * nt;
         nent += shots[s].pre->ntr;
     }
     std::vector<int> e_shot(nent), e_cell(nent), e_sr(nent);
     std::vector<long> e_off(nent);
     // unique (shot, cell) for the injection, receiver order kept within a cell
     std::vector<int> u_shot, u_cell, u_beg;
-    std::vector<long> u_off;
+    std::vector<long> u_off, u_full;
     int k = 0;
     for (int s = 0; s < S; ++s) {
         const Record2D *r = shots[s].pre;
@@ -449,20 +455,24 @@ static Tables build_tables(const std::vector<ShotIn> &shots, const Geom &g, std:
             u_shot.push_back(s);
             u_cell.push_back(cell);
             u_beg.push_back((int)u_off.size());
-            for (int itr : bycell[cell]) u_off.push_back(roff[s] + (long)itr * r->ns);
+            for (int itr : bycell[cell]) {
+                u_off.push_back(roff[s] + (long)itr * r->ns);
+                u_full.push_back(foff[s] + (long)itr * nt);
+            }
         }
     }
     u_beg.push_back((int)u_off.size());
     const int nu = (int)u_cell.size();
     // one device block: ints then longs
     const size_t ni = (size_t)3 * nent + 2 * nu + nu + 1;
-    const size_t nl = (size_t)nent + u_off.size();
+    const size_t nl = (size_t)nent + 2 * u_off.size();
     char *base = c.tables.as<char>(ni * sizeof(int) + nl * sizeof(long) + 64);
     long *lb = reinterpret_cast<long *>(base);
     int *ib = reinterpret_cast<int *>(base + nl * sizeof(long));
     std::vector<long> lh(nl);
     std::copy(e_off.begin(), e_off.end(), lh.begin());
     std::copy(u_off.begin(), u_off.end(), lh.begin() + nent);
+    std::copy(u_full.begin(), u_full.end(), lh.begin() + nent + u_off.size());
     std::vector<int> ih;
     ih.reserve(ni);
     ih.insert(ih.end(), e_shot.begin(), e_shot.end());
@@ -484,6 +494,8 @@ static Tables build_tables(const std::vector<ShotIn> &shots, const Geom &g, std:
     t.inj.shot = ib + 3 * nent;
     t.inj.cell = ib + 3 * nent + nu;
     t.inj.beg = ib + 3 * nent + 2 * nu;
+    t.inj_full = t.inj;
+    t.inj_full.trace_off = lb + nent + u_off.size();
     return t;
 }
 
@@ -624,6 +636,9 @@ struct BatchState {
     std::vector<long> roff;
     long rec_total = 0;
     float *rec = nullptr;      // predicted records, then the adjoint source
+    std::vector<long> foff;    // per-shot offsets of the interpolated adjoint sources
+    long full_total = 0;
+    float *adjfull = nullptr;  // adjoint sources at every time step (sampleRate > 1), or null
     double *acc = nullptr;     // [S][nx*nz]
     long nacc = 0;
     long cells_per_launch = 0;
@@ -707,7 +722,7 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
 
     std::vector<ShotIn> si(S);
     for (int s = 0; s < S; ++s) si[s] = {recs[s], nullptr, s};
-    b.tb = build_tables(si, g, b.roff, b.rec_total);
+    b.tb = build_tables(si, g, nt, b.roff, b.rec_total, b.foff, b.full_total);
     b.rec = c.rec.as<float>(b.rec_total + 64);
     HIPCHECK(hipMemsetAsync(b.rec, 0, b.rec_total * sizeof(float), c.st));
     if (adjoint) {
@@ -854,8 +869,12 @@ static void adjoint_loop(BatchState &b)
                 where[t] = fout;
             }
         }
-        launch_inject(b.tb.inj, const_cast<float *>(cur), b.SS, b.CS, K, b.rec, it, g.dt,
-                      g.invdx, g.invdz, c.st);
+        if (b.adjfull)
+            launch_inject(b.tb.inj_full, const_cast<float *>(cur), b.SS, b.CS, K, b.adjfull, it,
+                          g.dt, g.invdx, g.invdz, c.st);
+        else
+            launch_inject(b.tb.inj, const_cast<float *>(cur), b.SS, b.CS, K, b.rec, it, g.dt,
+                          g.invdx, g.invdz, c.st);
         float *out = pick_free(AW, cur, prev);
         ad.out = out;
         ad.cur = cur;
@@ -906,7 +925,8 @@ static std::vector<long> batch_key(int pass, const BatchState &b, bool f1, bool 
                           (const void *)b.wav_dev, (const void *)b.rec, (const void *)b.acc,
                           (const void *)b.h16, (const void *)b.ckinv, (const void *)b.ckmx,
                           (const void *)b.tb.rcv.trace_off, (const void *)b.tb.rcv.shot,
-                          (const void *)b.tb.inj.trace_off, (const void *)b.tb.inj.shot})
+                          (const void *)b.tb.inj.trace_off, (const void *)b.tb.inj.shot,
+                          (const void *)b.adjfull})
         k.push_back((long)(uintptr_t)p);
     const FdTile &t = b.tile;
     for (long v : {(long)t.kind, (long)t.tz, (long)t.spb, (long)t.txm, (long)t.zfast, (long)t.pd,
@@ -1202,11 +1222,15 @@ void propagate_core(const Call &P)
             if (P.records_to_host)
                 for (int k = 0; k < nb; ++k) rec_d2h(recs[k], b.rec + b.roff[k]);
             if (problem == 2) continue;
-            for (int k = 0; k < nb; ++k)
-                if (recs[k]->sampleRate != 1 || recs[k]->ns != nt)
-                    fail("<error>: gradient with sampleRate %d: trace interpolation (FFT) is not "
-                         "available in this build; record every time step",
-                         recs[k]->sampleRate);
+            bool full = false;
+            for (int k = 0; k < nb; ++k) full |= recs[k]->sampleRate != 1 || recs[k]->ns != nt;
+            if (full) {
+                // propagation.c:635: the adjoint sources at every time step
+                b.adjfull = c.adjfull.as<float>((size_t)b.full_total + 64);
+                for (int k = 0; k < nb; ++k)
+                    interpolate_traces_device(b.rec + b.roff[k], recs[k]->ntr, recs[k]->ns,
+                                              recs[k]->sampleRate, b.adjfull + b.foff[k], nt);
+            }
             adjoint_pass(b);
             launch_finalize(b.acc, b.ASS, b.ACS, gb, nacc, nb, vpw, rhow, (int)nacc, 2.0, 0, c.st);
             if (P.precondition == 1) {
@@ -1457,10 +1481,6 @@ extern "C" void iso_acoustic2d_propagation_engine(Record2D *record2d, Acoustic2d
             }
         }
     } else {
-        if (record2d->sampleRate != 1 || record2d->ns != nt)
-            fail("<error>: adjoint with sampleRate %d needs trace interpolation (FFT), not "
-                 "available in this build",
-                 record2d->sampleRate);
         if (chkpt->nt != nt || !chkpt->p) fail("<error>: checkpoint not allocated for nt=%d", nt);
         for (int it = 0; it < nt; ++it) {
             if (sc.ckslot[it] < 0) continue;
@@ -1469,6 +1489,12 @@ extern "C" void iso_acoustic2d_propagation_engine(Record2D *record2d, Acoustic2d
                                       g.Nx, hipMemcpyHostToDevice, c.st));
         }
         rec_h2d(b.rec, record2d);
+        if (record2d->sampleRate != 1 || record2d->ns != nt) {
+            // propagation.c:635
+            b.adjfull = c.adjfull.as<float>((size_t)b.full_total + 64);
+            interpolate_traces_device(b.rec, record2d->ntr, record2d->ns, record2d->sampleRate,
+                                      b.adjfull, nt);
+        }
         adjoint_pass(b);
         float *gb = c.misc[0].as<float>(nacc);
         launch_finalize(b.acc, b.ASS, b.ACS, gb, nacc, 1, vpd, rhod, (int)nacc, 2.0, 0, c.st);

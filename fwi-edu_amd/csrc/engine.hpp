@@ -94,6 +94,7 @@ struct Ctx {
     DevBuf ck16;                  // FP16 checkpoint slots
     DevBuf ckaux;                 // per-slot inverse scales and max scratch
     DevBuf fftspec, fftwin, ffttmp;   // trace filters
+    DevBuf adjfull;               // adjoint sources interpolated to every time step
     std::map<std::pair<int, int>, std::pair<void *, void *>> fft_plans;  // (n, batch) -> R2C, C2R
 };
 
@@ -154,6 +155,9 @@ void release_buffers(Ctx &c);
 // [batch][n] on the device, filtered in place.
 std::vector<float> filter_window(int kind, int n, float fpass, float fcut, float dt);
 void filter_traces_device(int kind, float *x, int n, int batch, float fpass, float fcut, float dt);
+// interpolate_traces (common.c:482-581): in [ntr][ns] at r times the step ->
+// out [ntr][nt_out], samples (ns-1)*r+1 ... nt_out-1 zero; r <= 1 copies.
+void interpolate_traces_device(const float *in, int ntr, int ns, int r, float *out, int nt_out);
 void release_fft(Ctx &c);
 
 }  // namespace fwiedu

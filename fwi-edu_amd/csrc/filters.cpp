@@ -1,6 +1,7 @@
-// FFT band filters of traces on the device: apply_traces_lowpass /
+// FFT work on traces on the device: the band filters apply_traces_lowpass /
 // apply_traces_bandpass (common.c:584-794) as used by compute_adjoint_source2d
-// with filter 0 / 1 (optimization.c:368-418).
+// with filter 0 / 1 (optimization.c:368-418), and interpolate_traces
+// (common.c:482-581) for adjoint sources recorded at sampleRate > 1.
 //
 // The reference transforms each trace with FFTW single precision (r2c), scales
 // the first n/2+1 bins by a cosine window, transforms back (c2r) and divides by
@@ -80,11 +81,49 @@ void filter_traces_device(int kind, float *x, int n, int batch, float fpass, flo
     launch_div(x, (long)batch * n, (float)n, c.st);
 }
 
+// C2C plans are keyed (-n, batch) beside the (n, batch) R2C / C2R pairs.
+static hipfftHandle c2c_plan(Ctx &c, int n, int batch)
+{
+    auto key = std::make_pair(-n, batch);
+    auto it = c.fft_plans.find(key);
+    if (it != c.fft_plans.end()) return (hipfftHandle)it->second.first;
+    hipfftHandle p = nullptr;
+    FFTCHECK(hipfftPlan1d(&p, n, HIPFFT_C2C, batch));
+    c.fft_plans[key] = std::make_pair((void *)p, (void *)nullptr);
+    return p;
+}
+
+void interpolate_traces_device(const float *in, int ntr, int ns, int r, float *out, int nt_out)
+{
+    Ctx &c = ctx();
+    if (ntr <= 0 || ns <= 0) fail("<error>: invalid input (ntr=%d, nt=%d) in function interpolate_traces!", ntr, ns);
+    if (r <= 1) {
+        // common.c:511-516 copies (the caller has ns == nt_out)
+        if (ns > nt_out) fail("<error>: interpolate_traces: %d samples into %d", ns, nt_out);
+        HIPCHECK(hipMemcpy2DAsync(out, (size_t)nt_out * 4, in, (size_t)ns * 4, (size_t)ns * 4, ntr,
+                                  hipMemcpyDeviceToDevice, c.st));
+        if (ns < nt_out)
+            HIPCHECK(hipMemset2DAsync(out + ns, (size_t)nt_out * 4, 0, (size_t)(nt_out - ns) * 4,
+                                      ntr, c.st));
+        return;
+    }
+    const int M = (ns - 1) * r + 1;
+    if (M > nt_out) fail("<error>: interpolate_traces: %d samples into %d", M, nt_out);
+    float2 *buf = c.fftspec.as<float2>((size_t)ntr * M + 64);
+    launch_zero_stuff(in, ntr, ns, r, buf, M, c.st);
+    hipfftHandle p = c2c_plan(c, M, ntr);
+    FFTCHECK(hipfftSetStream(p, c.st));
+    FFTCHECK(hipfftExecC2C(p, (hipfftComplex *)buf, (hipfftComplex *)buf, HIPFFT_FORWARD));
+    launch_interp_window(buf, ntr, M, r, c.st);
+    FFTCHECK(hipfftExecC2C(p, (hipfftComplex *)buf, (hipfftComplex *)buf, HIPFFT_BACKWARD));
+    launch_interp_out(buf, ntr, M, out, nt_out, c.st);
+}
+
 void release_fft(Ctx &c)
 {
     for (auto &e : c.fft_plans) {
-        (void)hipfftDestroy((hipfftHandle)e.second.first);
-        (void)hipfftDestroy((hipfftHandle)e.second.second);
+        if (e.second.first) (void)hipfftDestroy((hipfftHandle)e.second.first);
+        if (e.second.second) (void)hipfftDestroy((hipfftHandle)e.second.second);
     }
     c.fft_plans.clear();
 }

@@ -1158,6 +1158,59 @@ void launch_transpose(const float *in, float *out, int rows, int cols, hipStream
     transpose_kernel<<<grid, block, 0, st>>>(in, out, rows, cols);
 }
 
+// interpolate_traces (common.c:482-581) pieces: zero-stuffing into a complex
+// buffer [ntr][M], the rectangular window (gain r on bins k < M/(2r) and
+// k > M - M/(2r)), and the real part divided by M into [ntr][nt_out] with a
+// zero tail.
+__global__ void zero_stuff_kernel(const float *in, int ntr, int ns, int r, float2 *buf, int M)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)ntr * M) return;
+    const int itr = (int)(e / M), j = (int)(e - (long)itr * M);
+    float2 v = make_float2(0.0f, 0.0f);
+    if (j % r == 0) v.x = in[(long)itr * ns + j / r];
+    buf[e] = v;
+}
+
+void launch_zero_stuff(const float *in, int ntr, int ns, int r, float2 *buf, int M, hipStream_t st)
+{
+    const long n = (long)ntr * M;
+    zero_stuff_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(in, ntr, ns, r, buf, M);
+}
+
+__global__ void interp_window_kernel(float2 *buf, int ntr, int M, int r)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)ntr * M) return;
+    const int k = (int)(e % M);
+    const int keep = M / (2 * r);
+    const float w = (k < keep || k >= M - keep + 1) ? (float)r : 0.0f;
+    float2 v = buf[e];
+    v.x *= w;
+    v.y *= w;
+    buf[e] = v;
+}
+
+void launch_interp_window(float2 *buf, int ntr, int M, int r, hipStream_t st)
+{
+    const long n = (long)ntr * M;
+    interp_window_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(buf, ntr, M, r);
+}
+
+__global__ void interp_out_kernel(const float2 *buf, int ntr, int M, float *out, int nt_out)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)ntr * nt_out) return;
+    const int itr = (int)(e / nt_out), j = (int)(e - (long)itr * nt_out);
+    out[e] = j < M ? buf[(long)itr * M + j].x / (float)M : 0.0f;
+}
+
+void launch_interp_out(const float2 *buf, int ntr, int M, float *out, int nt_out, hipStream_t st)
+{
+    const long n = (long)ntr * nt_out;
+    interp_out_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(buf, ntr, M, out, nt_out);
+}
+
 // ---------------------------------------------------------------------------
 // Gradient finalisation and preconditioning (propagation.c:610-623, 732-743,
 // 953-993; optimization.c:13-97)

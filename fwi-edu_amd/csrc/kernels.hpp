@@ -155,6 +155,9 @@ void launch_taper(float *x, int ntr, int ns, const float *coefT, int Lt, const f
 void launch_spec_window(float2 *spec, long total, int nb, const float *win, hipStream_t st);
 void launch_div(float *x, long n, float d, hipStream_t st);
 void launch_transpose(const float *in, float *out, int rows, int cols, hipStream_t st);
+void launch_zero_stuff(const float *in, int ntr, int ns, int r, float2 *buf, int M, hipStream_t st);
+void launch_interp_window(float2 *buf, int ntr, int M, int r, hipStream_t st);
+void launch_interp_out(const float2 *buf, int ntr, int M, float *out, int nt_out, hipStream_t st);
 
 // ---- small helpers -------------------------------------------------------
 void launch_freesurface(float *p, int Nx, int ld, int pml, hipStream_t st);
