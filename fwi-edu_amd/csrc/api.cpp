@@ -62,13 +62,7 @@ extern "C" float calculate_objective_value(Record2D *record2d, int Nshot)
     Ctx &c = ctx();
     long n;
     float *d = upload_records(c.rec, record2d, Nshot, n);
-    double *scr = scratch_res();
-    double *res = scr + reduce_scratch_doubles();
-    launch_sum_sq(d, n, scr, res, c.st);
-    double h[2];
-    d2h(h, res, 2 * sizeof(double));
-    sync();
-    return (float)(h[0] * 0.5);
+    return objective_device(d, n);
 }
 
 // calculate_step_length, optimization.c:134-175
@@ -80,17 +74,7 @@ extern "C" float calculate_step_length(Record2D *record2d_pre0, Record2D *record
     float *a = upload_records(c.rec, record2d_pre0, Nshot, n0);
     float *b = upload_records(c.obs, record2d_pre1, Nshot, n1);
     if (n0 != n1) fail("<error>: calculate_step_length: record sizes differ");
-    double *scr = scratch_res();
-    double *res = scr + reduce_scratch_doubles();
-    launch_sum_step(a, b, n0, scr, res, c.st);
-    double h[2];
-    d2h(h, res, 2 * sizeof(double));
-    sync();
-    if (h[1] == 0.0) {
-        fprintf(stderr, "<warning>: denominator is zero in step length calculation, returning 0.\n");
-        return 0.0f;
-    }
-    return (float)(h[0] / h[1]);
+    return step_length_device(a, b, n0);
 }
 
 // compute_model_perturbation, optimization.c:178-298: min/max, histogram and
@@ -104,53 +88,8 @@ extern "C" float compute_model_perturbation(float fraction, float *slowness, flo
     float *sl = c.misc[1].as<float>((size_t)len + 64);
     h2d(g, gradient, (size_t)len * sizeof(float));
     h2d(sl, slowness, (size_t)len * sizeof(float));
-    float *fs = c.misc[2].as<float>(2 * 512 + 64);
-    launch_minmax(g, len, fs, fs + 2 * 512, c.st);
-    float mm[2];
-    d2h(mm, fs + 2 * 512, 2 * sizeof(float));
-    sync();
-    const float Min = mm[0], Max = mm[1];
-    const float interval = (float)((Max - Min) / 1000.0);
-    int *hist = c.misc[3].as<int>(1001 + 64);
-    HIPCHECK(hipMemsetAsync(hist, 0, 1001 * sizeof(int), c.st));
-    launch_histogram(g, len, Min, interval, hist, c.st);
-    int h[1001];
-    d2h(h, hist, sizeof(h));
-    double *scr = scratch_res();
-    double *res = scr + reduce_scratch_doubles();
-    launch_sum(sl, len, scr, res, c.st);
-    double avg[2];
-    d2h(avg, res, 2 * sizeof(double));
-    sync();
-    int peak = 0, pos = 0;
-    for (int i = 0; i < 1001; ++i)
-        if (h[i] > peak) {
-            peak = h[i];
-            pos = i;
-        }
-    int hi = pos, lo = pos;
-    const float cap = 0.99f;
-    float acc = (float)h[hi] / (float)len;
-    while (acc < cap) {
-        bool moved = false;
-        if (hi < 1000) {
-            hi++;
-            acc += (float)h[hi] / (float)len;
-            moved = true;
-        }
-        if (lo > 0) {
-            lo--;
-            acc += (float)h[lo] / (float)len;
-            moved = true;
-        }
-        if (!moved) break;
-    }
-    float a = Min + (float)lo * interval, b = Min + (float)hi * interval;
-    a = a < 0 ? -a : a;
-    b = b < 0 ? -b : b;
-    float scalar = (a > b) ? a : b;
-    scalar = (float)((avg[0] / len * (double)fraction) / (double)scalar);
-    launch_scale(g, sl, len, scalar, c.st);
+    // the slowness sum is taken before the result overwrites sl (stream order)
+    const float scalar = model_perturbation_device(fraction, sl, g, sl, len);
     d2h(out, sl, (size_t)len * sizeof(float));
     sync();
     return scalar;

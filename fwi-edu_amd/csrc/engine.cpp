@@ -1221,6 +1221,11 @@ void propagate_core(const Call &P)
             }
             if (P.records_to_host)
                 for (int k = 0; k < nb; ++k) rec_d2h(recs[k], b.rec + b.roff[k]);
+            if (P.pre_dev)
+                for (int k = 0; k < nb; ++k)
+                    HIPCHECK(hipMemcpyAsync(P.pre_dev + P.obs_off[grp[b0 + k]], b.rec + b.roff[k],
+                                            (size_t)recs[k]->ntr * recs[k]->ns * sizeof(float),
+                                            hipMemcpyDeviceToDevice, c.st));
             if (problem == 2) continue;
             bool full = false;
             for (int k = 0; k < nb; ++k) full |= recs[k]->sampleRate != 1 || recs[k]->ns != nt;
@@ -1333,7 +1338,9 @@ struct Session {
     std::vector<Record2D> obs, pre;        // headers; traces stay on the device
     DevBuf obs_dev;
     std::vector<long> off;
+    long rec_total = 0;
     DevBuf grad_dev;
+    DevBuf pre0, pre1, slow, pert;         // FWI iteration state
 };
 
 extern "C" void *fwiedu_session_create(float **vp, float **rho, Record2D *record2d_obs, int Nshot,
@@ -1362,6 +1369,7 @@ extern "C" void *fwiedu_session_create(float **vp, float **rho, Record2D *record
         S->off[s] = tot;
         tot += (long)record2d_obs[s].ntr * record2d_obs[s].ns;
     }
+    S->rec_total = tot;
     float *d = S->obs_dev.as<float>(tot + 64);
     for (int s = 0; s < Nshot; ++s)
         h2d(d + S->off[s], record2d_obs[s].traces_p[0],
@@ -1418,10 +1426,205 @@ extern "C" void fwiedu_session_destroy(void *sp)
     Session *S = static_cast<Session *>(sp);
     if (!S) return;
     sync();
-    S->obs_dev.release();
-    S->grad_dev.release();
+    for (DevBuf *d : {&S->obs_dev, &S->grad_dev, &S->pre0, &S->pre1, &S->slow, &S->pert})
+        d->release();
     delete S;
 }
+
+// One iteration of the FWI driver (acoustic2d_fwi.c:409-465) with every array
+// on the device: gradient (problem 1), smoothing along z then x, objective,
+// perturbation from the smoothed gradient (before its water rows are zeroed,
+// T15), trial model, line-search trial (problem 2), objective, step length,
+// update of slowness, velocity and (Gardner) density, water rows and clipping.
+extern "C" void fwiedu_session_iterate(void *sp, FwieduIteration *it, float **vp_out,
+                                       float **rho_out, float **grad_out)
+{
+    Session *S = static_cast<Session *>(sp);
+    if (!S || !it) fail("<error>: fwiedu_session_iterate: null argument");
+    if (it->rz < 1 || it->rx < 1) fail("<error>: fwiedu_session_iterate: smoothing lengths >= 1");
+    Ctx &c = ctx();
+    const int nx = S->P.nx, nz = S->P.nz;
+    const long n = (long)nx * nz;
+    float *pre0 = S->pre0.as<float>(S->rec_total + 64);
+    float *pre1 = S->pre1.as<float>(S->rec_total + 64);
+    float *slow = S->slow.as<float>(n + 64);
+    float *pert = S->pert.as<float>(n + 64);
+    float *grad = S->P.grad_dev;
+    // slowness of the current model (acoustic2d_fwi.c:365, 463)
+    float *vpd = c.model[0].as<float>(n), *rhod = c.model[1].as<float>(n);
+    h2d(vpd, S->vp.data(), n * sizeof(float));
+    h2d(rhod, S->rho.data(), n * sizeof(float));
+    launch_model_op(MOP_RECIP, vpd, nullptr, slow, n, nz, 0, 0, 0, 0, c.st);
+
+    // GRADIENT (models already on the device)
+    Call P = S->P;
+    P.freq = it->freq;
+    P.filter = (int)it->filter;
+    P.pre_dev = pre0;
+    P.models_on_device = true;
+    double f = 0.0;
+    P.misfit = &f;
+    propagate_core(P);
+    smooth2d_device(grad, pert, nx, nz, it->rz, 1);
+    smooth2d_device(pert, grad, nx, nz, it->rx, 0);
+
+    // STEP-LENGTH
+    it->fun1 = objective_device(pre0, S->rec_total);
+    it->scalar = model_perturbation_device(it->fraction, slow, grad, pert, (int)n);
+    launch_model_op(MOP_WATER, nullptr, nullptr, grad, n, nz, 0.0f, 0, 0, it->waterdepth, c.st);
+    auto set_model = [&](const float *sl) {
+        // slow2v, water, clip; density from Gardner, water, clip
+        launch_model_op(MOP_RECIP, sl, nullptr, vpd, n, nz, 0, 0, 0, 0, c.st);
+        launch_model_op(MOP_WATER, nullptr, nullptr, vpd, n, nz, it->vp_water, 0, 0,
+                        it->waterdepth, c.st);
+        launch_model_op(MOP_CLIP, vpd, nullptr, vpd, n, nz, 0, it->vpmin, it->vpmax, 0, c.st);
+        if (it->gardner) {
+            launch_model_op(MOP_GARDNER, vpd, nullptr, rhod, n, nz, 0, 0, 0, 0, c.st);
+            launch_model_op(MOP_WATER, nullptr, nullptr, rhod, n, nz, it->rho_water, 0, 0,
+                            it->waterdepth, c.st);
+            launch_model_op(MOP_CLIP, rhod, nullptr, rhod, n, nz, 0, it->rhomin, it->rhomax, 0,
+                            c.st);
+        }
+    };
+    float *trial = c.misc[1].as<float>(n + 64);
+    launch_model_op(MOP_ADD, slow, pert, trial, n, nz, 0, 0, 0, 0, c.st);
+    set_model(trial);
+    // the adjoint-source taper reads vp on the host: keep the host copy current
+    d2h(S->vp.data(), vpd, n * sizeof(float));
+    d2h(S->rho.data(), rhod, n * sizeof(float));
+    sync();
+
+    // line-search trial (problem 2)
+    Call P2 = P;
+    P2.problem = 2;
+    P2.pre_dev = pre1;
+    P2.grad_dev = nullptr;
+    double f2 = 0.0;
+    P2.misfit = &f2;
+    propagate_core(P2);
+    it->fun2 = objective_device(pre1, S->rec_total);
+    it->steplen = step_length_device(pre0, pre1, S->rec_total);
+
+    // UPDATE Model
+    launch_model_op(MOP_SCALE, pert, nullptr, pert, n, nz, it->steplen, 0, 0, 0, c.st);
+    launch_model_op(MOP_ADD, slow, pert, slow, n, nz, 0, 0, 0, 0, c.st);
+    set_model(slow);
+    d2h(S->vp.data(), vpd, n * sizeof(float));
+    d2h(S->rho.data(), rhod, n * sizeof(float));
+    if (vp_out && vp_out[0]) d2h(vp_out[0], vpd, n * sizeof(float));
+    if (rho_out && rho_out[0]) d2h(rho_out[0], rhod, n * sizeof(float));
+    if (grad_out && grad_out[0]) d2h(grad_out[0], grad, n * sizeof(float));
+    sync();
+}
+
+// ---------------------------------------------------------------------------
+// Optimisation pieces on device arrays, shared by the C entry points of
+// optimization.c and the device-resident FWI iteration.
+namespace fwiedu {
+
+// calculate_objective_value (optimization.c:99-131) of n samples.
+float objective_device(const float *d, long n)
+{
+    Ctx &c = ctx();
+    double *scr = c.scratch.as<double>(reduce_scratch_doubles() + 64);
+    double *res = scr + reduce_scratch_doubles();
+    launch_sum_sq(d, n, scr, res, c.st);
+    double h[2];
+    d2h(h, res, 2 * sizeof(double));
+    sync();
+    return (float)(h[0] * 0.5);
+}
+
+// calculate_step_length (optimization.c:134-175).
+float step_length_device(const float *a, const float *b, long n)
+{
+    Ctx &c = ctx();
+    double *scr = c.scratch.as<double>(reduce_scratch_doubles() + 64);
+    double *res = scr + reduce_scratch_doubles();
+    launch_sum_step(a, b, n, scr, res, c.st);
+    double h[2];
+    d2h(h, res, 2 * sizeof(double));
+    sync();
+    if (h[1] == 0.0) {
+        fprintf(stderr, "<warning>: denominator is zero in step length calculation, returning 0.\n");
+        return 0.0f;
+    }
+    return (float)(h[0] / h[1]);
+}
+
+// compute_model_perturbation (optimization.c:178-298): min/max, histogram and
+// the slowness sum on the device, the 1001-bin growth on the host; out =
+// -scalar * g.  g and out may not alias.
+float model_perturbation_device(float fraction, const float *sl, const float *g, float *out,
+                                int len)
+{
+    Ctx &c = ctx();
+    float *fs = c.misc[2].as<float>(2 * 512 + 64 + 1001 + 64);
+    int *hist = reinterpret_cast<int *>(fs + 2 * 512 + 64);
+    launch_minmax(g, len, fs, fs + 2 * 512, c.st);
+    float mm[2];
+    d2h(mm, fs + 2 * 512, 2 * sizeof(float));
+    sync();
+    const float Min = mm[0], Max = mm[1];
+    const float interval = (float)((Max - Min) / 1000.0);
+    launch_fill_u32(reinterpret_cast<unsigned *>(hist), 1001, 0u, c.st);
+    launch_histogram(g, len, Min, interval, hist, c.st);
+    int h[1001];
+    d2h(h, hist, sizeof(h));
+    double *scr = c.scratch.as<double>(reduce_scratch_doubles() + 64);
+    double *res = scr + reduce_scratch_doubles();
+    launch_sum(sl, len, scr, res, c.st);
+    double avg[2];
+    d2h(avg, res, 2 * sizeof(double));
+    sync();
+    int peak = 0, pos = 0;
+    for (int i = 0; i < 1001; ++i)
+        if (h[i] > peak) {
+            peak = h[i];
+            pos = i;
+        }
+    int hi = pos, lo = pos;
+    const float cap = 0.99f;
+    float acc = (float)h[hi] / (float)len;
+    while (acc < cap) {
+        bool moved = false;
+        if (hi < 1000) {
+            hi++;
+            acc += (float)h[hi] / (float)len;
+            moved = true;
+        }
+        if (lo > 0) {
+            lo--;
+            acc += (float)h[lo] / (float)len;
+            moved = true;
+        }
+        if (!moved) break;
+    }
+    float a = Min + (float)lo * interval, b = Min + (float)hi * interval;
+    a = a < 0 ? -a : a;
+    b = b < 0 ? -b : b;
+    float scalar = (a > b) ? a : b;
+    scalar = (float)((avg[0] / len * (double)fraction) / (double)scalar);
+    launch_scale(g, out, len, scalar, c.st);
+    return scalar;
+}
+
+// smooth2D (common.c:927-1040) of a device [nx][nz] array.
+void smooth2d_device(const float *in, float *out, int nx, int nz, int N, int zdir)
+{
+    Ctx &c = ctx();
+    std::vector<float> w(2 * N + 1);
+    for (int i = 0; i < 2 * N + 1; ++i) {
+        w[i] = (float)cos(3.141592653589793 * (i - N) / (2.0 * N));
+        w[i] *= w[i];
+    }
+    float *wd = c.taper.as<float>(w.size() + 64);
+    h2d(wd, w.data(), w.size() * sizeof(float));
+    launch_smooth2d(in, out, nx, nz, N, zdir, wd, c.st);
+    sync();  // the weight table is reused by the next call
+}
+
+}  // namespace fwiedu
 
 // Per-shot engine (propagation.c:356-775) with the reference's host checkpoint.
 extern "C" void iso_acoustic2d_propagation_engine(Record2D *record2d, Acoustic2dCheckPoint *chkpt,

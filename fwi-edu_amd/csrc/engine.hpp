@@ -142,10 +142,16 @@ struct Call {
     bool records_to_host = true;
     float *grad_dev = nullptr;        // device gradient output (session)
     double *misfit = nullptr;         // 0.5 * sum of squared adjoint sources (problem 1/2)
+    float *pre_dev = nullptr;         // device copy of every adjoint source, at obs_off
     bool models_on_device = false;    // c.model[] already hold vp, rho
 };
 
 void propagate_core(const Call &P);
+float objective_device(const float *d, long n);
+float step_length_device(const float *a, const float *b, long n);
+float model_perturbation_device(float fraction, const float *sl, const float *g, float *out,
+                                int len);
+void smooth2d_device(const float *in, float *out, int nx, int nz, int N, int zdir);
 void call_footprint(const Call &P, bool ck16, size_t *fixed, size_t *per_shot);
 void check_common(int nz, int nx, float dx, float dz, float dt, int nt, int pml, int fs);
 size_t held_bytes(const Ctx &c);

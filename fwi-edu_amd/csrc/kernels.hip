@@ -10,6 +10,7 @@
 
 #include "kernels.hpp"
 
+#include <float.h>
 #include <math.h>
 
 #pragma clang fp contract(off)
@@ -1508,6 +1509,77 @@ __global__ void freesurface_kernel(float *p, int Nx, int ld, int pml)
 void launch_freesurface(float *p, int Nx, int ld, int pml, hipStream_t st)
 {
     freesurface_kernel<<<(Nx + 63) / 64, 64, 0, st>>>(p, Nx, ld, pml);
+}
+
+// ---------------------------------------------------------------------------
+// Model update of the FWI driver on the device (acoustic2d_fwi.c:418-465).
+// Models are [nx][nz] (z fastest).
+
+// smooth2D (common.c:927-1040): cos^2 window of half-length N along z (zdir 1)
+// or x; float products accumulated in double in window order, divided by
+// (sum of weights + FLT_MIN).  w: the 2N+1 float weights.
+__global__ void smooth2d_kernel(const float *in, float *out, int nx, int nz, int N, int zdir,
+                                const float *w)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)nx * nz) return;
+    const int ix = (int)(e / nz), iz = (int)(e - (long)ix * nz);
+    const int j = zdir ? iz : ix, n = zdir ? nz : nx;
+    const int ws = (j >= N) ? 0 : N - j;
+    const int ds = (j >= N) ? j - N : 0;
+    const int we = (n - 1 - j < N) ? 2 * N - (N - (n - 1 - j)) : 2 * N;
+    double a1 = 0.0, a2 = 0.0;
+    for (int i = ws, i2 = ds; i <= we; ++i, ++i2) {
+        const float v = zdir ? in[(long)ix * nz + i2] : in[(long)i2 * nz + iz];
+        a1 += (double)(w[i] * v);
+        a2 += (double)w[i];
+    }
+    out[e] = (float)(a1 / (a2 + (double)FLT_MIN));
+}
+
+void launch_smooth2d(const float *in, float *out, int nx, int nz, int N, int zdir, const float *w,
+                     hipStream_t st)
+{
+    const long n = (long)nx * nz;
+    smooth2d_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(in, out, nx, nz, N, zdir, w);
+}
+
+// op: 0 out = a + b (vadd), 1 out = s * a (nvmul), 2 out = a != 0 ? 1 / a : 0
+// (v2slow, slow2v), 3 clip a to [lo, hi] (clip_model), 4 out = 310 a^0.25
+// (vp2density, common.c:298-330: 310.0f * powf(vp, 0.25f)), 5 rows iz < wd of
+// every column set to s (set_waterlayer_value)
+__global__ void model_op_kernel(int op, const float *a, const float *b, float *out, long n, int nz,
+                                float s, float lo, float hi, int wd)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    switch (op) {
+    case 0: out[e] = a[e] + b[e]; break;
+    case 1: out[e] = s * a[e]; break;
+    case 2: out[e] = (a[e] != 0.0f) ? 1.0f / a[e] : 0.0f; break;
+    case 3: {
+        const float v = a[e];
+        out[e] = v > hi ? hi : (v < lo ? lo : v);
+        break;
+    }
+    case 4: {
+        // powf(v, 0.25f) correctly rounded: the double power rounded once to float
+        const float q = (float)pow((double)a[e], 0.25);
+        out[e] = 310.0f * q;
+        break;
+    }
+    case 5:
+        if ((int)(e % nz) < wd) out[e] = s;
+        break;
+    }
+}
+
+void launch_model_op(int op, const float *a, const float *b, float *out, long n, int nz, float s,
+                     float lo, float hi, int wd, hipStream_t st)
+{
+    if (n > 0)
+        model_op_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(op, a, b, out, n, nz, s, lo,
+                                                                     hi, wd);
 }
 
 // ---------------------------------------------------------------------------

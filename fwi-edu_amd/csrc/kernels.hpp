@@ -159,6 +159,13 @@ void launch_zero_stuff(const float *in, int ntr, int ns, int r, float2 *buf, int
 void launch_interp_window(float2 *buf, int ntr, int M, int r, hipStream_t st);
 void launch_interp_out(const float2 *buf, int ntr, int M, float *out, int nt_out, hipStream_t st);
 
+// ---- model update of the FWI iteration -------------------------------------
+enum { MOP_ADD = 0, MOP_SCALE, MOP_RECIP, MOP_CLIP, MOP_GARDNER, MOP_WATER };
+void launch_smooth2d(const float *in, float *out, int nx, int nz, int N, int zdir, const float *w,
+                     hipStream_t st);
+void launch_model_op(int op, const float *a, const float *b, float *out, long n, int nz, float s,
+                     float lo, float hi, int wd, hipStream_t st);
+
 // ---- small helpers -------------------------------------------------------
 void launch_freesurface(float *p, int Nx, int ld, int pml, hipStream_t st);
 void launch_fill_u32(unsigned *p, int n, unsigned v, hipStream_t st);

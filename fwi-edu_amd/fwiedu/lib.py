@@ -30,6 +30,15 @@ class Acoustic2dCheckPoint(ctypes.Structure):
     ]
 
 
+class FwieduIteration(ctypes.Structure):
+    """FwieduIteration of include/fwiedu.h (one FWI driver iteration)."""
+    _fields_ = [("freq", c_float), ("filter", c_float), ("rz", c_int), ("rx", c_int),
+                ("waterdepth", c_int), ("vp_water", c_float), ("rho_water", c_float),
+                ("vpmin", c_float), ("vpmax", c_float), ("rhomin", c_float), ("rhomax", c_float),
+                ("gardner", c_int), ("fraction", c_float),
+                ("fun1", c_float), ("fun2", c_float), ("steplen", c_float), ("scalar", c_float)]
+
+
 def _sig(lib, name, res, args):
     f = getattr(lib, name)
     f.restype = res
@@ -80,6 +89,8 @@ def load():
     _sig(L, "fwiedu_session_set_model", None, [ctypes.c_void_p, c_float_pp, c_float_pp])
     _sig(L, "fwiedu_session_gradient", c_float, [ctypes.c_void_p, c_float_pp])
     _sig(L, "fwiedu_session_destroy", None, [ctypes.c_void_p])
+    _sig(L, "fwiedu_session_iterate", None,
+         [ctypes.c_void_p, POINTER(FwieduIteration), c_float_pp, c_float_pp, c_float_pp])
     _sig(L, "fwiedu_set_batch", None, [c_int])
     _sig(L, "fwiedu_set_tile", None, [c_int, c_int])
     _sig(L, "fwiedu_set_xcd_remap", None, [c_int])
@@ -111,7 +122,7 @@ EXPORTED = [
     "apply_gradient_precondition", "apply_gradient_shot_precondition",
     "calculate_objective_value", "compute_model_perturbation", "calculate_step_length",
     "fwiedu_session_create", "fwiedu_session_set_model", "fwiedu_session_gradient",
-    "fwiedu_session_destroy", "fwiedu_set_batch", "fwiedu_set_tile", "fwiedu_set_march", "fwiedu_set_tile_order", "fwiedu_set_xcd_remap", "fwiedu_set_lanes", "fwiedu_set_layout", "fwiedu_set_graphs", "fwiedu_set_memory_fraction", "fwiedu_set_kernel_timing",
+    "fwiedu_session_destroy", "fwiedu_session_iterate", "fwiedu_set_batch", "fwiedu_set_tile", "fwiedu_set_march", "fwiedu_set_tile_order", "fwiedu_set_xcd_remap", "fwiedu_set_lanes", "fwiedu_set_layout", "fwiedu_set_graphs", "fwiedu_set_memory_fraction", "fwiedu_set_kernel_timing",
     "fwiedu_set_checkpoint_bits", "fwiedu_stat", "fwiedu_reset_stats", "fwiedu_release",
     "fwiedu_device_ok", "fwiedu_version",
     "fwiedu_server_create", "fwiedu_server_submit", "fwiedu_server_done", "fwiedu_server_wait",

@@ -207,6 +207,27 @@ void fwiedu_session_set_model(void *session, float **vp, float **rho);
  * the processed residual).  grad may be NULL (the gradient stays on the device). */
 float fwiedu_session_gradient(void *session, float **grad);
 void fwiedu_session_destroy(void *session);
+/* One iteration of the FWI driver (acoustic2d_fwi.c:409-465) with the model,
+ * slowness, gradient, perturbation and both sets of adjoint sources on the
+ * device: gradient at (freq, filter), smooth2D along z (rz) then x (rx),
+ * objective, compute_model_perturbation(fraction) of the smoothed gradient,
+ * water rows of the gradient zeroed, trial model (slow2v, water, clip, Gardner
+ * density when gardner = 1), line-search trial, objective, step length, model
+ * update.  The session's model becomes the updated one; fun1, fun2, steplen and
+ * scalar are written back; vp, rho, grad receive the new model and the smoothed
+ * gradient when not NULL. */
+typedef struct FwieduIteration {
+    float freq, filter;
+    int rz, rx;
+    int waterdepth;
+    float vp_water, rho_water;
+    float vpmin, vpmax, rhomin, rhomax;
+    int gardner;
+    float fraction;
+    float fun1, fun2, steplen, scalar;
+} FwieduIteration;
+void fwiedu_session_iterate(void *session, FwieduIteration *it, float **vp, float **rho,
+                            float **grad);
 
 /* ---- Serving loop (no counterpart in the reference) ------------------------
  * A job is one iso_acoustic2d_propagation call of problem 0 (modeling: records
