@@ -102,6 +102,13 @@ def load():
     _sig(L, "fwiedu_set_memory_fraction", None, [c_double])
     _sig(L, "fwiedu_set_kernel_timing", None, [c_int])
     _sig(L, "fwiedu_set_checkpoint_bits", None, [c_int])
+    _sig(L, "calculate_peak_frequency", c_float, [c_float_p, c_int, c_float])
+    _sig(L, "create_grid_observe2d", None, [R, c_int, c_float, c_float, c_int])
+    _sig(L, "write_acoustic2d_traces", None, [c_char_p, R, c_int])
+    _sig(L, "get_shots_number", c_int, [c_char_p])
+    _sig(L, "read_gather2d_from_sufile", None, [R, c_char_p, c_int, c_int])
+    _sig(L, "fwiedu_observe2d_from_json", None,
+         [R, c_char_p, c_int, c_float_p, c_int, c_float_p, c_int, c_float, c_int, c_float_p, c_int])
     _sig(L, "fwiedu_stat", c_double, [c_char_p])
     _sig(L, "fwiedu_reset_stats", None, [])
     _sig(L, "fwiedu_release", None, [])
@@ -127,6 +134,8 @@ EXPORTED = [
     "fwiedu_device_ok", "fwiedu_version",
     "fwiedu_server_create", "fwiedu_server_submit", "fwiedu_server_done", "fwiedu_server_wait",
     "fwiedu_server_stat", "fwiedu_server_destroy",
+    "calculate_peak_frequency", "create_grid_observe2d", "write_acoustic2d_traces",
+    "get_shots_number", "read_gather2d_from_sufile", "fwiedu_observe2d_from_json",
 ]
 
 

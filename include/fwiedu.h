@@ -229,6 +229,27 @@ typedef struct FwieduIteration {
 void fwiedu_session_iterate(void *session, FwieduIteration *it, float **vp, float **rho,
                             float **grad);
 
+/* ---- Data formats of the drivers (host only) -------------------------------- */
+/* calculate_peak_frequency (common.h:230, common.c:363-413): frequency of the
+ * first maximum of |rfft(s)|. */
+float calculate_peak_frequency(const float *s, int nt, float dt);
+/* create_grid_observe2d (observation.h:152, observation.c:338-420): grid
+ * indices, window [min igx - 30, max igx + 30] and window-local isx / igx. */
+void create_grid_observe2d(Record2D *record2d, int Nshot, float dx, float dz, int nx);
+/* write_acoustic2d_traces (dataio.h, dataio.c:306-372): all shots to the SU
+ * file "<prefix>_<Nshot>_<ns>_<traces>_<dt ms %05.2f>ms.su". */
+void write_acoustic2d_traces(char *prefix, Record2D *record2d, int Nshot);
+/* get_shots_number (dataio.c:400-443): gathers in an SU file. */
+int get_shots_number(const char *filename);
+/* read_gather2d_from_sufile (dataio.c:445-503): geometry and traces of Nshot
+ * gathers (gx, gz, traces_p allocated with the SU conventions). */
+void read_gather2d_from_sufile(Record2D *record2d, const char *recordfile, int Nshot, int nt);
+/* init_observe2d_from_JSON (observation.c:422-532) on the "record" array of a
+ * JSON acquisition file (the reference's takes the array parsed by cJSON). */
+void fwiedu_observe2d_from_json(Record2D *record2d, const char *jsonfile, int Nshot, float *x,
+                                int nx, float *z, int nz, float dt, int nt, float *src,
+                                int sampleRate);
+
 /* ---- Serving loop (no counterpart in the reference) ------------------------
  * A job is one iso_acoustic2d_propagation call of problem 0 (modeling: records
  * into pre) or 1 (gradient: adjoint sources into pre, gradient into grad),
