@@ -470,3 +470,12 @@ extern "C" void fwiedu_observe2d_from_json(Record2D *record2d, const char *jsonf
         r.fpeak = fpeak;
     }
 }
+
+// Shots in the "record" array of a JSON acquisition file.
+extern "C" int fwiedu_json_shot_count(const char *jsonfile)
+{
+    const Json root = parse_json_file(jsonfile);
+    const Json *recs = root.get("record");
+    if (!recs || recs->kind != Json::ARR) fail("<error>: JSON file %s has no \"record\" array", jsonfile);
+    return (int)recs->items.size();
+}

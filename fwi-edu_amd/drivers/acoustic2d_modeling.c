@@ -1,0 +1,74 @@
+/* acoustic2d_modeling: forward modeling of all shots of a JSON acquisition on
+ * the GPU through libfwiedu's iso_acoustic2d_propagation (problem 0), the
+ * flow of the reference driver (main/acoustic2d_modeling.c:110-388): models
+ * and wavelet from float32 files, Gardner density or a density file, water
+ * rows, JSON geometry, grid geometry, modeling, the SU gather
+ * "shots_p_<Nshot>_<ns>_<traces>_<dt>ms.su".
+ *
+ * Parameters (key=value): nx nz nt dx dz dt srcfile vpfile jsonfile
+ * [rhofile] [Gardner=1] [freesurface=0] [pmlThick=30] [waterdepth=0]
+ * [vp_water=1500] [rho_water=1000] [sampleRate=1] [prefix=shots_p].
+ * Geometry from parameters (fsx, ds, ...) is not in this build: use jsonfile.
+ */
+#include <math.h>
+#include <time.h>
+
+#include "../../include/fwiedu.h"
+#include "args.h"
+
+static double now(void)
+{
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return t.tv_sec + 1e-9 * t.tv_nsec;
+}
+
+int main(int argc, char **argv)
+{
+    args_init(argc, argv);
+    const int nx = arg_int("nx", 0, 1), nz = arg_int("nz", 0, 1), nt = arg_int("nt", 0, 1);
+    const float dx = arg_float("dx", 0, 1), dz = arg_float("dz", 0, 1), dt = arg_float("dt", 0, 1);
+    const int freesurface = arg_int("freesurface", 0, 0), pml = arg_int("pmlThick", 30, 0);
+    const int gardner = arg_int("Gardner", 1, 0), wd = arg_int("waterdepth", 0, 0);
+    const float vpw = arg_float("vp_water", 1500.0f, 0), rhow = arg_float("rho_water", 1000.0f, 0);
+    const int sampleRate = arg_int("sampleRate", 1, 0);
+    const char *srcfile = arg_string("srcfile", 1), *vpfile = arg_string("vpfile", 1);
+    const char *rhofile = arg_string("rhofile", 0);
+    const char *jsonfile = arg_string("jsonfile", 0);
+    const char *prefix = arg_string("prefix", 0);
+    if (!jsonfile) die("this build reads the geometry from a JSON file: specify", "jsonfile=");
+    if (!gardner && !rhofile) die("didn't specify data file of density model:", "rhofile=");
+
+    float *x = (float *)malloc(sizeof(float) * nx), *z = (float *)malloc(sizeof(float) * nz);
+    for (int i = 0; i < nx; ++i) x[i] = (float)i * dx;
+    for (int i = 0; i < nz; ++i) z[i] = (float)i * dz;
+    float *src = (float *)malloc(sizeof(float) * nt);
+    float **vp = alloc2(nz, nx), **rho = alloc2(nz, nx);
+    read_floats(srcfile, src, (size_t)nt);
+    read_floats(vpfile, vp[0], (size_t)nx * nz);
+    if (gardner)
+        for (long i = 0; i < (long)nx * nz; ++i) rho[0][i] = 310.0f * powf(vp[0][i], 0.25f);
+    else
+        read_floats(rhofile, rho[0], (size_t)nx * nz);
+    set_water(vp, vpw, wd, nx);
+    set_water(rho, rhow, wd, nx);
+
+    const int Nshot = fwiedu_json_shot_count(jsonfile);
+    Record2D *obs = (Record2D *)calloc(Nshot, sizeof(Record2D));
+    Record2D *pre = (Record2D *)calloc(Nshot, sizeof(Record2D));
+    fwiedu_observe2d_from_json(obs, jsonfile, Nshot, x, nx, z, nz, dt, nt, src, sampleRate);
+    create_grid_observe2d(obs, Nshot, dx, dz, nx);
+    for (int s = 0; s < Nshot; ++s) {
+        pre[s] = obs[s];  /* header copy: geometry pointers shared, fresh traces */
+        pre[s].traces_p = alloc2(obs[s].ns, obs[s].ntr);
+    }
+    fprintf(stdout, "%s\nModeling %d shots, %d x %d, nt = %d ...\n", fwiedu_version(), Nshot, nz,
+            nx, nt);
+    fflush(stdout);
+    const double t0 = now();
+    iso_acoustic2d_propagation(vp, rho, obs, Nshot, nz, nx, dx, dz, dt, nt, pml, freesurface, 0, 0,
+                               0, 0, 0, 0, pre, NULL);
+    write_acoustic2d_traces((char *)(prefix ? prefix : "shots_p"), pre, Nshot);
+    fprintf(stdout, "Modeling takes %.3f s\n", now() - t0);
+    return 0;
+}

@@ -249,6 +249,8 @@ void read_gather2d_from_sufile(Record2D *record2d, const char *recordfile, int N
 void fwiedu_observe2d_from_json(Record2D *record2d, const char *jsonfile, int Nshot, float *x,
                                 int nx, float *z, int nz, float dt, int nt, float *src,
                                 int sampleRate);
+/* Shots in the "record" array of a JSON acquisition file. */
+int fwiedu_json_shot_count(const char *jsonfile);
 
 /* ---- Serving loop (no counterpart in the reference) ------------------------
  * A job is one iso_acoustic2d_propagation call of problem 0 (modeling: records
