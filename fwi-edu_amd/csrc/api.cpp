@@ -185,6 +185,8 @@ extern "C" void acoustic_iso2d_fd_it_6order(float **p2, float **p1, float **p0, 
     a.invdz = invdz;
     a.invdxdx = invdxdx;
     a.invdzdz = invdzdz;
+    a.cell_stride = 1;
+    a.fma = c.fma;
     launch_fd(FD_ADJ, a, FdTile(), c.st);
     std::vector<float> o((size_t)fsz);
     d2h(o.data(), d + (CO_N + 2) * fsz, fsz * sizeof(float));
@@ -427,6 +429,8 @@ extern "C" void fwiedu_set_lanes(int wpb, int zrun, int pd, int lds)
     c.tile_req.zrun = zrun;
     c.tile_req.pd = pd;
 }
+extern "C" void fwiedu_set_fma(int on) { ctx().fma = on ? 1 : 0; }
+
 extern "C" void fwiedu_set_graphs(int on)
 {
     Ctx &c = ctx();

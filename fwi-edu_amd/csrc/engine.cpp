@@ -247,7 +247,7 @@ FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const flo
         return c.tile_req;
     }
     const auto key = std::make_tuple(base.Nx, base.ld, base.ixs, base.ixe, base.izs, base.ize,
-                                     base.fs, base.nshot, base.cell_stride);
+                                     base.fs, base.nshot * 2 + base.fma, base.cell_stride);
     auto it = c.tuned.find(key);
     if (it != c.tuned.end()) return it->second;
     FdArgs a = base;
@@ -528,6 +528,7 @@ static FdArgs base_args(const Geom &g, const float *coef, int S, long shot_strid
     a.invdtdt = g.invdtdt;
     a.dt = g.dt;
     a.remap = 0;
+    a.fma = ctx().fma;
     return a;
 }
 
@@ -917,7 +918,7 @@ static void key_push_geom(std::vector<long> &k, const Geom &g)
 
 static std::vector<long> batch_key(int pass, const BatchState &b, bool f1, bool f2)
 {
-    std::vector<long> k = {pass, (long)f1, (long)f2, (long)b.nt, (long)b.S, (long)b.Sp,
+    std::vector<long> k = {pass, (long)f1, (long)f2, (long)ctx().fma, (long)b.nt, (long)b.S, (long)b.Sp,
                            (long)b.lanes, (long)b.ck16, b.sc ? (long)b.sc->d : 0L,
                            b.tb.rcv.n, b.tb.inj.n};
     key_push_geom(k, b.g);

@@ -69,6 +69,7 @@ struct Ctx {
     FdTile tile_req = {0, 1};     // tz == 0: autotune
     int layout_req = -1;          // -1 automatic, 0 per-shot arenas, 1 shots in lanes
     int graphs = 0;               // replay time loops from captured HIP graphs
+    int fma = 0;                  // stencil arithmetic: 0 strict source order, 1 FMA-contracted
     size_t mem_cap = 0;           // serving worker: device bytes its batches may use (0: free memory)
     bool capturing = false;
     std::map<std::vector<long>, GraphEntry> graph_cache;

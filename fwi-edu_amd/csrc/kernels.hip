@@ -64,6 +64,45 @@ __device__ inline int xcd_remap(int b, int nb)
     return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
+// The update of propagation.c:44-71 for one cell: 6th-order first and second
+// derivatives from the stencil arms (zmK / zpK: rows iz -/+ K, xmK / xpK:
+// columns ix -/+ K), then c1*p1 + c2*p0 + (c3*K)*(rho^-1_x dp/dx + rho^-1_z
+// dp/dz + rho^-1 (d2p/dx2 + d2p/dz2)).  FMA = false rounds every product and
+// sum in the order of the C source (bit-identical to the strict-order FP32
+// build); FMA = true fuses each product into the following sum, as an
+// FMA-contracting compile of the reference does (it is built with -ffast-math).
+template <bool FMA>
+__device__ __forceinline__ float fd_update(float pc, float p0v, float zm1, float zp1, float zm2,
+                                           float zp2, float zm3, float zp3, float xm1, float xp1,
+                                           float xm2, float xp2, float xm3, float xp3, float k1,
+                                           float k2, float kA, float kx, float kz, float kr,
+                                           float invdx, float invdz, float invdxdx, float invdzdz)
+{
+    const float C1 = 0.77793115f, C2 = -0.17388691f, C3 = 0.02338713f;
+    const float D0 = -2.8215452f, D1 = 1.57663238f, D2 = -0.18347238f, D3 = 0.01761260f;
+    if (!FMA) {
+        const float dpdz = (C1 * (zp1 - zm1) + C2 * (zp2 - zm2) + C3 * (zp3 - zm3)) * invdz;
+        const float dpdx = (C1 * (xp1 - xm1) + C2 * (xp2 - xm2) + C3 * (xp3 - xm3)) * invdx;
+        const float dzz = (D0 * pc + D1 * (zp1 + zm1) + D2 * (zp2 + zm2) + D3 * (zp3 + zm3)) * invdzdz;
+        const float dxx = (D0 * pc + D1 * (xp1 + xm1) + D2 * (xp2 + xm2) + D3 * (xp3 + xm3)) * invdxdx;
+        return k1 * pc + k2 * p0v + kA * (kx * dpdx + kz * dpdz + kr * (dxx + dzz));
+    } else {
+        const float dpdz =
+            __builtin_fmaf(C3, zp3 - zm3, __builtin_fmaf(C2, zp2 - zm2, C1 * (zp1 - zm1))) * invdz;
+        const float dpdx =
+            __builtin_fmaf(C3, xp3 - xm3, __builtin_fmaf(C2, xp2 - xm2, C1 * (xp1 - xm1))) * invdx;
+        const float d0 = D0 * pc;
+        const float dzz =
+            __builtin_fmaf(D3, zp3 + zm3, __builtin_fmaf(D2, zp2 + zm2, __builtin_fmaf(D1, zp1 + zm1, d0))) *
+            invdzdz;
+        const float dxx =
+            __builtin_fmaf(D3, xp3 + xm3, __builtin_fmaf(D2, xp2 + xm2, __builtin_fmaf(D1, xp1 + xm1, d0))) *
+            invdxdx;
+        const float t = __builtin_fmaf(kx, dpdx, __builtin_fmaf(kz, dpdz, kr * (dxx + dzz)));
+        return __builtin_fmaf(kA, t, __builtin_fmaf(k1, pc, k2 * p0v));
+    }
+}
+
 template <int MODE, int TZ, int SPB>
 __global__ __launch_bounds__(256) void fd_kernel(FdArgs a)
 {
@@ -71,8 +110,6 @@ __global__ __launch_bounds__(256) void fd_kernel(FdArgs a)
     constexpr int NB = (T::HX + NWAVE * T::CPW - 1) / (NWAVE * T::CPW);  // body loads / thread
     static_assert(T::HX * 6 <= 256, "one halo load per thread");
     __shared__ float t[T::HX][T::HZ];
-    const float C1 = 0.77793115f, C2 = -0.17388691f, C3 = 0.02338713f;
-    const float D0 = -2.8215452f, D1 = 1.57663238f, D2 = -0.18347238f, D3 = 0.01761260f;
 
     const int L = a.remap ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int tile = L / a.ngroups, grp = L - tile * a.ngroups;
@@ -171,19 +208,18 @@ __global__ __launch_bounds__(256) void fd_kernel(FdArgs a)
             const int c = 3 + w * T::CPW * CXP + j * T::CPW + xl, r = zl + 3;
             const int i = cell[j];
             const float pc = t[c][r];
-            const float dpdz = (C1 * (t[c][r + 1] - t[c][r - 1]) + C2 * (t[c][r + 2] - t[c][r - 2]) +
-                                C3 * (t[c][r + 3] - t[c][r - 3])) * a.invdz;
-            const float dpdx = (C1 * (t[c + 1][r] - t[c - 1][r]) + C2 * (t[c + 2][r] - t[c - 2][r]) +
-                                C3 * (t[c + 3][r] - t[c - 3][r])) * a.invdx;
-            const float dzz = (D0 * pc + D1 * (t[c][r + 1] + t[c][r - 1]) +
-                               D2 * (t[c][r + 2] + t[c][r - 2]) +
-                               D3 * (t[c][r + 3] + t[c][r - 3])) * a.invdzdz;
-            const float dxx = (D0 * pc + D1 * (t[c + 1][r] + t[c - 1][r]) +
-                               D2 * (t[c + 2][r] + t[c - 2][r]) +
-                               D3 * (t[c + 3][r] + t[c - 3][r])) * a.invdxdx;
             const float p0v = p0c[j];
-            const float v = k1[j] * pc + k2[j] * p0v +
-                            kA[j] * (kx[j] * dpdx + kz[j] * dpdz + kr[j] * (dxx + dzz));
+            const float v =
+                a.fma ? fd_update<true>(pc, p0v, t[c][r - 1], t[c][r + 1], t[c][r - 2], t[c][r + 2],
+                                        t[c][r - 3], t[c][r + 3], t[c - 1][r], t[c + 1][r],
+                                        t[c - 2][r], t[c + 2][r], t[c - 3][r], t[c + 3][r], k1[j],
+                                        k2[j], kA[j], kx[j], kz[j], kr[j], a.invdx, a.invdz,
+                                        a.invdxdx, a.invdzdz)
+                      : fd_update<false>(pc, p0v, t[c][r - 1], t[c][r + 1], t[c][r - 2], t[c][r + 2],
+                                         t[c][r - 3], t[c][r + 3], t[c - 1][r], t[c + 1][r],
+                                         t[c - 2][r], t[c + 2][r], t[c - 3][r], t[c + 3][r], k1[j],
+                                         k2[j], kA[j], kx[j], kz[j], kr[j], a.invdx, a.invdz,
+                                         a.invdxdx, a.invdzdz);
 
             float val = v;
             if (MODE == FD_FWD && i == scell) {
@@ -241,8 +277,6 @@ template <int MODE, int SPB, int PD>
 __global__ __launch_bounds__(1024) void fd_march_kernel(FdArgs a)
 {
     extern __shared__ float ring[];  // [SPB][8][ZB]
-    const float C1 = 0.77793115f, C2 = -0.17388691f, C3 = 0.02338713f;
-    const float D0 = -2.8215452f, D1 = 1.57663238f, D2 = -0.18347238f, D3 = 0.01761260f;
     const int ZB = blockDim.x, tid = threadIdx.x;
     // block order: shot group fastest, so the groups of one strip run together
     // and share its coefficient lines in L2
@@ -325,21 +359,18 @@ __global__ __launch_bounds__(1024) void fd_march_kernel(FdArgs a)
                 const long so = (long)s * a.shot_stride;
                 float *__restrict__ p2 = a.out + so;
                 const float pc = R(si, ix);
-                const float dpdz = (C1 * (Rz(si, ix, 1) - Rz(si, ix, -1)) +
-                                    C2 * (Rz(si, ix, 2) - Rz(si, ix, -2)) +
-                                    C3 * (Rz(si, ix, 3) - Rz(si, ix, -3))) * a.invdz;
-                const float dpdx = (C1 * (R(si, ix + 1) - R(si, ix - 1)) +
-                                    C2 * (R(si, ix + 2) - R(si, ix - 2)) +
-                                    C3 * (R(si, ix + 3) - R(si, ix - 3))) * a.invdx;
-                const float dzz = (D0 * pc + D1 * (Rz(si, ix, 1) + Rz(si, ix, -1)) +
-                                   D2 * (Rz(si, ix, 2) + Rz(si, ix, -2)) +
-                                   D3 * (Rz(si, ix, 3) + Rz(si, ix, -3))) * a.invdzdz;
-                const float dxx = (D0 * pc + D1 * (R(si, ix + 1) + R(si, ix - 1)) +
-                                   D2 * (R(si, ix + 2) + R(si, ix - 2)) +
-                                   D3 * (R(si, ix + 3) + R(si, ix - 3))) * a.invdxdx;
                 const float p0v = p0c[si];
+                const float zm1 = Rz(si, ix, -1), zp1 = Rz(si, ix, 1), zm2 = Rz(si, ix, -2);
+                const float zp2 = Rz(si, ix, 2), zm3 = Rz(si, ix, -3), zp3 = Rz(si, ix, 3);
+                const float xm1 = R(si, ix - 1), xp1 = R(si, ix + 1), xm2 = R(si, ix - 2);
+                const float xp2 = R(si, ix + 2), xm3 = R(si, ix - 3), xp3 = R(si, ix + 3);
                 const float v =
-                    k[0] * pc + k[1] * p0v + k[2] * (k[3] * dpdx + k[4] * dpdz + k[5] * (dxx + dzz));
+                    a.fma ? fd_update<true>(pc, p0v, zm1, zp1, zm2, zp2, zm3, zp3, xm1, xp1, xm2, xp2,
+                                            xm3, xp3, k[0], k[1], k[2], k[3], k[4], k[5], a.invdx,
+                                            a.invdz, a.invdxdx, a.invdzdz)
+                          : fd_update<false>(pc, p0v, zm1, zp1, zm2, zp2, zm3, zp3, xm1, xp1, xm2,
+                                             xp2, xm3, xp3, k[0], k[1], k[2], k[3], k[4], k[5],
+                                             a.invdx, a.invdz, a.invdxdx, a.invdzdz);
                 int scell = -1;
                 if (MODE != FD_ADJ && a.src_cell) scell = a.src_cell[s];
                 float val = v;
@@ -509,7 +540,7 @@ __device__ __forceinline__ void lput(frsrc f, long cell, long CS, unsigned voff,
 // the forward field of the imaging condition and s the shot of the lane.
 // Accumulators are [cell][shot] (acc_stride 1).  Arithmetic order and the
 // epilogue (source, free-surface mirror, accumulation) are those of fd_kernel.
-template <int MODE>
+template <int MODE, bool FMA>
 __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long fsz, long colb,
                                              int ix, int iz, unsigned s, int scell, frsrc P2,
                                              frsrc U, long CS,
@@ -517,18 +548,12 @@ __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long f
                                              float q5, float q6, float xm1, float xp1, float xm2,
                                              float xp2, float xm3, float xp3, float p0v)
 {
-    const float C1 = 0.77793115f, C2 = -0.17388691f, C3 = 0.02338713f;
-    const float D0 = -2.8215452f, D1 = 1.57663238f, D2 = -0.18347238f, D3 = 0.01761260f;
     const long i = colb + iz;
     const float k1 = cf[CO_C1 * fsz + i], k2 = cf[CO_C2 * fsz + i], kA = cf[CO_A * fsz + i];
     const float kx = cf[CO_IRX * fsz + i], kz = cf[CO_IRZ * fsz + i], kr = cf[CO_IR * fsz + i];
     const float pc = q3;
-    const float dpdz = (C1 * (q4 - q2) + C2 * (q5 - q1) + C3 * (q6 - q0)) * a.invdz;
-    const float dpdx = (C1 * (xp1 - xm1) + C2 * (xp2 - xm2) + C3 * (xp3 - xm3)) * a.invdx;
-    const float dzz = (D0 * pc + D1 * (q4 + q2) + D2 * (q5 + q1) + D3 * (q6 + q0)) * a.invdzdz;
-    const float dxx =
-        (D0 * pc + D1 * (xp1 + xm1) + D2 * (xp2 + xm2) + D3 * (xp3 + xm3)) * a.invdxdx;
-    const float v = k1 * pc + k2 * p0v + kA * (kx * dpdx + kz * dpdz + kr * (dxx + dzz));
+    const float v = fd_update<FMA>(pc, p0v, q2, q4, q1, q5, q0, q6, xm1, xp1, xm2, xp2, xm3, xp3, k1,
+                                   k2, kA, kx, kz, kr, a.invdx, a.invdz, a.invdxdx, a.invdzdz);
 
     float val = v;
     if (MODE == FD_FWD && (int)i == scell) {
@@ -569,7 +594,7 @@ __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long f
 // global memory (the columns of the neighbouring waves, mostly L1/L2 hits).
 // The eight loads of the next PD rows are issued before a row is computed, so
 // a wave always has memory requests in flight.
-template <int MODE, int PD>
+template <int MODE, int PD, bool FMA>
 __global__ __launch_bounds__(512) void fd_lanes_kernel(FdArgs a)
 {
     const int lane = threadIdx.x & 63;
@@ -639,7 +664,7 @@ __global__ __launch_bounds__(512) void fd_lanes_kernel(FdArgs a)
         const float q6 = r[0];
         const float xm1 = r[1], xp1 = r[2], xm2 = r[3], xp2 = r[4], xm3 = r[5], xp3 = r[6];
         const float p0v = r[7];
-        lanes_update<MODE>(a, cf, fsz, colb, ix, iz, s, scell, P2, U, CS, q0, q1, q2, q3, q4, q5, q6,
+        lanes_update<MODE, FMA>(a, cf, fsz, colb, ix, iz, s, scell, P2, U, CS, q0, q1, q2, q3, q4, q5, q6,
                            xm1, xp1, xm2, xp2, xm3, xp3, p0v);
         q0 = q1;
         q1 = q2;
@@ -663,12 +688,21 @@ static void launch_lanes_t(const FdArgs &a0, FdTile tl, hipStream_t st)
     a.lz_nblk = (int)nb;
     const long nlaunch = a.remap ? (nb + 7) / 8 * 8 : nb;
     const dim3 grid((unsigned)nlaunch), block(tl.wpb * 64);
-    if (tl.pd >= 3)
-        fd_lanes_kernel<MODE, 3><<<grid, block, 0, st>>>(a);
-    else if (tl.pd == 2)
-        fd_lanes_kernel<MODE, 2><<<grid, block, 0, st>>>(a);
-    else
-        fd_lanes_kernel<MODE, 1><<<grid, block, 0, st>>>(a);
+    if (a.fma) {
+        if (tl.pd >= 3)
+            fd_lanes_kernel<MODE, 3, true><<<grid, block, 0, st>>>(a);
+        else if (tl.pd == 2)
+            fd_lanes_kernel<MODE, 2, true><<<grid, block, 0, st>>>(a);
+        else
+            fd_lanes_kernel<MODE, 1, true><<<grid, block, 0, st>>>(a);
+    } else {
+        if (tl.pd >= 3)
+            fd_lanes_kernel<MODE, 3, false><<<grid, block, 0, st>>>(a);
+        else if (tl.pd == 2)
+            fd_lanes_kernel<MODE, 2, false><<<grid, block, 0, st>>>(a);
+        else
+            fd_lanes_kernel<MODE, 1, false><<<grid, block, 0, st>>>(a);
+    }
 }
 
 // Shots in lanes with an LDS row exchange.  A block of WPB waves owns WPB
@@ -679,7 +713,7 @@ static void launch_lanes_t(const FdArgs &a0, FdTile tl, hipStream_t st)
 // of 7 * WPB.  The LDS row is double buffered: one barrier per row.  Waves
 // whose column lies past ixe still load and publish their column (an active
 // neighbour may need it) but compute nothing.
-template <int MODE, int WPB, int PD>
+template <int MODE, int WPB, int PD, bool FMA>
 __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
 {
     constexpr int NC = WPB + 6;                    // columns of the LDS row
@@ -760,7 +794,7 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
             const float xm1 = row[buf][2 + w][lane], xp1 = row[buf][4 + w][lane];
             const float xm2 = row[buf][1 + w][lane], xp2 = row[buf][5 + w][lane];
             const float xm3 = row[buf][0 + w][lane], xp3 = row[buf][6 + w][lane];
-            lanes_update<MODE>(a, cf, fsz, colb, ix, iz, s, scell, P2, U, CS, q0, q1, q2, q3, q4,
+            lanes_update<MODE, FMA>(a, cf, fsz, colb, ix, iz, s, scell, P2, U, CS, q0, q1, q2, q3, q4,
                                q5, q6, xm1, xp1, xm2, xp2, xm3, xp3, p0v);
         }
         q0 = q1;
@@ -775,10 +809,17 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
 template <int MODE, int WPB>
 static void launch_lanes_lds_w(const FdArgs &a, const dim3 &grid, int pd, hipStream_t st)
 {
-    if (pd >= 2)
-        fd_lanes_lds_kernel<MODE, WPB, 2><<<grid, WPB * 64, 0, st>>>(a);
-    else
-        fd_lanes_lds_kernel<MODE, WPB, 1><<<grid, WPB * 64, 0, st>>>(a);
+    if (a.fma) {
+        if (pd >= 2)
+            fd_lanes_lds_kernel<MODE, WPB, 2, true><<<grid, WPB * 64, 0, st>>>(a);
+        else
+            fd_lanes_lds_kernel<MODE, WPB, 1, true><<<grid, WPB * 64, 0, st>>>(a);
+    } else {
+        if (pd >= 2)
+            fd_lanes_lds_kernel<MODE, WPB, 2, false><<<grid, WPB * 64, 0, st>>>(a);
+        else
+            fd_lanes_lds_kernel<MODE, WPB, 1, false><<<grid, WPB * 64, 0, st>>>(a);
+    }
 }
 
 template <int MODE>

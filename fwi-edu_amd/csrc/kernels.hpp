@@ -65,6 +65,7 @@ struct FdArgs {
     int zfast;              // tile order: z-neighbours adjacent (1) or x-neighbours (0)
     int mx_len, mx_nseg, mz_len, mz_nseg;  // column-march plan
     int lz_run, lz_nrun, lz_ngrp, lz_nblk; // shots-in-lanes plan
+    int fma;                // 1: FMA-contracted update arithmetic, 0: strict source order
     // source (FD_FWD / FD_RECON)
     const int *src_cell;    // [nshot] flat padded index or -1
     const float *src_wav;   // [nshot][nt]

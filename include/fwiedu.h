@@ -166,6 +166,12 @@ void fwiedu_set_xcd_remap(int on);
  * Applies to iso_acoustic2d_propagation and sessions; fwiedu_set_tile(0, 0)
  * returns to the autotuner. */
 void fwiedu_set_lanes(int wpb, int zrun, int pd, int lds);
+/* Stencil arithmetic.  0 (default): every product and sum rounded in the
+ * order of the reference's C source (bit-identical to a strict IEEE build of
+ * it).  1: each product fused into the following sum (FMA), as a contracting
+ * compile of the reference (it is built with -ffast-math) rounds; fewer
+ * instructions, results within FP32 rounding of the strict ones. */
+void fwiedu_set_fma(int on);
 /* 1: run each batch's forward and adjoint time loops as HIP graphs, captured
  * on first use and replayed whenever the same batch (same device buffers,
  * sizes, schedule and launch shape) comes again: the next gradient of a

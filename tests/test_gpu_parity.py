@@ -540,3 +540,59 @@ def test_device_fwi_iteration(L, oracles, gardner):
             vp_o, rho_o = ref["vp"], ref["rho"]
     finally:
         L.fwiedu_session_destroy(sess)
+
+
+def test_fma_arithmetic(L, oracles):
+    """fwiedu_set_fma(1): the update with fused multiply-adds.  Every launch
+    shape and both layouts still give the same bits as each other (the result
+    depends only on the arithmetic mode), and the results stay within the
+    reference tolerances of the FP32 oracle (records 1e-5, gradient 1e-4,
+    misfit 1e-5 relative, or 1.5x the spread between the two FP32 oracle
+    builds when larger) and close to the FP64 oracle."""
+    w = synth.workload("tiny", nshot=5, freesurface=1, shot_depth_cells=1, nx=70)
+    say("case: fma arithmetic tiny fs=1 5 shots")
+    obs = observed(oracles["f32s"], w, False)
+    L.fwiedu_set_fma(1)
+    try:
+        shapes = [("tile", 64, 8, 1), ("tile", 32, 1, 0), ("march", 16, 2, 4), ("lanes", 8, 16, 0),
+                  ("lanes", 4, 8, 1)]
+        out = []
+        for kind, p, q, r in shapes:
+            if kind == "tile":
+                L.fwiedu_set_tile(p, q)
+                L.fwiedu_set_tile_order(r)
+            elif kind == "march":
+                L.fwiedu_set_march(p, q, r)
+            else:
+                L.fwiedu_set_lanes(p, q, 1, r)
+            out.append(gradient(L, w, obs, True))
+        L.fwiedu_set_tile(0, 0)
+        for (pre, g), sh in zip(out[1:], shapes[1:]):
+            assert np.array_equal(g, out[0][1]), sh
+            assert np.array_equal(recs(pre), recs(out[0][0])), sh
+        for case in ("tiny_fs1_T2", "c0"):
+            p = dict(CASES[case])
+            wc = synth.workload(p.pop("name"), **p)
+            rec_g = recs(observed(L, wc, True))
+            ob = observed(oracles["f32s"], wc, False)
+            pre_g, g_g = gradient(L, wc, ob, True)
+            f_g = L.calculate_objective_value(pre_g.recs, wc.nshot)
+            ref = {}
+            for k in ("f32", "f32s", "f64"):
+                pre_o, g_o = gradient(oracles[k], wc, ob, False)
+                ref[k] = (recs(observed(oracles[k], wc, False)), g_o,
+                          oracles[k].objective(pre_o.recs, wc.nshot))
+            spread = (rel(ref["f32"][0], ref["f32s"][0]), rel(ref["f32"][1], ref["f32s"][1]),
+                      abs(ref["f32"][2] - ref["f32s"][2]) / ref["f32s"][2])
+            e32 = (rel(rec_g, ref["f32"][0]), rel(g_g, ref["f32"][1]),
+                   abs(f_g - ref["f32"][2]) / ref["f32"][2])
+            e64 = (rel(rec_g, ref["f64"][0]), rel(g_g, ref["f64"][1]))
+            say(f"  {case}: vs f32 (records, grad, misfit) {e32}, vs f64 {e64}, spread {spread}")
+            assert e32[0] <= max(1e-5, 1.5 * spread[0])
+            assert e32[1] <= max(1e-4, 1.5 * spread[1])
+            assert e32[2] <= max(1e-5, 1.5 * spread[2])
+            assert e64[0] <= 1e-4 and e64[1] <= 1e-3
+    finally:
+        L.fwiedu_set_fma(0)
+        L.fwiedu_set_tile(0, 0)
+        L.fwiedu_set_tile_order(0)
