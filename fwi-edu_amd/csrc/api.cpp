@@ -413,16 +413,17 @@ extern "C" void fwiedu_set_lanes(int wpb, int zrun, int pd, int lds)
     const bool zok = zrun == 8 || zrun == 16 || zrun == 32 || zrun == 64 || zrun == 128 ||
                      zrun == 256;
     if (lds) {
-        if ((wpb != 4 && wpb != 8 && wpb != 16) || !zok || pd < 1 || pd > 2)
+        if ((wpb != 4 && wpb != 8 && wpb != 16) || !zok || pd < 1 || pd > 4)
             fail("<error>: fwiedu_set_lanes (LDS row exchange): 4, 8 or 16 columns per block, "
-                 "8 to 256 rows per block, 1 or 2 rows of loads in flight");
+                 "8 to 256 rows per block, 1 to 4 rows of loads in flight");
     } else if ((wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8) || !zok || pd < 1 || pd > 3) {
         fail("<error>: fwiedu_set_lanes: 1, 2, 4 or 8 columns per block, 8 to 256 rows per wave, "
              "1 to 3 rows of loads in flight");
     }
-    const int rm = c.tile_req.remap;
+    const int rm = c.tile_req.remap, zf = c.tile_req.zfast;
     c.tile_req = FdTile();
     c.tile_req.remap = rm;
+    c.tile_req.zfast = zf;
     c.tile_req.tz = 64;
     c.tile_req.kind = lds ? 3 : 2;
     c.tile_req.wpb = wpb;

@@ -265,19 +265,21 @@ FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const flo
         for (int wpb : {4, 8})
             for (int zrun : {8, 16})
                 for (int rm : {0, 1})
-                    for (int pd : {1, 2}) {
+                    for (int sgo : {0, 1}) {
+                        if (sgo && base.nshot <= 64) continue;  // one shot group: same order
                         FdTile t;
                         t.tz = 64;
                         t.kind = 2;
                         t.wpb = wpb;
                         t.zrun = zrun;
                         t.remap = rm;
-                        t.pd = pd;
+                        t.zfast = sgo;
+                        t.pd = 1;
                         cand.push_back(t);
                     }
-        for (int wpb : {8})
-            for (int zrun : {32})
-                for (int rm : {0})
+        for (int wpb : {8, 16})
+            for (int zrun : {16, 32})
+                for (int rm : {0, 1})
                     for (int pd : {2}) {
                         FdTile t;
                         t.tz = 64;

@@ -536,7 +536,8 @@ __device__ __forceinline__ void lput(frsrc f, long cell, long CS, unsigned voff,
 
 // One cell of the stencil for the 64 shots of a wave (shots-in-lanes layout):
 // q0..q6 are rows iz-3..iz+3 of the wave's column, x** the row-iz values of
-// the columns ix-3..ix+3, p0v the previous field; P2 is the output field, U
+// the columns ix-3..ix+3, p0v the previous field, k the six coefficients c1,
+// c2, c3*K, rho^-1_x, rho^-1_z, rho^-1 of the cell; P2 is the output field, U
 // the forward field of the imaging condition and s the shot of the lane.
 // Accumulators are [cell][shot] (acc_stride 1).  Arithmetic order and the
 // epilogue (source, free-surface mirror, accumulation) are those of fd_kernel.
@@ -546,14 +547,14 @@ __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long f
                                              frsrc U, long CS,
                                              float q0, float q1, float q2, float q3, float q4,
                                              float q5, float q6, float xm1, float xp1, float xm2,
-                                             float xp2, float xm3, float xp3, float p0v)
+                                             float xp2, float xm3, float xp3, float p0v,
+                                             const float *k)
 {
     const long i = colb + iz;
-    const float k1 = cf[CO_C1 * fsz + i], k2 = cf[CO_C2 * fsz + i], kA = cf[CO_A * fsz + i];
-    const float kx = cf[CO_IRX * fsz + i], kz = cf[CO_IRZ * fsz + i], kr = cf[CO_IR * fsz + i];
     const float pc = q3;
-    const float v = fd_update<FMA>(pc, p0v, q2, q4, q1, q5, q0, q6, xm1, xp1, xm2, xp2, xm3, xp3, k1,
-                                   k2, kA, kx, kz, kr, a.invdx, a.invdz, a.invdxdx, a.invdzdz);
+    const float v = fd_update<FMA>(pc, p0v, q2, q4, q1, q5, q0, q6, xm1, xp1, xm2, xp2, xm3, xp3,
+                                   k[0], k[1], k[2], k[3], k[4], k[5], a.invdx, a.invdz, a.invdxdx,
+                                   a.invdzdz);
 
     float val = v;
     if (MODE == FD_FWD && (int)i == scell) {
@@ -593,7 +594,9 @@ __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long f
 // Shots in lanes, no LDS: the 6 x-neighbours of a wave's cell are loaded from
 // global memory (the columns of the neighbouring waves, mostly L1/L2 hits).
 // The eight loads of the next PD rows are issued before a row is computed, so
-// a wave always has memory requests in flight.
+// a wave always has memory requests in flight.  (Issuing the six scalar
+// coefficient loads that early as well was measured slower: they stay with
+// the row's arithmetic.)
 template <int MODE, int PD, bool FMA>
 __global__ __launch_bounds__(512) void fd_lanes_kernel(FdArgs a)
 {
@@ -611,7 +614,11 @@ __global__ __launch_bounds__(512) void fd_lanes_kernel(FdArgs a)
     }
     const int zr = b % a.lz_nrun;
     b /= a.lz_nrun;
-    const int sg = b % a.lz_ngrp, xg = b / a.lz_ngrp;
+    // shot group outside the column groups (x-neighbouring blocks close in
+    // launch order, so their shared halo columns meet in L2) or inside them
+    // (the shot groups of one column group together, sharing coefficients)
+    const int sg = a.lz_sgslow ? b / a.lz_nxg : b % a.lz_ngrp;
+    const int xg = a.lz_sgslow ? b % a.lz_nxg : b / a.lz_ngrp;
     const int ix = a.ixs + xg * wpb + w;
     if (ix > a.ixe) return;  // the whole wave
     const int iz0 = a.izs + zr * a.lz_run;
@@ -664,8 +671,11 @@ __global__ __launch_bounds__(512) void fd_lanes_kernel(FdArgs a)
         const float q6 = r[0];
         const float xm1 = r[1], xp1 = r[2], xm2 = r[3], xp2 = r[4], xm3 = r[5], xp3 = r[6];
         const float p0v = r[7];
-        lanes_update<MODE, FMA>(a, cf, fsz, colb, ix, iz, s, scell, P2, U, CS, q0, q1, q2, q3, q4, q5, q6,
-                           xm1, xp1, xm2, xp2, xm3, xp3, p0v);
+        const long i = colb + iz;
+        const float k[6] = {cf[CO_C1 * fsz + i],  cf[CO_C2 * fsz + i],  cf[CO_A * fsz + i],
+                            cf[CO_IRX * fsz + i], cf[CO_IRZ * fsz + i], cf[CO_IR * fsz + i]};
+        lanes_update<MODE, FMA>(a, cf, fsz, colb, ix, iz, s, scell, P2, U, CS, q0, q1, q2, q3, q4,
+                                q5, q6, xm1, xp1, xm2, xp2, xm3, xp3, p0v, k);
         q0 = q1;
         q1 = q2;
         q2 = q3;
@@ -686,6 +696,8 @@ static void launch_lanes_t(const FdArgs &a0, FdTile tl, hipStream_t st)
     const int nxg = (a.ixe - a.ixs + 1 + tl.wpb - 1) / tl.wpb;
     const long nb = (long)nxg * a.lz_nrun * a.lz_ngrp;
     a.lz_nblk = (int)nb;
+    a.lz_nxg = nxg;
+    a.lz_sgslow = tl.zfast;
     const long nlaunch = a.remap ? (nb + 7) / 8 * 8 : nb;
     const dim3 grid((unsigned)nlaunch), block(tl.wpb * 64);
     if (a.fma) {
@@ -729,7 +741,11 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
     }
     const int zr = b % a.lz_nrun;
     b /= a.lz_nrun;
-    const int sg = b % a.lz_ngrp, xg = b / a.lz_ngrp;
+    // shot group outside the column groups (x-neighbouring blocks close in
+    // launch order, so their shared halo columns meet in L2) or inside them
+    // (the shot groups of one column group together, sharing coefficients)
+    const int sg = a.lz_sgslow ? b / a.lz_nxg : b % a.lz_ngrp;
+    const int xg = a.lz_sgslow ? b % a.lz_nxg : b / a.lz_ngrp;
     const int x0 = a.ixs + xg * WPB;
     const int ix = x0 + w;
     const bool active = ix <= a.ixe;
@@ -794,8 +810,11 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
             const float xm1 = row[buf][2 + w][lane], xp1 = row[buf][4 + w][lane];
             const float xm2 = row[buf][1 + w][lane], xp2 = row[buf][5 + w][lane];
             const float xm3 = row[buf][0 + w][lane], xp3 = row[buf][6 + w][lane];
+            const long i = colb + iz;
+            const float k[6] = {cf[CO_C1 * fsz + i],  cf[CO_C2 * fsz + i],  cf[CO_A * fsz + i],
+                                cf[CO_IRX * fsz + i], cf[CO_IRZ * fsz + i], cf[CO_IR * fsz + i]};
             lanes_update<MODE, FMA>(a, cf, fsz, colb, ix, iz, s, scell, P2, U, CS, q0, q1, q2, q3, q4,
-                               q5, q6, xm1, xp1, xm2, xp2, xm3, xp3, p0v);
+                                    q5, q6, xm1, xp1, xm2, xp2, xm3, xp3, p0v, k);
         }
         q0 = q1;
         q1 = q2;
@@ -810,12 +829,20 @@ template <int MODE, int WPB>
 static void launch_lanes_lds_w(const FdArgs &a, const dim3 &grid, int pd, hipStream_t st)
 {
     if (a.fma) {
-        if (pd >= 2)
+        if (pd >= 4)
+            fd_lanes_lds_kernel<MODE, WPB, 4, true><<<grid, WPB * 64, 0, st>>>(a);
+        else if (pd == 3)
+            fd_lanes_lds_kernel<MODE, WPB, 3, true><<<grid, WPB * 64, 0, st>>>(a);
+        else if (pd == 2)
             fd_lanes_lds_kernel<MODE, WPB, 2, true><<<grid, WPB * 64, 0, st>>>(a);
         else
             fd_lanes_lds_kernel<MODE, WPB, 1, true><<<grid, WPB * 64, 0, st>>>(a);
     } else {
-        if (pd >= 2)
+        if (pd >= 4)
+            fd_lanes_lds_kernel<MODE, WPB, 4, false><<<grid, WPB * 64, 0, st>>>(a);
+        else if (pd == 3)
+            fd_lanes_lds_kernel<MODE, WPB, 3, false><<<grid, WPB * 64, 0, st>>>(a);
+        else if (pd == 2)
             fd_lanes_lds_kernel<MODE, WPB, 2, false><<<grid, WPB * 64, 0, st>>>(a);
         else
             fd_lanes_lds_kernel<MODE, WPB, 1, false><<<grid, WPB * 64, 0, st>>>(a);
@@ -833,6 +860,8 @@ static void launch_lanes_lds_t(const FdArgs &a0, FdTile tl, hipStream_t st)
     const int nxg = (a.ixe - a.ixs + 1 + tl.wpb - 1) / tl.wpb;
     const long nb = (long)nxg * a.lz_nrun * a.lz_ngrp;
     a.lz_nblk = (int)nb;
+    a.lz_nxg = nxg;
+    a.lz_sgslow = tl.zfast;
     const dim3 grid((unsigned)(a.remap ? (nb + 7) / 8 * 8 : nb));
     if (tl.wpb >= 16)
         launch_lanes_lds_w<MODE, 16>(a, grid, tl.pd, st);

@@ -148,7 +148,10 @@ void fwiedu_set_batch(int shots_per_launch);
  * once per geometry and keep the fastest. */
 void fwiedu_set_tile(int tz, int spb);
 /* Tile order of a forced tile shape: 1 puts vertically adjacent tiles next to
- * each other in launch order, 0 horizontally adjacent ones. */
+ * each other in launch order, 0 horizontally adjacent ones.  For a forced
+ * shots-in-lanes shape (set before fwiedu_set_lanes): 1 runs all column groups
+ * of one 64-shot group before the next group, 0 the shot groups of a column
+ * group together. */
 void fwiedu_set_tile_order(int zfast);
 /* Force the column-marching stencil: txm (16, 32, 64) columns per block, spb
  * (1, 2, 4) shots per block, pd (1, 2, 4) columns of loads kept in flight.
@@ -162,7 +165,7 @@ void fwiedu_set_xcd_remap(int on);
  * of a wavefront are 64 shots of one cell): wpb columns per block, each wave
  * marching zrun (8 ... 256) rows with the loads of the next pd rows in flight.
  * lds = 0: x-neighbours loaded from memory (wpb 1, 2, 4, 8; pd 1 ... 3);
- * lds = 1: x-neighbours exchanged through LDS (wpb 4, 8, 16; pd 1, 2).
+ * lds = 1: x-neighbours exchanged through LDS (wpb 4, 8, 16; pd 1 ... 4).
  * Applies to iso_acoustic2d_propagation and sessions; fwiedu_set_tile(0, 0)
  * returns to the autotuner. */
 void fwiedu_set_lanes(int wpb, int zrun, int pd, int lds);

@@ -319,7 +319,7 @@ def test_every_stencil_shape_bit_identical(L, oracles, fs):
     shapes += [("lanes", wpb, zrun, (rm, pd, 0)) for wpb in (1, 4, 8) for zrun in (8, 16, 64, 128)
                for rm in (0, 1) for pd in (1, 2, 3)]
     shapes += [("lanes", wpb, zrun, (rm, pd, 1)) for wpb in (4, 8, 16) for zrun in (8, 32, 256)
-               for rm in (0, 1) for pd in (1, 2)]
+               for rm in (0, 1) for pd in (1, 2, 3, 4)]
     try:
         for kind, p, spb, zf in shapes:
             if kind == "tile":
@@ -354,11 +354,19 @@ def test_lanes_layout_matches_arenas(L):
             L.fwiedu_set_batch(batch)
             out[name] = gradient(L, w, obs, True)
             assert flib.stat("layout") == layout, name
+        L.fwiedu_set_batch(0)
+        for order in (0, 1):
+            # forced shapes: shot groups inside / outside the column groups
+            L.fwiedu_set_tile_order(order)
+            L.fwiedu_set_lanes(4, 8, 1, 0)
+            out[f"lanes_order{order}"] = gradient(L, w, obs, True)
     finally:
         L.fwiedu_set_layout(-1)
         L.fwiedu_set_batch(0)
+        L.fwiedu_set_tile(0, 0)
+        L.fwiedu_set_tile_order(0)
     pre_a, g_a = out["arenas"]
-    for name in ("lanes", "lanes33"):
+    for name in ("lanes", "lanes33", "lanes_order0", "lanes_order1"):
         pre_l, g_l = out[name]
         assert np.array_equal(g_l, g_a), name
         assert np.array_equal(recs(pre_l), recs(pre_a)), name
