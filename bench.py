@@ -78,14 +78,16 @@ def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
 # XCD order): FETCH_SIZE x 2 + WRITE_SIZE from rocprofv3 counter runs scaled by
 # a 512 MiB copy; the runs and the calibration are in
 # profiles/fd_lanes_recon_c1_traffic.md.
-C1_LANES_TRAFFIC = {(8, 8, 1, 0): 447.5e6, (8, 16, 1, 1): 345.5e6, (4, 8, 1, 0): 518.8e6,
-                    (8, 8, 2, 0): 447.3e6}
+C1_LANES_TRAFFIC = {(2, 8, 8, 1, 0): 447.5e6, (2, 8, 16, 1, 1): 345.5e6, (2, 4, 8, 1, 0): 518.8e6,
+                    (2, 8, 8, 2, 0): 447.3e6, (3, 8, 16, 2, 0): 413.9e6, (3, 8, 16, 2, 1): 352.4e6}
 
 
 def measured_traffic(w, st):
-    if w.name != "c1" or w.nshot != 120 or int(st["tile_kind"]) != 2:
+    kind = int(st["tile_kind"])
+    if w.name != "c1" or w.nshot != 120 or kind not in (2, 3) or int(st["tile_zfast"]) != 0:
         return None
-    key = (int(st["tile_wpb"]), int(st["tile_zrun"]), int(st["tile_pd"]), int(st["tile_remap"]))
+    key = (kind, int(st["tile_wpb"]), int(st["tile_zrun"]), int(st["tile_pd"]),
+           int(st["tile_remap"]))
     return C1_LANES_TRAFFIC.get(key)
 
 
@@ -107,7 +109,7 @@ def stencil_shape(st):
                  prefetch=int(st["tile_pd"]))
     else:
         d.update(cols_per_block=int(st["tile_wpb"]), rows_per_wave=int(st["tile_zrun"]),
-                 prefetch_rows=int(st["tile_pd"]))
+                 prefetch_rows=int(st["tile_pd"]), block_order=int(st["tile_zfast"]))
     return d
 
 
@@ -207,7 +209,8 @@ def main():
     ap.add_argument("--ckpt-bits", type=int, default=32, choices=(32, 16),
                     help="checkpoint storage: FP32 or FP16 with a per-snapshot scale")
     ap.add_argument("--lanes", default="",
-                    help="force the shots-in-lanes stencil shape WPB,ZRUN,PD,XCD (default: autotuned)")
+                    help="force the shots-in-lanes stencil shape WPB,ZRUN,PD,XCD[,LDS] "
+                         "(default: autotuned)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -219,9 +222,10 @@ def main():
         sys.exit(1)
     L.fwiedu_set_checkpoint_bits(args.ckpt_bits)
     if args.lanes:
-        wpb, zrun, pd, xcd = (int(v) for v in args.lanes.split(","))
+        v = [int(x) for x in args.lanes.split(",")]
+        wpb, zrun, pd, xcd = v[:4]
         L.fwiedu_set_xcd_remap(xcd)
-        L.fwiedu_set_lanes(wpb, zrun, pd, 0)
+        L.fwiedu_set_lanes(wpb, zrun, pd, v[4] if len(v) > 4 else 0)
     over = {"nshot": args.nshot} if args.nshot else {}
     if args.nt:
         over["nt"] = args.nt

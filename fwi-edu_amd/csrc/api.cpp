@@ -376,7 +376,11 @@ extern "C" float calculate_memory_cost_checkpoint(int nt, int NX, int NZ)
 // fwiedu_* controls
 
 extern "C" void fwiedu_set_batch(int s) { ctx().batch_req = s < 0 ? 0 : s; }
-extern "C" void fwiedu_set_tile_order(int zfast) { ctx().tile_req.zfast = zfast ? 1 : 0; }
+extern "C" void fwiedu_set_tile_order(int order)
+{
+    if (order < 0 || order > 2) fail("<error>: fwiedu_set_tile_order: 0, 1 or 2 (got %d)", order);
+    ctx().tile_req.zfast = order;
+}
 
 extern "C" void fwiedu_set_tile(int tz, int spb)
 {

@@ -265,8 +265,7 @@ FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const flo
         for (int wpb : {4, 8})
             for (int zrun : {8, 16})
                 for (int rm : {0, 1})
-                    for (int sgo : {0, 1}) {
-                        if (sgo && base.nshot <= 64) continue;  // one shot group: same order
+                    for (int sgo : {0, 2}) {
                         FdTile t;
                         t.tz = 64;
                         t.kind = 2;
@@ -280,10 +279,12 @@ FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const flo
         for (int wpb : {8, 16})
             for (int zrun : {16, 32})
                 for (int rm : {0, 1})
-                    for (int pd : {2}) {
+                    for (int order : {0, 2}) {
                         FdTile t;
                         t.tz = 64;
                         t.kind = 3;
+                        t.zfast = order;
+                        const int pd = 2;
                         t.wpb = wpb;
                         t.zrun = zrun;
                         t.remap = rm;
@@ -1012,7 +1013,7 @@ static size_t per_shot_bytes(const Geom &g, const Schedule &sc, bool adjoint, bo
 // multiple of 64; it is used when at least 48 shots share a launch (below
 // that a third of the lanes or more would idle) unless a layout or a stencil
 // shape is forced.
-static int choose_batch(int n, size_t per_shot, bool &lanes)
+static int choose_batch(int n, size_t per_shot, long fsz, bool &lanes)
 {
     Ctx &c = ctx();
     long Smax;
@@ -1046,6 +1047,19 @@ static int choose_batch(int n, size_t per_shot, bool &lanes)
             S = Smax / 64 * 64;
         else if (c.tile_req.tz <= 0 && c.layout_req < 0)
             lanes = false;
+    }
+    if (lanes) {
+        // the lanes stencils address a field with 32-bit buffer offsets: one
+        // field of the padded batch must stay below 4 GiB
+        const long max_lanes = (0xFFFFFFFFL / (fsz * 4)) / 64 * 64;
+        if (max_lanes < 64) {
+            if (c.tile_req.tz > 0 && c.tile_req.kind >= 2)
+                fail("<error>: the shots-in-lanes stencil cannot address this grid (one field of 64 "
+                     "shots exceeds 4 GiB)");
+            lanes = false;
+        } else if ((S + 63) / 64 * 64 > max_lanes) {
+            S = max_lanes;
+        }
     }
     return (int)S;
 }
@@ -1164,7 +1178,7 @@ void propagate_core(const Call &P)
         bool lanes = false;
         const bool ck16 = c.ckpt_bits == 16;
         const int S = choose_batch(
-            (int)grp.size(), per_shot_bytes(g, sc, problem == 1, ck16, recmax, nt), lanes);
+            (int)grp.size(), per_shot_bytes(g, sc, problem == 1, ck16, recmax, nt), g.fsz, lanes);
         c.stats.batch = S;
         c.stats.layout = lanes ? 1 : 0;
         for (size_t b0 = 0; b0 < grp.size(); b0 += S) {

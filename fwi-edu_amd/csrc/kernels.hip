@@ -591,6 +591,27 @@ __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long f
     }
 }
 
+// Block -> (z-run, 64-shot group, column group) of the lanes stencils.
+// order 0: z-runs fastest, then shot groups (the shot groups of one column
+// strip together, sharing its coefficient lines); 1: z-runs, then column
+// groups, shot groups outermost; 2: column groups fastest (the blocks that
+// share halo columns run side by side and meet in L2), then z-runs, then
+// shot groups.
+__device__ __forceinline__ void lanes_block(const FdArgs &a, int b, int &zr, int &sg, int &xg)
+{
+    if (a.lz_order == 2) {
+        xg = b % a.lz_nxg;
+        b /= a.lz_nxg;
+        zr = b % a.lz_nrun;
+        sg = b / a.lz_nrun;
+        return;
+    }
+    zr = b % a.lz_nrun;
+    b /= a.lz_nrun;
+    sg = a.lz_order ? b / a.lz_nxg : b % a.lz_ngrp;
+    xg = a.lz_order ? b % a.lz_nxg : b / a.lz_ngrp;
+}
+
 // Shots in lanes, no LDS: the 6 x-neighbours of a wave's cell are loaded from
 // global memory (the columns of the neighbouring waves, mostly L1/L2 hits).
 // The eight loads of the next PD rows are issued before a row is computed, so
@@ -612,13 +633,8 @@ __global__ __launch_bounds__(512) void fd_lanes_kernel(FdArgs a)
         b = (b & 7) * per + (b >> 3);
         if (b >= a.lz_nblk) return;
     }
-    const int zr = b % a.lz_nrun;
-    b /= a.lz_nrun;
-    // shot group outside the column groups (x-neighbouring blocks close in
-    // launch order, so their shared halo columns meet in L2) or inside them
-    // (the shot groups of one column group together, sharing coefficients)
-    const int sg = a.lz_sgslow ? b / a.lz_nxg : b % a.lz_ngrp;
-    const int xg = a.lz_sgslow ? b % a.lz_nxg : b / a.lz_ngrp;
+    int zr, sg, xg;
+    lanes_block(a, b, zr, sg, xg);
     const int ix = a.ixs + xg * wpb + w;
     if (ix > a.ixe) return;  // the whole wave
     const int iz0 = a.izs + zr * a.lz_run;
@@ -697,7 +713,7 @@ static void launch_lanes_t(const FdArgs &a0, FdTile tl, hipStream_t st)
     const long nb = (long)nxg * a.lz_nrun * a.lz_ngrp;
     a.lz_nblk = (int)nb;
     a.lz_nxg = nxg;
-    a.lz_sgslow = tl.zfast;
+    a.lz_order = tl.zfast;
     const long nlaunch = a.remap ? (nb + 7) / 8 * 8 : nb;
     const dim3 grid((unsigned)nlaunch), block(tl.wpb * 64);
     if (a.fma) {
@@ -739,13 +755,8 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
         b = (b & 7) * per + (b >> 3);
         if (b >= a.lz_nblk) return;  // the whole block
     }
-    const int zr = b % a.lz_nrun;
-    b /= a.lz_nrun;
-    // shot group outside the column groups (x-neighbouring blocks close in
-    // launch order, so their shared halo columns meet in L2) or inside them
-    // (the shot groups of one column group together, sharing coefficients)
-    const int sg = a.lz_sgslow ? b / a.lz_nxg : b % a.lz_ngrp;
-    const int xg = a.lz_sgslow ? b % a.lz_nxg : b / a.lz_ngrp;
+    int zr, sg, xg;
+    lanes_block(a, b, zr, sg, xg);
     const int x0 = a.ixs + xg * WPB;
     const int ix = x0 + w;
     const bool active = ix <= a.ixe;
@@ -861,7 +872,7 @@ static void launch_lanes_lds_t(const FdArgs &a0, FdTile tl, hipStream_t st)
     const long nb = (long)nxg * a.lz_nrun * a.lz_ngrp;
     a.lz_nblk = (int)nb;
     a.lz_nxg = nxg;
-    a.lz_sgslow = tl.zfast;
+    a.lz_order = tl.zfast;
     const dim3 grid((unsigned)(a.remap ? (nb + 7) / 8 * 8 : nb));
     if (tl.wpb >= 16)
         launch_lanes_lds_w<MODE, 16>(a, grid, tl.pd, st);

@@ -42,7 +42,7 @@ struct FdTile {
     int spb = 1;
     int kind = 0;
     int txm = 0;
-    int zfast = 0;  // kind 0: tile order; kinds 2, 3: 1 = shot groups outermost
+    int zfast = 0;  // kind 0: tile order; kinds 2, 3: block order 0, 1, 2 (see lanes_block)
     int pd = 1;  // columns prefetched ahead by the march (1, 2, 4)
     int remap = 0;  // XCD-aware block order
     int wpb = 4;    // kind 2: waves (= adjacent columns) per block
@@ -65,7 +65,7 @@ struct FdArgs {
     int zfast;              // tile order: z-neighbours adjacent (1) or x-neighbours (0)
     int mx_len, mx_nseg, mz_len, mz_nseg;  // column-march plan
     int lz_run, lz_nrun, lz_ngrp, lz_nblk; // shots-in-lanes plan
-    int lz_nxg, lz_sgslow;
+    int lz_nxg, lz_order;
     int fma;                // 1: FMA-contracted update arithmetic, 0: strict source order
     // source (FD_FWD / FD_RECON)
     const int *src_cell;    // [nshot] flat padded index or -1

@@ -147,12 +147,13 @@ void fwiedu_set_batch(int shots_per_launch);
  * (1, 2, 4, 8).  tz = 0 (default) lets the library's autotuner time every shape
  * once per geometry and keep the fastest. */
 void fwiedu_set_tile(int tz, int spb);
-/* Tile order of a forced tile shape: 1 puts vertically adjacent tiles next to
- * each other in launch order, 0 horizontally adjacent ones.  For a forced
- * shots-in-lanes shape (set before fwiedu_set_lanes): 1 runs all column groups
- * of one 64-shot group before the next group, 0 the shot groups of a column
- * group together. */
-void fwiedu_set_tile_order(int zfast);
+/* Block order of a forced launch shape.  Square tiles: 1 puts vertically
+ * adjacent tiles next to each other in launch order, 0 horizontally adjacent
+ * ones.  Shots in lanes (set before fwiedu_set_lanes): 0 z-runs fastest, then
+ * the 64-shot groups of a column group; 1 z-runs fastest, then column groups,
+ * shot groups outermost; 2 column groups fastest, then z-runs, shot groups
+ * outermost. */
+void fwiedu_set_tile_order(int order);
 /* Force the column-marching stencil: txm (16, 32, 64) columns per block, spb
  * (1, 2, 4) shots per block, pd (1, 2, 4) columns of loads kept in flight.
  * fwiedu_set_tile(0, 0) returns to the autotuner. */

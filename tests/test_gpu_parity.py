@@ -355,18 +355,19 @@ def test_lanes_layout_matches_arenas(L):
             out[name] = gradient(L, w, obs, True)
             assert flib.stat("layout") == layout, name
         L.fwiedu_set_batch(0)
-        for order in (0, 1):
-            # forced shapes: shot groups inside / outside the column groups
-            L.fwiedu_set_tile_order(order)
-            L.fwiedu_set_lanes(4, 8, 1, 0)
-            out[f"lanes_order{order}"] = gradient(L, w, obs, True)
+        for order in (0, 1, 2):
+            # forced shapes in every block order, without and with LDS
+            for lds in (0, 1):
+                L.fwiedu_set_tile_order(order)
+                L.fwiedu_set_lanes(4, 8, 2, lds)
+                out[f"lanes_order{order}_{lds}"] = gradient(L, w, obs, True)
     finally:
         L.fwiedu_set_layout(-1)
         L.fwiedu_set_batch(0)
         L.fwiedu_set_tile(0, 0)
         L.fwiedu_set_tile_order(0)
     pre_a, g_a = out["arenas"]
-    for name in ("lanes", "lanes33", "lanes_order0", "lanes_order1"):
+    for name in ["lanes", "lanes33"] + [f"lanes_order{o}_{l}" for o in (0, 1, 2) for l in (0, 1)]:
         pre_l, g_l = out[name]
         assert np.array_equal(g_l, g_a), name
         assert np.array_equal(recs(pre_l), recs(pre_a)), name
