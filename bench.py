@@ -11,6 +11,7 @@ fwiedu_session).  Prints one JSON line.
   python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--plain-shots 4] [--cpu-shots 4]
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -64,31 +65,72 @@ def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
         times.append(time.perf_counter() - t0)
         log(f"step {i + 1}/{steps}: {times[-1] * 1e3:.1f} ms, misfit {misfit:.6g}")
     L.fwiedu_set_kernel_timing(0)
-    stats = {k: flib.stat(k) for k in ("fd_launches", "fd_timed", "fd_ms_avg", "fd_cells_per_launch",
-                                       "fd_shots_per_launch", "batch", "device_bytes", "tile_tz",
-                                       "tile_spb", "tile_kind", "tile_zfast", "tile_remap",
-                                       "tile_txm", "tile_pd", "tile_wpb", "tile_zrun", "layout")}
+    names = ["fd_launches", "fd_timed", "fd_ms_avg", "fd_cells_per_launch",
+             "fd_shots_per_launch", "batch", "device_bytes", "tile_tz", "tile_spb", "tile_kind",
+             "tile_zfast", "tile_remap", "tile_txm", "tile_pd", "tile_wpb", "tile_zrun", "layout"]
+    names += [f"fd{m}_{k}" for m in range(3) for k in
+              ("timed", "ms_avg", "cells_per_launch", "shot_cells_per_launch",
+               "acc_shot_cells_per_launch")]
+    stats = {k: flib.stat(k) for k in names}
     L.fwiedu_session_destroy(sess)
     L.fwiedu_set_batch(0)
     return times, stats, misfit
 
 
-# HBM bytes of one reconstruction launch of the c1 workload at 120 shots in
-# lanes, per stencil shape (columns per block, rows per wave, rows in flight,
-# XCD order): FETCH_SIZE x 2 + WRITE_SIZE from rocprofv3 counter runs scaled by
-# a 512 MiB copy; the runs and the calibration are in
-# profiles/fd_lanes_recon_c1_traffic.md.
-C1_LANES_TRAFFIC = {(2, 8, 8, 1, 0): 447.5e6, (2, 8, 16, 1, 1): 345.5e6, (2, 4, 8, 1, 0): 518.8e6,
-                    (2, 8, 8, 2, 0): 447.3e6, (3, 8, 16, 2, 0): 413.9e6, (3, 8, 16, 2, 1): 352.4e6}
+# HBM bytes per launch of each stencil mode of the c1 gradient at 120 shots in
+# lanes (forward and adjoint: averaged over the gradient's launches, one in five
+# with its accumulation epilogue), per stencil shape (kernel kind 2 lanes /
+# 3 lanes with LDS row exchange, columns per block, rows per wave, rows in
+# flight, XCD order): FETCH_SIZE x 2 + WRITE_SIZE from rocprofv3 counter runs
+# scaled by a 512 MiB copy; the runs and the calibration are in
+# profiles/fd_lanes_c1_traffic.md.
+C1_LANES_TRAFFIC = {
+    "reconstruction": {(2, 8, 8, 1, 0): 447.5e6, (2, 8, 16, 1, 1): 345.5e6,
+                       (2, 4, 8, 1, 0): 518.8e6, (2, 8, 8, 2, 0): 447.3e6,
+                       (3, 8, 16, 2, 0): 413.9e6, (3, 8, 16, 2, 1): 352.4e6},
+    "forward": {(3, 8, 16, 2, 0): 469.1e6, (3, 8, 16, 2, 1): 409.7e6},
+    "adjoint": {(3, 8, 16, 2, 0): 483.4e6, (3, 8, 16, 2, 1): 424.7e6},
+}
 
 
-def measured_traffic(w, st):
+def measured_traffic(w, st, mode):
     kind = int(st["tile_kind"])
     if w.name != "c1" or w.nshot != 120 or kind not in (2, 3) or int(st["tile_zfast"]) != 0:
         return None
     key = (kind, int(st["tile_wpb"]), int(st["tile_zrun"]), int(st["tile_pd"]),
            int(st["tile_remap"]))
-    return C1_LANES_TRAFFIC.get(key)
+    return C1_LANES_TRAFFIC.get(mode, {}).get(key)
+
+
+MODE_TAGS = {"forward": "FD_FWD", "reconstruction": "FD_RECON", "adjoint": "FD_ADJ"}
+# algorithmic bytes of a launch's accumulation epilogue per shot-cell: the
+# pseudo-Hessian (FP64 read + write) of the forward, the imaging condition
+# (forward field read, FP64 gradient read + write) of the adjoint
+EPILOGUE_BYTES = {"forward": 16.0, "reconstruction": 0.0, "adjoint": 20.0}
+
+
+def stencil_modes(w, st):
+    """Per stencil mode, from the HIP-event samples of the timed gradients:
+    average launch time and algorithmic bytes per launch.  The stencil moves
+    12 B per shot-cell (p1, p0 read, p2 written) over the interior cells plus
+    the 7 coefficient streams of the reference formulation, 28 B per cell,
+    shared by the launch's shots; the forward and adjoint add their
+    accumulation epilogue over the nx x nz model cells on the launches where it
+    runs (every 5th step)."""
+    out = {}
+    total = sum(st[f"fd{m}_ms_avg"] * st[f"fd{m}_timed"] for m in range(3)) or 1.0
+    for m, name in enumerate(("forward", "reconstruction", "adjoint")):
+        n = int(st[f"fd{m}_timed"])
+        ms = st[f"fd{m}_ms_avg"]
+        cells = st[f"fd{m}_cells_per_launch"]
+        model_share = (w.nx * w.nz) / cells if cells > 0 else 0.0
+        b = (28.0 * cells + 12.0 * st[f"fd{m}_shot_cells_per_launch"] +
+             EPILOGUE_BYTES[name] * st[f"fd{m}_acc_shot_cells_per_launch"] * model_share)
+        ach = b / (ms * 1e-3) / 1e9 if ms > 0 else None
+        out[name] = {"avg_launch_us": ms * 1e3, "bytes_per_launch": b, "achieved": ach,
+                     "frac": ach / HBM_PEAK_GBS if ach else None, "launches_timed": n,
+                     "time_share": ms * n / total}
+    return out
 
 
 KERNELS = {0: "fd_kernel (square tiles, per-shot arenas)",
@@ -116,7 +158,6 @@ def stencil_shape(st):
 def copy_calibration(mb):
     """A plain device-to-device copy of known size, so that hardware byte
     counters (FETCH_SIZE, WRITE_SIZE) of the stencil can be scaled."""
-    import ctypes
     hip = ctypes.CDLL("libamdhip64.so")
     n = mb << 20
     a, b = ctypes.c_void_p(), ctypes.c_void_p()
@@ -136,9 +177,9 @@ def serving_leg(args):
     per dimension at 10 m, nt uniform in 1000-4000, 1-64 shots, submitted at
     their arrival times to a server with `--serving-slots` workers."""
     from fwiedu import serving
-    t0 = time.perf_counter()
     stream = serving.request_stream(args.serving, args.serving_rate, 303)
-    log(f"serving: {args.serving} requests built in {time.perf_counter() - t0:.1f} s")
+    log(f"serving: {args.serving} requests at {args.serving_rate}/s "
+        f"(each built shortly before its arrival)")
     srv = serving.Server(slots=args.serving_slots, mem_fraction=0.8, graphs=True)
     try:
         jobs, wall = serving.run_stream(srv, stream, log=log)
@@ -153,7 +194,7 @@ def serving_leg(args):
                "mean_queue_s": float(np.mean([j.queue for j in jobs])),
                "peak_concurrent_jobs": int(srv.stat("peak_concurrency")),
                "stream": "seed 303; grids 128-1024 x 128-1024 at 10 m, dt 0.8 ms, Ricker 10 Hz; "
-                         "nt 1000-4000; 1-64 shots; 30 % gradient jobs"}
+                         "nt 1000-4000; 1-64 shots; 30 % gradient jobs (observed traces zero)"}
     finally:
         srv.close()
     log(f"serving: p50 {out['p50_latency_s']:.3f} s, p99 {out['p99_latency_s']:.3f} s, "
@@ -161,9 +202,21 @@ def serving_leg(args):
     return out
 
 
-def cpu_baseline(w, nshots):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, nshots, shots_1t):
     """The FP32 oracle (reference algorithm and compiler flags, OpenMP over all
-    host threads) on `nshots` gradient shots of the same workload."""
+    host threads) on `nshots` gradient shots of the same workload, then on
+    `shots_1t` of them with one OpenMP thread."""
     from oracle import Oracle
     o = Oracle("f32")
     sub = synth.workload(w.name, nshot=nshots)
@@ -172,16 +225,33 @@ def cpu_baseline(w, nshots):
     o.propagation(vpt.pp, rht.pp, obs.recs, nshots, sub.nz, sub.nx, sub.dx, sub.dz, sub.dt, sub.nt,
                   sub.pml, sub.freesurface, 0.0, 2.0, 1, 0, obs.recs, None)
     vps, rhs = Array2D(sub.vp_start), Array2D(sub.rho_start)
-    pre = obs.header_copy()
-    g = Array2D(np.zeros((sub.nx, sub.nz), np.float32))
-    t0 = time.perf_counter()
-    o.propagation(vps.pp, rhs.pp, obs.recs, nshots, sub.nz, sub.nx, sub.dx, sub.dz, sub.dt, sub.nt,
-                  sub.pml, sub.freesurface, 0.0, 2.0, 1, 1, pre.recs, g.pp)
-    dt = time.perf_counter() - t0
+
+    def gradient(n):
+        pre = obs.header_copy()
+        g = Array2D(np.zeros((sub.nx, sub.nz), np.float32))
+        t0 = time.perf_counter()
+        o.propagation(vps.pp, rhs.pp, obs.recs, n, sub.nz, sub.nx, sub.dx, sub.dz, sub.dt, sub.nt,
+                      sub.pml, sub.freesurface, 0.0, 2.0, 1, 1, pre.recs, g.pp)
+        return time.perf_counter() - t0
+
+    dt = gradient(nshots)
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": nshots / dt, "unit": "gradient shots/s", "cores": cores, "kind": "port",
-            "sample": f"{nshots} of the 120 gradient shots of the same workload "
-                      f"({dt:.1f} s), oracle/liboracle_f32.so (-O3 -ffast-math, OpenMP)"}
+    out = {"value": nshots / dt, "unit": "gradient shots/s", "cores": cores, "kind": "port",
+           "sample": f"{nshots} of the {w.nshot} gradient shots of the same workload "
+                     f"({dt:.1f} s), oracle/liboracle_f32.so (-O3 -ffast-math, OpenMP)",
+           "cpu": cpu_model()}
+    if shots_1t > 0:
+        gomp = ctypes.CDLL("libgomp.so.1")
+        gomp.omp_set_num_threads(1)
+        try:
+            dt1 = gradient(min(shots_1t, nshots))
+        finally:
+            gomp.omp_set_num_threads(cores)
+        out["value_1_thread"] = min(shots_1t, nshots) / dt1
+        out["sample_1_thread"] = f"{min(shots_1t, nshots)} gradient shot(s), one OpenMP thread ({dt1:.1f} s)"
+    log(f"cpu baseline: {out['value']:.3f} shots/s on {cores} threads"
+        + (f", {out['value_1_thread']:.4f} on 1" if "value_1_thread" in out else ""))
+    return out
 
 
 def main():
@@ -198,6 +268,8 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="shots per launch (0 = all that fit)")
     ap.add_argument("--plain-shots", type=int, default=4)
     ap.add_argument("--cpu-shots", type=int, default=4)
+    ap.add_argument("--cpu-1thread-shots", type=int, default=1,
+                    help="gradient shots of the CPU baseline timed on one thread (0 = skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-host", action="store_true",
                     help="skip the host-array (PCIe-inclusive) iso_acoustic2d_propagation timing")
@@ -241,13 +313,8 @@ def main():
     shots_per_s = w.nshot / step_s
     cu = w.cell_updates_per_gradient_shot()
     S = int(st["fd_shots_per_launch"])
-    cells = interior_cells(w)
-    # algorithmic bytes of one launch: p1, p0 read and p2 written per shot-cell
-    # (12 B), plus the 7 coefficient streams of the reference formulation
-    # (28 B per cell) shared by the S shots of the launch
-    bytes_per_launch = cells * (12.0 * S + 28.0)
-    avg_ms = st["fd_ms_avg"]
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
+    modes = stencil_modes(w, st)
+    dom = max(modes, key=lambda m: modes[m]["time_share"])
 
     # the reference boundary as the FWI driver calls it: host models and host
     # records in, host adjoint sources and host gradient out (PCIe included)
@@ -280,7 +347,7 @@ def main():
 
     if args.copy_mb:
         copy_calibration(args.copy_mb)
-    cpu = None if args.no_cpu else cpu_baseline(w, args.cpu_shots)
+    cpu = None if args.no_cpu else cpu_baseline(w, args.cpu_shots, args.cpu_1thread_shots)
     if cpu:
         log(f"cpu baseline: {cpu['value']:.3f} shots/s on {cpu['cores']} threads")
 
@@ -311,16 +378,18 @@ def main():
         "batched_over_plain": (shots_per_s / plain["gradient_shots_per_s"]) if plain else None,
         "host_boundary": host,
         "serving": serve,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                     "traffic": measured_traffic(w, st),
-                     "traffic_source": "profiles/fd_lanes_recon_c1_traffic.md (counter runs of the "
-                                       "same stencil shape; null when the autotuner picked a shape "
-                                       "not measured there)",
-                     "kernel": KERNELS[int(st["tile_kind"])].split(" ")[0] +
-                               "<FD_RECON> (reconstruction sweep)",
-                     "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_ms * 1e3,
-                     "launches_timed": int(st["fd_timed"])},
+        "roofline": {"bound": "hbm", "achieved": modes[dom]["achieved"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": modes[dom]["frac"],
+                     "traffic": measured_traffic(w, st, dom),
+                     "traffic_source": "profiles/fd_lanes_c1_traffic.md (counter runs of the same "
+                                       "stencil shape; null when the autotuner picked a shape not "
+                                       "measured there)",
+                     "kernel": KERNELS[int(st["tile_kind"])].split(" ")[0] + f"<{MODE_TAGS[dom]}> "
+                               f"({dom}: the stencil mode with the largest share of kernel time)",
+                     "bytes_per_launch": modes[dom]["bytes_per_launch"],
+                     "avg_launch_us": modes[dom]["avg_launch_us"],
+                     "launches_timed": modes[dom]["launches_timed"]},
+        "stencil_modes": modes,
         "cpu_baseline": cpu,
         "device_bytes": st["device_bytes"],
     }

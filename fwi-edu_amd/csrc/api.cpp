@@ -473,6 +473,17 @@ extern "C" double fwiedu_stat(const char *name)
     if (n == "fd_cells_timed") return s.fd_cells_timed;
     if (n == "fd_cells_per_launch") return s.fd_timed ? s.fd_cells_timed / s.fd_timed : 0.0;
     if (n == "fd_shots_per_launch") return (double)s.shots_per_launch;
+    // per stencil mode m (0 forward, 1 reconstruction, 2 adjoint): "fd<m>_..."
+    if (n.size() > 4 && n[0] == 'f' && n[1] == 'd' && n[2] >= '0' && n[2] <= '2' && n[3] == '_') {
+        const int m = n[2] - '0';
+        const std::string k = n.substr(4);
+        const double t = (double)s.md_timed[m];
+        if (k == "timed") return t;
+        if (k == "ms_avg") return t ? s.md_ms[m] / t : 0.0;
+        if (k == "cells_per_launch") return t ? s.md_cells[m] / t : 0.0;
+        if (k == "shot_cells_per_launch") return t ? s.md_shot_cells[m] / t : 0.0;
+        if (k == "acc_shot_cells_per_launch") return t ? s.md_acc_shot_cells[m] / t : 0.0;
+    }
     if (n == "batch") return (double)s.batch;
     if (n == "tile_tz") return (double)s.tile_tz;
     if (n == "tile_spb") return (double)s.tile_spb;

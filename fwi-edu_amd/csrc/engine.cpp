@@ -185,9 +185,9 @@ void build_coefs(const Geom &g, const float *vpw, const float *rhow, float fpeak
 void fd_launch(int mode, const FdArgs &a, long cells, FdTile tile)
 {
     Ctx &c = ctx();
-    // sampled: the reconstruction sweep, the stencil with no accumulation epilogue
-    const bool timed = c.ktiming > 0 && !c.capturing && mode == FD_RECON &&
-                       (c.recon_launches++ % c.ktiming) == 0;
+    // every ktiming-th launch of each stencil mode, outside graph capture
+    const bool timed = c.ktiming > 0 && !c.capturing && mode >= 0 && mode < 3 &&
+                       (c.mode_launches[mode]++ % c.ktiming) == 0;
     if (timed) {
         if (c.ev_used + 2 > 4096) collect_timing();
         while (c.ev.size() < c.ev_used + 2) {
@@ -200,7 +200,7 @@ void fd_launch(int mode, const FdArgs &a, long cells, FdTile tile)
     launch_fd(mode, a, tile, c.st);
     if (timed) {
         HIPCHECK(hipEventRecord(c.ev[c.ev_used + 1], c.st));
-        c.ev_cells.push_back({(long)c.ev_used, cells});
+        c.ev_cells.push_back({(long)c.ev_used, mode, cells, (long)a.nshot, a.acc_on});
         c.ev_used += 2;
     }
     c.stats.fd_launches++;
@@ -213,12 +213,21 @@ void collect_timing()
     Ctx &c = ctx();
     if (c.ev_cells.empty()) return;
     sync();
-    for (auto &pc : c.ev_cells) {
+    Stats &s = c.stats;
+    for (const auto &t : c.ev_cells) {
         float ms = 0.f;
-        HIPCHECK(hipEventElapsedTime(&ms, c.ev[pc.first], c.ev[pc.first + 1]));
-        c.stats.fd_ms_timed += ms;
-        c.stats.fd_cells_timed += (double)pc.second;
-        c.stats.fd_timed++;
+        HIPCHECK(hipEventElapsedTime(&ms, c.ev[t.ev], c.ev[t.ev + 1]));
+        const double sc = (double)t.cells * (double)t.shots;
+        s.md_timed[t.mode]++;
+        s.md_ms[t.mode] += ms;
+        s.md_cells[t.mode] += (double)t.cells;
+        s.md_shot_cells[t.mode] += sc;
+        if (t.acc_on) s.md_acc_shot_cells[t.mode] += sc;
+        if (t.mode == FD_RECON) {
+            s.fd_ms_timed += ms;
+            s.fd_cells_timed += (double)t.cells;
+            s.fd_timed++;
+        }
     }
     c.ev_cells.clear();
     c.ev_used = 0;

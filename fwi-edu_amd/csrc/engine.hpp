@@ -39,9 +39,17 @@ struct DevBuf {
 struct Stats {
     long fd_launches = 0;
     long fd_cells = 0;        // interior cell-updates issued (all shots)
-    long fd_timed = 0;
+    long fd_timed = 0;            // sampled reconstruction launches (no accumulation epilogue)
     double fd_ms_timed = 0;
     double fd_cells_timed = 0;
+    // sampled launches per stencil mode (0 forward, 1 reconstruction, 2 adjoint):
+    // count, milliseconds, interior cells, interior cells x shots, and the
+    // interior cells x shots of the launches whose accumulation epilogue ran
+    long md_timed[3] = {0, 0, 0};
+    double md_ms[3] = {0, 0, 0};
+    double md_cells[3] = {0, 0, 0};
+    double md_shot_cells[3] = {0, 0, 0};
+    double md_acc_shot_cells[3] = {0, 0, 0};
     long batch = 0;
     long shots_per_launch = 0;
     double device_bytes = 0;
@@ -64,7 +72,7 @@ struct Ctx {
     int batch_req = 0;
     double memfrac = 0.85;
     int ktiming = 0;
-    long recon_launches = 0;
+    long mode_launches[3] = {0, 0, 0};
     int ckpt_bits = 32;
     FdTile tile_req = {0, 1};     // tz == 0: autotune
     int layout_req = -1;          // -1 automatic, 0 per-shot arenas, 1 shots in lanes
@@ -77,7 +85,14 @@ struct Ctx {
     Stats stats;
     std::vector<hipEvent_t> ev;   // pool of events for sampled kernel timing
     size_t ev_used = 0;
-    std::vector<std::pair<long, long>> ev_cells;  // (event pair index, cells)
+    struct TimedLaunch {
+        long ev;            // index of the launch's first event
+        int mode;
+        long cells;         // interior cells
+        long shots;
+        int acc_on;
+    };
+    std::vector<TimedLaunch> ev_cells;
     // device buffers
     DevBuf model[2];              // full vp, rho [nx][nz]
     DevBuf setup;                 // padded vp, rho, D, Beta + weights

@@ -114,17 +114,17 @@ def request_workload(i, seed, nz, nx, nt, nshot, dx=10.0):
 def request_stream(n, rate, seed, nz=(128, 1024), nx=(128, 1024), nt=(1000, 4000),
                    shots=(1, 64), p_gradient=0.3):
     """n requests with Poisson arrivals at `rate` per second: (arrival time,
-    problem, workload); sizes uniform in the given inclusive ranges."""
+    problem, workload parameters); sizes uniform in the given inclusive ranges.
+    Workloads are built when a request is submitted (request_workload(*p))."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, n))
     out = []
     for i in range(n):
         problem = 1 if rng.random() < p_gradient else 0
-        w = request_workload(i, seed * 1000 + i, int(rng.integers(nz[0], nz[1] + 1)),
-                             int(rng.integers(nx[0], nx[1] + 1)),
-                             int(rng.integers(nt[0], nt[1] + 1)),
-                             int(rng.integers(shots[0], shots[1] + 1)))
-        out.append((float(t[i] - t[0]), problem, w))
+        p = (i, seed * 1000 + i, int(rng.integers(nz[0], nz[1] + 1)),
+             int(rng.integers(nx[0], nx[1] + 1)), int(rng.integers(nt[0], nt[1] + 1)),
+             int(rng.integers(shots[0], shots[1] + 1)))
+        out.append((float(t[i] - t[0]), problem, p))
     return out
 
 
@@ -135,28 +135,61 @@ def observed_for(L, w):
     return obs
 
 
-def run_stream(server, stream, log=None):
-    """Submit every request at its arrival time from a submitter thread and
-    wait for all of them; returns the jobs and the wall time from the first
-    submission to the last completion."""
-    L = flib.load()
-    jobs = [Job(w, p, observed_for(L, w) if p == 1 else None) for _, p, w in stream]
+def load_observed(w):
+    """Observed records of a serving-load gradient request: all-zero traces, so
+    the adjoint source is the predicted records themselves.  The work of a
+    gradient does not depend on the values of the observed traces, and the
+    stream then needs no modeling run per request."""
+    return w.shots()
+
+
+class Done:
+    """What is kept of a completed request (its arrays are released)."""
+
+    def __init__(self, job):
+        self.problem, self.latency, self.queue = job.problem, job.latency, job.queue
+        self.misfit, self.nshot = job.misfit, job.w.nshot
+
+    def shots(self):
+        return self.nshot
+
+
+def run_stream(server, stream, log=None, ahead=8):
+    """Submit every request at its arrival time from a submitter thread (which
+    builds each request's workload and arrays shortly before its arrival, at
+    most `ahead` requests early) and wait for all of them, releasing each one's
+    arrays when it completes.  Returns the completed requests and the wall time
+    from the first submission to the last completion."""
+    n = len(stream)
+    slots = [None] * n
+    ready = threading.Semaphore(0)
+    built = threading.Semaphore(ahead)
     t0 = time.perf_counter()
 
     def submitter():
-        for (ta, _, _), j in zip(stream, jobs):
+        for k, (ta, problem, p) in enumerate(stream):
+            built.acquire()
+            w = request_workload(*p)
+            obs = load_observed(w) if problem == 1 else None
+            job = Job(w, problem, obs)
             dt = ta - (time.perf_counter() - t0)
             if dt > 0:
                 time.sleep(dt)
-            server.submit(j)
+            server.submit(job)
+            slots[k] = job
+            ready.release()
 
     th = threading.Thread(target=submitter)
     th.start()
-    for k, j in enumerate(jobs):
-        while j.id is None:
-            time.sleep(0.0005)
-        server.wait(j)
-        if log and (k + 1) % max(1, len(jobs) // 10) == 0:
-            log(f"serving: {k + 1}/{len(jobs)} done")
+    done = []
+    for k in range(n):
+        ready.acquire()
+        job = slots[k]
+        server.wait(job)
+        done.append(Done(job))
+        slots[k] = None
+        built.release()
+        if log and (k + 1) % max(1, n // 10) == 0:
+            log(f"serving: {k + 1}/{n} done")
     th.join()
-    return jobs, time.perf_counter() - t0
+    return done, time.perf_counter() - t0

@@ -188,8 +188,9 @@ void fwiedu_set_graphs(int on);
 void fwiedu_set_layout(int layout);
 /* Fraction of free device memory the engine may use (default 0.85). */
 void fwiedu_set_memory_fraction(double frac);
-/* Sample every `every`-th reconstruction-sweep launch (the stencil with no
- * accumulation epilogue) with HIP events on the library's stream (0 = off). */
+/* Sample every `every`-th launch of each stencil mode (forward,
+ * reconstruction sweep, adjoint) with HIP events on the library's stream
+ * (0 = off).  Launches replayed from a captured HIP graph are not sampled. */
 void fwiedu_set_kernel_timing(int every);
 /* Checkpoint storage precision of iso_acoustic2d_propagation and sessions:
  * 32 (default) or 16.  With 16, each checkpointed field u(kd-1), u(kd) of each
@@ -200,8 +201,12 @@ void fwiedu_set_kernel_timing(int every);
  * (the reference's FP32 structure) always stores FP32. */
 void fwiedu_set_checkpoint_bits(int bits);
 /* Named counters of the last calls: "fd_launches", "fd_timed", "fd_ms_avg",
- * "fd_cells_per_launch", "fd_shots_per_launch", "batch", "device_bytes" ...
- * Unknown names return -1. */
+ * "fd_cells_per_launch" (sampled reconstruction launches), "fd_shots_per_launch",
+ * "batch", "device_bytes" ...; per stencil mode m = 0 forward, 1 reconstruction,
+ * 2 adjoint: "fd<m>_timed", "fd<m>_ms_avg", "fd<m>_cells_per_launch",
+ * "fd<m>_shot_cells_per_launch", "fd<m>_acc_shot_cells_per_launch" (shot-cells
+ * of the sampled launches whose pseudo-Hessian or imaging epilogue ran, per
+ * sampled launch).  Unknown names return -1. */
 double fwiedu_stat(const char *name);
 void fwiedu_reset_stats(void);
 /* Device-resident gradient session: the model, the observed records and all
