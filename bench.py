@@ -71,6 +71,8 @@ def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
     names += [f"fd{m}_{k}" for m in range(3) for k in
               ("timed", "ms_avg", "cells_per_launch", "shot_cells_per_launch",
                "acc_shot_cells_per_launch")]
+    names += [f"tile{m}_{k}" for m in range(3) for k in
+              ("kind", "tz", "spb", "txm", "zfast", "pd", "remap", "wpb", "zrun")]
     stats = {k: flib.stat(k) for k in names}
     L.fwiedu_session_destroy(sess)
     L.fwiedu_set_batch(0)
@@ -94,14 +96,16 @@ C1_LANES_TRAFFIC = {
 
 
 def measured_traffic(w, st, mode):
-    kind = int(st["tile_kind"])
-    if w.name != "c1" or w.nshot != 120 or kind not in (2, 3) or int(st["tile_zfast"]) != 0:
+    m = MODE_INDEX[mode]
+    kind = int(st[f"tile{m}_kind"])
+    if w.name != "c1" or w.nshot != 120 or kind not in (2, 3) or int(st[f"tile{m}_zfast"]) != 0:
         return None
-    key = (kind, int(st["tile_wpb"]), int(st["tile_zrun"]), int(st["tile_pd"]),
-           int(st["tile_remap"]))
+    key = (kind, int(st[f"tile{m}_wpb"]), int(st[f"tile{m}_zrun"]), int(st[f"tile{m}_pd"]),
+           int(st[f"tile{m}_remap"]))
     return C1_LANES_TRAFFIC.get(mode, {}).get(key)
 
 
+MODE_INDEX = {"forward": 0, "reconstruction": 1, "adjoint": 2}
 MODE_TAGS = {"forward": "FD_FWD", "reconstruction": "FD_RECON", "adjoint": "FD_ADJ"}
 # algorithmic bytes of a launch's accumulation epilogue per shot-cell: the
 # pseudo-Hessian (FP64 read + write) of the forward, the imaging condition
@@ -139,19 +143,26 @@ KERNELS = {0: "fd_kernel (square tiles, per-shot arenas)",
            3: "fd_lanes_lds_kernel (shots in lanes, LDS row exchange)"}
 
 
-def stencil_shape(st):
-    k = int(st["tile_kind"])
-    d = {"kernel": KERNELS[k], "layout": "shots in lanes" if st["layout"] else "per-shot arenas",
-         "xcd_order": int(st["tile_remap"])}
+def tile_shape(st, m):
+    k = int(st[f"tile{m}_kind"])
+    d = {"kernel": KERNELS[k], "xcd_order": int(st[f"tile{m}_remap"])}
     if k == 0:
-        d.update(rows=int(st["tile_tz"]), shots_per_block=int(st["tile_spb"]),
-                 z_fast_order=int(st["tile_zfast"]))
+        d.update(rows=int(st[f"tile{m}_tz"]), shots_per_block=int(st[f"tile{m}_spb"]),
+                 z_fast_order=int(st[f"tile{m}_zfast"]))
     elif k == 1:
-        d.update(cols=int(st["tile_txm"]), shots_per_block=int(st["tile_spb"]),
-                 prefetch=int(st["tile_pd"]))
+        d.update(cols=int(st[f"tile{m}_txm"]), shots_per_block=int(st[f"tile{m}_spb"]),
+                 prefetch=int(st[f"tile{m}_pd"]))
     else:
-        d.update(cols_per_block=int(st["tile_wpb"]), rows_per_wave=int(st["tile_zrun"]),
-                 prefetch_rows=int(st["tile_pd"]), block_order=int(st["tile_zfast"]))
+        d.update(cols_per_block=int(st[f"tile{m}_wpb"]), rows_per_wave=int(st[f"tile{m}_zrun"]),
+                 prefetch_rows=int(st[f"tile{m}_pd"]), block_order=int(st[f"tile{m}_zfast"]))
+    return d
+
+
+def stencil_shape(st):
+    """The autotuned launch shape of each stencil mode of the last batch."""
+    d = {"layout": "shots in lanes" if st["layout"] else "per-shot arenas"}
+    for name, m in MODE_INDEX.items():
+        d[name] = tile_shape(st, m)
     return d
 
 
@@ -384,7 +395,8 @@ def main():
                      "traffic_source": "profiles/fd_lanes_c1_traffic.md (counter runs of the same "
                                        "stencil shape; null when the autotuner picked a shape not "
                                        "measured there)",
-                     "kernel": KERNELS[int(st["tile_kind"])].split(" ")[0] + f"<{MODE_TAGS[dom]}> "
+                     "kernel": KERNELS[int(st[f"tile{MODE_INDEX[dom]}_kind"])].split(" ")[0] +
+                               f"<{MODE_TAGS[dom]}> "
                                f"({dom}: the stencil mode with the largest share of kernel time)",
                      "bytes_per_launch": modes[dom]["bytes_per_launch"],
                      "avg_launch_us": modes[dom]["avg_launch_us"],

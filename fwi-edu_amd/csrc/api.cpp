@@ -473,6 +473,20 @@ extern "C" double fwiedu_stat(const char *name)
     if (n == "fd_cells_timed") return s.fd_cells_timed;
     if (n == "fd_cells_per_launch") return s.fd_timed ? s.fd_cells_timed / s.fd_timed : 0.0;
     if (n == "fd_shots_per_launch") return (double)s.shots_per_launch;
+    // launch shape per stencil mode m: "tile<m>_kind", "tile<m>_wpb", ...
+    if (n.size() > 6 && n.compare(0, 4, "tile") == 0 && n[4] >= '0' && n[4] <= '2' && n[5] == '_') {
+        const FdTile &t = s.tiles[n[4] - '0'];
+        const std::string k = n.substr(6);
+        if (k == "kind") return t.kind;
+        if (k == "tz") return t.tz;
+        if (k == "spb") return t.spb;
+        if (k == "txm") return t.txm;
+        if (k == "zfast") return t.zfast;
+        if (k == "pd") return t.pd;
+        if (k == "remap") return t.remap;
+        if (k == "wpb") return t.wpb;
+        if (k == "zrun") return t.zrun;
+    }
     // per stencil mode m (0 forward, 1 reconstruction, 2 adjoint): "fd<m>_..."
     if (n.size() > 4 && n[0] == 'f' && n[1] == 'd' && n[2] >= '0' && n[2] <= '2' && n[3] == '_') {
         const int m = n[2] - '0';

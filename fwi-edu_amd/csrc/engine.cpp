@@ -217,10 +217,11 @@ void collect_timing()
     for (const auto &t : c.ev_cells) {
         float ms = 0.f;
         HIPCHECK(hipEventElapsedTime(&ms, c.ev[t.ev], c.ev[t.ev + 1]));
-        const double sc = (double)t.cells * (double)t.shots;
+        // t.cells counts interior cell-updates of all the launch's shots
+        const double sc = (double)t.cells;
         s.md_timed[t.mode]++;
         s.md_ms[t.mode] += ms;
-        s.md_cells[t.mode] += (double)t.cells;
+        s.md_cells[t.mode] += t.shots > 0 ? sc / (double)t.shots : 0.0;
         s.md_shot_cells[t.mode] += sc;
         if (t.acc_on) s.md_acc_shot_cells[t.mode] += sc;
         if (t.mode == FD_RECON) {
@@ -234,14 +235,18 @@ void collect_timing()
 }
 
 // ---------------------------------------------------------------------------
-// Launch-shape autotuner.  For each new (geometry, shots-per-launch, layout)
-// it times the launch shapes of the layout (per-shot arenas: tile height,
-// shots per block, block order, column march; shots in lanes: columns per
-// block and rows per wave) on the reconstruction sweep over three distinct zero-filled work fields of the batch
-// (the output is zero again, so nothing the passes read changes), and keeps
-// the fastest.
+// Launch-shape autotuner.  For each new (stencil mode, geometry,
+// shots-per-launch, layout) it times the launch shapes of the layout
+// (per-shot arenas: tile height, shots per block, block order, column march;
+// shots in lanes: kernel, columns per block, rows per wave, rows of loads in
+// flight, block order) on that mode's stencil over three distinct zero-filled
+// work fields of the batch, with no source and no accumulation epilogue (the
+// output is zero again, so nothing the passes read changes), and keeps the
+// fastest.  The modes are tuned apart because their epilogues change the
+// register budget: deeper prefetch pays in the reconstruction sweep and not
+// in the forward and adjoint stencils.
 
-FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const float *prev)
+FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur, const float *prev)
 {
     Ctx &c = ctx();
     const bool lanes = base.cell_stride > 1;
@@ -256,7 +261,8 @@ FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const flo
         return c.tile_req;
     }
     const auto key = std::make_tuple(base.Nx, base.ld, base.ixs, base.ixe, base.izs, base.ize,
-                                     base.fs, base.nshot * 2 + base.fma, base.cell_stride);
+                                     base.fs, (base.nshot * 2 + base.fma) * 4 + mode,
+                                     base.cell_stride);
     auto it = c.tuned.find(key);
     if (it != c.tuned.end()) return it->second;
     FdArgs a = base;
@@ -266,6 +272,7 @@ FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const flo
     a.src_cell = nullptr;
     a.acc = nullptr;
     a.acc_on = 0;
+    a.u = nullptr;
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
@@ -288,18 +295,18 @@ FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const flo
         for (int wpb : {8, 16})
             for (int zrun : {16, 32})
                 for (int rm : {0, 1})
-                    for (int order : {0, 2}) {
-                        FdTile t;
-                        t.tz = 64;
-                        t.kind = 3;
-                        t.zfast = order;
-                        const int pd = 2;
-                        t.wpb = wpb;
-                        t.zrun = zrun;
-                        t.remap = rm;
-                        t.pd = pd;
-                        cand.push_back(t);
-                    }
+                    for (int order : {0, 2})
+                        for (int pd : {2, 4}) {
+                            FdTile t;
+                            t.tz = 64;
+                            t.kind = 3;
+                            t.zfast = order;
+                            t.wpb = wpb;
+                            t.zrun = zrun;
+                            t.remap = rm;
+                            t.pd = pd;
+                            cand.push_back(t);
+                        }
     }
     for (int tz : {64, 32})
         for (int spb : {1, 2, 4, 8})
@@ -330,9 +337,9 @@ FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const flo
     float best_ms = 1e30f;
     for (const FdTile &t : cand) {
         {
-            launch_fd(FD_RECON, a, t, c.st);
+            launch_fd(mode, a, t, c.st);
             HIPCHECK(hipEventRecord(e0, c.st));
-            for (int r = 0; r < 8; ++r) launch_fd(FD_RECON, a, t, c.st);
+            for (int r = 0; r < 8; ++r) launch_fd(mode, a, t, c.st);
             HIPCHECK(hipEventRecord(e1, c.st));
             HIPCHECK(hipEventSynchronize(e1));
             float ms = 0.f;
@@ -655,7 +662,8 @@ struct BatchState {
     double *acc = nullptr;     // [S][nx*nz]
     long nacc = 0;
     long cells_per_launch = 0;
-    FdTile tile;
+    FdTile tile_m[3];          // launch shape per stencil mode (FD_FWD, FD_RECON, FD_ADJ)
+    FdTile tile;               // the reconstruction sweep's (reported)
     float *F(int k) const { return arena + (long)k * KS; }
 };
 
@@ -743,8 +751,12 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
         HIPCHECK(hipMemsetAsync(b.acc, 0, (size_t)b.Sp * b.nacc * sizeof(double), c.st));
     }
     b.cells_per_launch = interior_cells(g) * S;
-    b.tile = autotune_tile(base_args(g, coef, S, b.SS, b.CS), b.F(BatchPlan::A2),
-                           b.F(BatchPlan::A0), b.F(BatchPlan::A1));
+    for (int m = 0; m < 3; ++m) {
+        b.tile_m[m] = autotune_tile(m, base_args(g, coef, S, b.SS, b.CS), b.F(BatchPlan::A2),
+                                    b.F(BatchPlan::A0), b.F(BatchPlan::A1));
+        c.stats.tiles[m] = b.tile_m[m];
+    }
+    b.tile = b.tile_m[FD_RECON];
     c.stats.tile_tz = b.tile.tz;
     c.stats.tile_spb = b.tile.spb;
     c.stats.tile_kind = b.tile.kind;
@@ -800,7 +812,7 @@ static void forward_loop(BatchState &b, bool save_ckpt, bool precond)
         a.prev = prev;
         a.it = it;
         a.acc_on = (precond && it % MST == 0) ? 1 : 0;
-        fd_launch(FD_FWD, a, b.cells_per_launch, b.tile);
+        fd_launch(FD_FWD, a, b.cells_per_launch, b.tile_m[FD_FWD]);
         if (ck && b.ck16) ck_store(b, sc.ckslot[it + 1], out);
         prev = cur;
         cur = out;
@@ -875,7 +887,7 @@ static void adjoint_loop(BatchState &b)
                 fr.cur = fcur;
                 fr.prev = fprev;
                 fr.it = itt;
-                fd_launch(FD_RECON, fr, b.cells_per_launch, b.tile);
+                fd_launch(FD_RECON, fr, b.cells_per_launch, b.tile_m[FD_RECON]);
                 fprev = fcur;
                 fcur = fout;
                 stored[t] = 1;
@@ -900,7 +912,7 @@ static void adjoint_loop(BatchState &b)
             ck_load(b, sc.ckslot[it], b.F(BatchPlan::RU));
             ad.u = b.F(BatchPlan::RU);
         }
-        fd_launch(FD_ADJ, ad, b.cells_per_launch, b.tile);
+        fd_launch(FD_ADJ, ad, b.cells_per_launch, b.tile_m[FD_ADJ]);
         prev = cur;
         cur = out;
     }
@@ -941,10 +953,10 @@ static std::vector<long> batch_key(int pass, const BatchState &b, bool f1, bool 
                           (const void *)b.tb.inj.trace_off, (const void *)b.tb.inj.shot,
                           (const void *)b.adjfull})
         k.push_back((long)(uintptr_t)p);
-    const FdTile &t = b.tile;
-    for (long v : {(long)t.kind, (long)t.tz, (long)t.spb, (long)t.txm, (long)t.zfast, (long)t.pd,
-                   (long)t.remap, (long)t.wpb, (long)t.zrun})
-        k.push_back(v);
+    for (const FdTile &t : b.tile_m)
+        for (long v : {(long)t.kind, (long)t.tz, (long)t.spb, (long)t.txm, (long)t.zfast,
+                       (long)t.pd, (long)t.remap, (long)t.wpb, (long)t.zrun})
+            k.push_back(v);
     return k;
 }
 

@@ -55,7 +55,8 @@ struct Stats {
     double device_bytes = 0;
     double h2d_bytes = 0, d2h_bytes = 0;
     int tile_tz = 0, tile_spb = 0, tile_kind = 0, tile_txm = 0, tile_zfast = 0, tile_pd = 0, tile_remap = 0;
-    int tile_wpb = 0, tile_zrun = 0, layout = 0, ckpt_bits = 32;
+    int tile_wpb = 0, tile_zrun = 0, layout = 0, ckpt_bits = 32;   // the reconstruction sweep's
+    FdTile tiles[3];              // launch shape per stencil mode of the last batch
     long graphs_built = 0, graph_launches = 0;
 };
 
@@ -88,7 +89,7 @@ struct Ctx {
     struct TimedLaunch {
         long ev;            // index of the launch's first event
         int mode;
-        long cells;         // interior cells
+        long cells;         // interior cell-updates (all shots of the launch)
         long shots;
         int acc_on;
     };
@@ -137,7 +138,7 @@ void adjoint_source_device(const Record2D *r, float **vp_full, float freq, int f
 
 // Sampled timing of stencil launches.
 void fd_launch(int mode, const FdArgs &a, long cells, FdTile tile);
-FdTile autotune_tile(const FdArgs &base, float *out, const float *cur, const float *prev);
+FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur, const float *prev);
 void collect_timing();
 void drop_graphs();
 

@@ -801,38 +801,43 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
     };
 #pragma unroll
     for (int p = 0; p < PD; ++p) load_row(min(iz0 + p, iz1), p);
-    for (int iz = iz0; iz <= iz1; ++iz) {
-        const int buf = iz & 1;
-        const float q6 = pn[0], p0v = pp[0];
-        row[buf][3 + w][lane] = q3;
+    // PD rows per outer iteration, each with its own prefetch slot: slot p is
+    // consumed and then reloaded with the row PD further down, so no register
+    // is copied while its load is in flight and the wait before a row covers
+    // only that row's loads (a rotating slot array makes the compiler wait for
+    // every outstanding load at each row)
+#pragma unroll 1
+    for (int izb = iz0; izb <= iz1; izb += PD) {
 #pragma unroll
-        for (int k = 0; k < NH; ++k)
-            if (k < nh) row[buf][hj[k]][lane] = ph[0][k];
+        for (int p = 0; p < PD; ++p) {
+            const int iz = izb + p;
+            if (iz > iz1) break;
+            const int buf = iz & 1;
+            const float q6 = pn[p], p0v = pp[p];
+            row[buf][3 + w][lane] = q3;
 #pragma unroll
-        for (int p = 0; p + 1 < PD; ++p) {
-            pn[p] = pn[p + 1];
-            pp[p] = pp[p + 1];
-#pragma unroll
-            for (int k = 0; k < NH; ++k) ph[p][k] = ph[p + 1][k];
+            for (int k = 0; k < NH; ++k)
+                if (k < nh) row[buf][hj[k]][lane] = ph[p][k];
+            load_row(min(iz + PD, iz1), p);
+            __syncthreads();
+            if (active) {
+                const float xm1 = row[buf][2 + w][lane], xp1 = row[buf][4 + w][lane];
+                const float xm2 = row[buf][1 + w][lane], xp2 = row[buf][5 + w][lane];
+                const float xm3 = row[buf][0 + w][lane], xp3 = row[buf][6 + w][lane];
+                const long i = colb + iz;
+                const float k[6] = {cf[CO_C1 * fsz + i],  cf[CO_C2 * fsz + i],
+                                    cf[CO_A * fsz + i],   cf[CO_IRX * fsz + i],
+                                    cf[CO_IRZ * fsz + i], cf[CO_IR * fsz + i]};
+                lanes_update<MODE, FMA>(a, cf, fsz, colb, ix, iz, s, scell, P2, U, CS, q0, q1, q2,
+                                        q3, q4, q5, q6, xm1, xp1, xm2, xp2, xm3, xp3, p0v, k);
+            }
+            q0 = q1;
+            q1 = q2;
+            q2 = q3;
+            q3 = q4;
+            q4 = q5;
+            q5 = q6;
         }
-        load_row(min(iz + PD, iz1), PD - 1);
-        __syncthreads();
-        if (active) {
-            const float xm1 = row[buf][2 + w][lane], xp1 = row[buf][4 + w][lane];
-            const float xm2 = row[buf][1 + w][lane], xp2 = row[buf][5 + w][lane];
-            const float xm3 = row[buf][0 + w][lane], xp3 = row[buf][6 + w][lane];
-            const long i = colb + iz;
-            const float k[6] = {cf[CO_C1 * fsz + i],  cf[CO_C2 * fsz + i],  cf[CO_A * fsz + i],
-                                cf[CO_IRX * fsz + i], cf[CO_IRZ * fsz + i], cf[CO_IR * fsz + i]};
-            lanes_update<MODE, FMA>(a, cf, fsz, colb, ix, iz, s, scell, P2, U, CS, q0, q1, q2, q3, q4,
-                                    q5, q6, xm1, xp1, xm2, xp2, xm3, xp3, p0v, k);
-        }
-        q0 = q1;
-        q1 = q2;
-        q2 = q3;
-        q3 = q4;
-        q4 = q5;
-        q5 = q6;
     }
 }
 
