@@ -541,14 +541,19 @@ __device__ __forceinline__ void lput(frsrc f, long cell, long CS, unsigned voff,
 // the forward field of the imaging condition and s the shot of the lane.
 // Accumulators are [cell][shot] (acc_stride 1).  Arithmetic order and the
 // epilogue (source, free-surface mirror, accumulation) are those of fd_kernel.
-template <int MODE, bool FMA>
+// EPI selects the accumulation epilogue: 0 when a.acc_on says so, loading
+// the accumulator (and the adjoint's forward field) here; 1 never (a launch
+// with a.acc_on == 0); 2 always, with the cell's accumulator (accv) and, in
+// the adjoint, forward-field value (uv) loaded by the caller rows ahead, with
+// the rest of the row.
+template <int MODE, bool FMA, int EPI = 0>
 __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long fsz, long colb,
                                              int ix, int iz, unsigned s, int scell, frsrc P2,
                                              frsrc U, long CS,
                                              float q0, float q1, float q2, float q3, float q4,
                                              float q5, float q6, float xm1, float xp1, float xm2,
                                              float xp2, float xm3, float xp3, float p0v,
-                                             const float *k)
+                                             const float *k, double accv = 0.0, float uv = 0.f)
 {
     const long i = colb + iz;
     const float pc = q3;
@@ -576,16 +581,18 @@ __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long f
                               a.dt * a.invdx * a.invdz;
     }
     lput(P2, i, CS, s * 4u, stored);
-    if (MODE != FD_RECON && a.acc_on) {
+    if (MODE != FD_RECON && EPI != 1 && (EPI == 2 || a.acc_on)) {
         const int px = ix - a.pml, pz = iz - a.pml;
         if (px >= 0 && px < a.nx && pz >= 0 && pz < a.nz) {
             double *acc = a.acc + ((long)px * a.nz + pz) * a.acc_cell + s;
+            const double old = EPI == 2 ? accv : *acc;
             if (MODE == FD_FWD) {
                 const float qq = (stored - 2.0f * pc + p0v) * a.invdtdt;
-                *acc += (double)(qq * qq);
+                *acc = old + (double)(qq * qq);
             } else {
-                const float qq = (stored - 2.0f * pc + p0v) * a.invdtdt * lget(U, i, CS, s * 4u);
-                *acc += (double)qq;
+                const float u = EPI == 2 ? uv : lget(U, i, CS, s * 4u);
+                const float qq = (stored - 2.0f * pc + p0v) * a.invdtdt * u;
+                *acc = old + (double)qq;
             }
         }
     }
@@ -741,7 +748,7 @@ static void launch_lanes_t(const FdArgs &a0, FdTile tl, hipStream_t st)
 // of 7 * WPB.  The LDS row is double buffered: one barrier per row.  Waves
 // whose column lies past ixe still load and publish their column (an active
 // neighbour may need it) but compute nothing.
-template <int MODE, int WPB, int PD, bool FMA>
+template <int MODE, int WPB, int PD, bool FMA, bool ACC>
 __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
 {
     constexpr int NC = WPB + 6;                    // columns of the LDS row
@@ -790,8 +797,17 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
     float q0 = lget(P1, colb + iz0 - 3, CS, vo), q1 = lget(P1, colb + iz0 - 2, CS, vo);
     float q2 = lget(P1, colb + iz0 - 1, CS, vo), q3 = lget(P1, colb + iz0, CS, vo);
     float q4 = lget(P1, colb + iz0 + 1, CS, vo), q5 = lget(P1, colb + iz0 + 2, CS, vo);
-    // prefetched rows: own column z+3, p0 at z, halo columns at z
-    float pn[PD], pp[PD], ph[PD][NH];
+    // prefetched rows: own column z+3, p0 at z, halo columns at z; with ACC
+    // (the launches whose accumulation epilogue runs) also the cell's
+    // accumulator and (adjoint) forward-field value, loaded unconditionally
+    // at clamped in-range indices (the epilogue skips cells outside the
+    // model), so that the epilogue never waits for a load issued after the
+    // prefetched rows
+    float pn[PD], pp[PD], ph[PD][NH], pu[PD] = {};
+    double pa[PD] = {};
+    const double *accb = nullptr;
+    if (ACC)
+        accb = a.acc + (long)min(max(ix - a.pml, 0), a.nx - 1) * a.nz * a.acc_cell + s;
     auto load_row = [&](int iz, int p) {
         pn[p] = lget(P1, colb + iz + 3, CS, vo);
         pp[p] = lget(P0, colb + iz, CS, vo);
@@ -799,8 +815,20 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
         for (int k = 0; k < NH; ++k)
             if (k < nh) ph[p][k] = lget(P1, hcol[k] + iz, CS, vo);
     };
+    // issued after the row that used the slot's previous values is done, so
+    // that old and new value can share registers (no copy, hence no wait, at
+    // the loop's back edge)
+    auto load_acc = [&](int iz, int p) {
+        if (ACC) {
+            pa[p] = accb[(long)min(max(iz - a.pml, 0), a.nz - 1) * a.acc_cell];
+            if (MODE == FD_ADJ) pu[p] = lget(U, colb + iz, CS, vo);
+        }
+    };
 #pragma unroll
-    for (int p = 0; p < PD; ++p) load_row(min(iz0 + p, iz1), p);
+    for (int p = 0; p < PD; ++p) {
+        load_row(min(iz0 + p, iz1), p);
+        load_acc(min(iz0 + p, iz1), p);
+    }
     // PD rows per outer iteration, each with its own prefetch slot: slot p is
     // consumed and then reloaded with the row PD further down, so no register
     // is copied while its load is in flight and the wait before a row covers
@@ -810,8 +838,12 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
     for (int izb = iz0; izb <= iz1; izb += PD) {
 #pragma unroll
         for (int p = 0; p < PD; ++p) {
+            // rows past iz1 (the last group of a short run) go through the
+            // loads and barriers but compute nothing: no early exit, so the
+            // prefetched values never merge from two paths (a merge copies a
+            // register whose load is in flight and has to wait for it)
             const int iz = izb + p;
-            if (iz > iz1) break;
+            const bool live = iz <= iz1;
             const int buf = iz & 1;
             const float q6 = pn[p], p0v = pp[p];
             row[buf][3 + w][lane] = q3;
@@ -820,7 +852,7 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
                 if (k < nh) row[buf][hj[k]][lane] = ph[p][k];
             load_row(min(iz + PD, iz1), p);
             __syncthreads();
-            if (active) {
+            if (active && live) {
                 const float xm1 = row[buf][2 + w][lane], xp1 = row[buf][4 + w][lane];
                 const float xm2 = row[buf][1 + w][lane], xp2 = row[buf][5 + w][lane];
                 const float xm3 = row[buf][0 + w][lane], xp3 = row[buf][6 + w][lane];
@@ -828,9 +860,11 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
                 const float k[6] = {cf[CO_C1 * fsz + i],  cf[CO_C2 * fsz + i],
                                     cf[CO_A * fsz + i],   cf[CO_IRX * fsz + i],
                                     cf[CO_IRZ * fsz + i], cf[CO_IR * fsz + i]};
-                lanes_update<MODE, FMA>(a, cf, fsz, colb, ix, iz, s, scell, P2, U, CS, q0, q1, q2,
-                                        q3, q4, q5, q6, xm1, xp1, xm2, xp2, xm3, xp3, p0v, k);
+                lanes_update<MODE, FMA, ACC ? 2 : 1>(a, cf, fsz, colb, ix, iz, s, scell, P2, U,
+                                                     CS, q0, q1, q2, q3, q4, q5, q6, xm1, xp1,
+                                                     xm2, xp2, xm3, xp3, p0v, k, pa[p], pu[p]);
             }
+            load_acc(min(iz + PD, iz1), p);
             q0 = q1;
             q1 = q2;
             q2 = q3;
@@ -841,27 +875,36 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
     }
 }
 
+template <int MODE, int WPB, bool FMA, bool ACC>
+static void launch_lanes_lds_p(const FdArgs &a, const dim3 &grid, int pd, hipStream_t st)
+{
+    if (pd >= 4)
+        fd_lanes_lds_kernel<MODE, WPB, 4, FMA, ACC><<<grid, WPB * 64, 0, st>>>(a);
+    else if (pd == 3)
+        fd_lanes_lds_kernel<MODE, WPB, 3, FMA, ACC><<<grid, WPB * 64, 0, st>>>(a);
+    else if (pd == 2)
+        fd_lanes_lds_kernel<MODE, WPB, 2, FMA, ACC><<<grid, WPB * 64, 0, st>>>(a);
+    else
+        fd_lanes_lds_kernel<MODE, WPB, 1, FMA, ACC><<<grid, WPB * 64, 0, st>>>(a);
+}
+
+// The accumulation epilogue is a template choice: the launches without it
+// (the reconstruction sweep, four in five forward and adjoint steps) carry
+// none of its registers or loads.
 template <int MODE, int WPB>
 static void launch_lanes_lds_w(const FdArgs &a, const dim3 &grid, int pd, hipStream_t st)
 {
+    const bool acc = MODE != FD_RECON && a.acc_on;
     if (a.fma) {
-        if (pd >= 4)
-            fd_lanes_lds_kernel<MODE, WPB, 4, true><<<grid, WPB * 64, 0, st>>>(a);
-        else if (pd == 3)
-            fd_lanes_lds_kernel<MODE, WPB, 3, true><<<grid, WPB * 64, 0, st>>>(a);
-        else if (pd == 2)
-            fd_lanes_lds_kernel<MODE, WPB, 2, true><<<grid, WPB * 64, 0, st>>>(a);
+        if (acc)
+            launch_lanes_lds_p<MODE, WPB, true, MODE != FD_RECON>(a, grid, pd, st);
         else
-            fd_lanes_lds_kernel<MODE, WPB, 1, true><<<grid, WPB * 64, 0, st>>>(a);
+            launch_lanes_lds_p<MODE, WPB, true, false>(a, grid, pd, st);
     } else {
-        if (pd >= 4)
-            fd_lanes_lds_kernel<MODE, WPB, 4, false><<<grid, WPB * 64, 0, st>>>(a);
-        else if (pd == 3)
-            fd_lanes_lds_kernel<MODE, WPB, 3, false><<<grid, WPB * 64, 0, st>>>(a);
-        else if (pd == 2)
-            fd_lanes_lds_kernel<MODE, WPB, 2, false><<<grid, WPB * 64, 0, st>>>(a);
+        if (acc)
+            launch_lanes_lds_p<MODE, WPB, false, MODE != FD_RECON>(a, grid, pd, st);
         else
-            fd_lanes_lds_kernel<MODE, WPB, 1, false><<<grid, WPB * 64, 0, st>>>(a);
+            launch_lanes_lds_p<MODE, WPB, false, false>(a, grid, pd, st);
     }
 }
 
