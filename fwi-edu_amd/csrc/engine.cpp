@@ -192,14 +192,19 @@ void fd_launch(int mode, const FdArgs &a, long cells, FdTile tile)
         if (c.ev_used + 2 > 4096) collect_timing();
         while (c.ev.size() < c.ev_used + 2) {
             hipEvent_t e;
-            HIPCHECK(hipEventCreate(&e));
+            // timing only: no system-scope fence (cache writeback and
+            // invalidation) when the event completes, which would slow the
+            // following launches and add to the measured time
+            HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
             c.ev.push_back(e);
         }
-        HIPCHECK(hipEventRecord(c.ev[c.ev_used], c.st));
+        // the events ride in the launch's own dispatch: kernel time only
+        set_stencil_events(c.ev[c.ev_used], c.ev[c.ev_used + 1]);
     }
     launch_fd(mode, a, tile, c.st);
     if (timed) {
-        HIPCHECK(hipEventRecord(c.ev[c.ev_used + 1], c.st));
+        set_stencil_events(nullptr, nullptr);
+        HIPCHECK(take_stencil_launch_error());
         c.ev_cells.push_back({(long)c.ev_used, mode, cells, (long)a.nshot, a.acc_on});
         c.ev_used += 2;
     }
@@ -274,8 +279,8 @@ FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur,
     a.acc_on = 0;
     a.u = nullptr;
     hipEvent_t e0, e1;
-    HIPCHECK(hipEventCreate(&e0));
-    HIPCHECK(hipEventCreate(&e1));
+    HIPCHECK(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
+    HIPCHECK(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
     std::vector<FdTile> cand;
     if (lanes) {
         for (int wpb : {4, 8})
@@ -481,6 +486,54 @@ static Tables build_tables(const std::vector<ShotIn> &shots, const Geom &g, int 
         }
     }
     u_beg.push_back((int)u_off.size());
+    // Entries in (cell, shot) order: with shots in lanes the threads of one
+    // wave then sample or update consecutive shots of one cell (one coalesced
+    // segment) instead of cells one column apart.  Different (shot, cell)
+    // pairs touch different field values, so their order is free; the
+    // receivers of one pair keep their order.
+    {
+        std::vector<int> pe(nent);
+        for (int i = 0; i < nent; ++i) pe[i] = i;
+        std::stable_sort(pe.begin(), pe.end(), [&](int x, int y) {
+            return e_cell[x] != e_cell[y] ? e_cell[x] < e_cell[y] : e_shot[x] < e_shot[y];
+        });
+        auto perm = [&](auto &v) {
+            auto w = v;
+            for (int i = 0; i < nent; ++i) w[i] = v[pe[i]];
+            v.swap(w);
+        };
+        perm(e_shot);
+        perm(e_cell);
+        perm(e_sr);
+        perm(e_off);
+        const int n0 = (int)u_cell.size();
+        std::vector<int> pu(n0);
+        for (int i = 0; i < n0; ++i) pu[i] = i;
+        std::stable_sort(pu.begin(), pu.end(), [&](int x, int y) {
+            return u_cell[x] != u_cell[y] ? u_cell[x] < u_cell[y] : u_shot[x] < u_shot[y];
+        });
+        std::vector<int> ns(n0), nc(n0), nb;
+        std::vector<long> no, nf;
+        nb.reserve(n0 + 1);
+        no.reserve(u_off.size());
+        nf.reserve(u_full.size());
+        for (int i = 0; i < n0; ++i) {
+            const int j = pu[i];
+            ns[i] = u_shot[j];
+            nc[i] = u_cell[j];
+            nb.push_back((int)no.size());
+            for (int e = u_beg[j]; e < u_beg[j + 1]; ++e) {
+                no.push_back(u_off[e]);
+                nf.push_back(u_full[e]);
+            }
+        }
+        nb.push_back((int)no.size());
+        u_shot.swap(ns);
+        u_cell.swap(nc);
+        u_beg.swap(nb);
+        u_off.swap(no);
+        u_full.swap(nf);
+    }
     const int nu = (int)u_cell.size();
     // one device block: ints then longs
     const size_t ni = (size_t)3 * nent + 2 * nu + nu + 1;

@@ -6,6 +6,7 @@
 // of the same sequence of adds and multiplies as a strict C evaluation of the
 // reference.  The stencil is HBM-bound (about 1.3 flop per byte), so the fused
 // multiply-adds would buy nothing.
+#include <hip/hip_ext.h>
 #include <hip/hip_fp16.h>
 
 #include "kernels.hpp"
@@ -16,6 +17,39 @@
 #pragma clang fp contract(off)
 
 namespace fwiedu {
+
+// Stencil launches go through fd_go: with events set (set_stencil_events) the
+// launch carries them in its own dispatch (hipExtLaunchKernel), so their
+// elapsed time is the kernel's execution alone, as a kernel trace shows it.
+static thread_local hipEvent_t t_kev[2] = {nullptr, nullptr};
+static thread_local hipError_t t_kerr = hipSuccess;
+
+void set_stencil_events(hipEvent_t start, hipEvent_t stop)
+{
+    t_kev[0] = start;
+    t_kev[1] = stop;
+}
+
+hipError_t take_stencil_launch_error()
+{
+    const hipError_t e = t_kerr;
+    t_kerr = hipSuccess;
+    return e;
+}
+
+template <class K>
+static void fd_go(K kern, dim3 grid, dim3 block, size_t shm, hipStream_t st, const FdArgs &a)
+{
+    if (t_kev[0]) {
+        FdArgs arg = a;
+        void *args[] = {&arg};
+        const hipError_t e = hipExtLaunchKernel(reinterpret_cast<const void *>(kern), grid, block,
+                                                args, shm, st, t_kev[0], t_kev[1], 0);
+        if (e != hipSuccess) t_kerr = e;
+    } else {
+        kern<<<grid, block, shm, st>>>(a);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // The stencil step.
@@ -442,7 +476,7 @@ static void launch_march_t(const FdArgs &a0, int mx_len, hipStream_t st)
     a.ngroups = (a.nshot + SPB - 1) / SPB;
     const long nb = (long)a.mx_nseg * a.mz_nseg * a.ngroups;
     const size_t lds = (size_t)SPB * 8 * zb * sizeof(float);
-    fd_march_kernel<MODE, SPB, PD><<<dim3((unsigned)nb), zb, lds, st>>>(a);
+    fd_go(fd_march_kernel<MODE, SPB, PD>, dim3((unsigned)nb), dim3(zb), lds, st, a);
 }
 
 template <int MODE, int PD>
@@ -474,7 +508,7 @@ static void launch_fd_t(const FdArgs &a0, hipStream_t st)
     a.ztiles = (a.ize - ((a.izs >> 5) << 5) + 1 + TZ - 1) / TZ;
     a.ngroups = (a.nshot + SPB - 1) / SPB;
     const long nb = (long)a.xtiles * a.ztiles * a.ngroups;
-    fd_kernel<MODE, TZ, SPB><<<dim3((unsigned)nb), 256, 0, st>>>(a);
+    fd_go(fd_kernel<MODE, TZ, SPB>, dim3((unsigned)nb), dim3(256), 0, st, a);
 }
 
 template <int MODE, int TZ>
@@ -725,18 +759,18 @@ static void launch_lanes_t(const FdArgs &a0, FdTile tl, hipStream_t st)
     const dim3 grid((unsigned)nlaunch), block(tl.wpb * 64);
     if (a.fma) {
         if (tl.pd >= 3)
-            fd_lanes_kernel<MODE, 3, true><<<grid, block, 0, st>>>(a);
+            fd_go(fd_lanes_kernel<MODE, 3, true>, grid, block, 0, st, a);
         else if (tl.pd == 2)
-            fd_lanes_kernel<MODE, 2, true><<<grid, block, 0, st>>>(a);
+            fd_go(fd_lanes_kernel<MODE, 2, true>, grid, block, 0, st, a);
         else
-            fd_lanes_kernel<MODE, 1, true><<<grid, block, 0, st>>>(a);
+            fd_go(fd_lanes_kernel<MODE, 1, true>, grid, block, 0, st, a);
     } else {
         if (tl.pd >= 3)
-            fd_lanes_kernel<MODE, 3, false><<<grid, block, 0, st>>>(a);
+            fd_go(fd_lanes_kernel<MODE, 3, false>, grid, block, 0, st, a);
         else if (tl.pd == 2)
-            fd_lanes_kernel<MODE, 2, false><<<grid, block, 0, st>>>(a);
+            fd_go(fd_lanes_kernel<MODE, 2, false>, grid, block, 0, st, a);
         else
-            fd_lanes_kernel<MODE, 1, false><<<grid, block, 0, st>>>(a);
+            fd_go(fd_lanes_kernel<MODE, 1, false>, grid, block, 0, st, a);
     }
 }
 
@@ -879,13 +913,13 @@ template <int MODE, int WPB, bool FMA, bool ACC>
 static void launch_lanes_lds_p(const FdArgs &a, const dim3 &grid, int pd, hipStream_t st)
 {
     if (pd >= 4)
-        fd_lanes_lds_kernel<MODE, WPB, 4, FMA, ACC><<<grid, WPB * 64, 0, st>>>(a);
+        fd_go(fd_lanes_lds_kernel<MODE, WPB, 4, FMA, ACC>, grid, dim3(WPB * 64), 0, st, a);
     else if (pd == 3)
-        fd_lanes_lds_kernel<MODE, WPB, 3, FMA, ACC><<<grid, WPB * 64, 0, st>>>(a);
+        fd_go(fd_lanes_lds_kernel<MODE, WPB, 3, FMA, ACC>, grid, dim3(WPB * 64), 0, st, a);
     else if (pd == 2)
-        fd_lanes_lds_kernel<MODE, WPB, 2, FMA, ACC><<<grid, WPB * 64, 0, st>>>(a);
+        fd_go(fd_lanes_lds_kernel<MODE, WPB, 2, FMA, ACC>, grid, dim3(WPB * 64), 0, st, a);
     else
-        fd_lanes_lds_kernel<MODE, WPB, 1, FMA, ACC><<<grid, WPB * 64, 0, st>>>(a);
+        fd_go(fd_lanes_lds_kernel<MODE, WPB, 1, FMA, ACC>, grid, dim3(WPB * 64), 0, st, a);
 }
 
 // The accumulation epilogue is a template choice: the launches without it

@@ -80,6 +80,10 @@ struct FdArgs {
 };
 
 void launch_fd(int mode, const FdArgs &a, FdTile tile, hipStream_t st);
+// Events the next stencil launches of this thread carry in their dispatch
+// (start, stop; nullptr: none), and the error of the last such launch.
+void set_stencil_events(hipEvent_t start, hipEvent_t stop);
+hipError_t take_stencil_launch_error();
 int fd_tile_rows_padded(int tz, int izs, int ize);
 int march_lds_bytes(const FdArgs &a, FdTile tile);
 
