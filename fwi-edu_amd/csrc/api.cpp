@@ -514,16 +514,7 @@ extern "C" double fwiedu_stat(const char *name)
     if (n == "graph_launches") return (double)s.graph_launches;
     if (n == "h2d_bytes") return s.h2d_bytes;
     if (n == "d2h_bytes") return s.d2h_bytes;
-    if (n == "device_bytes") {
-        const Ctx &c = ctx();
-        double b = 0;
-        const DevBuf *all[] = {&c.model[0], &c.model[1], &c.setup, &c.coef,  &c.arena,
-                               &c.acc,      &c.rec,      &c.obs,   &c.src,   &c.tables,
-                               &c.taper,    &c.shot_out, &c.gsum,  &c.scratch, &c.misc[0],
-                               &c.misc[1],  &c.misc[2],  &c.misc[3]};
-        for (const DevBuf *d : all) b += (double)d->cap;
-        return b;
-    }
+    if (n == "device_bytes") return (double)held_bytes(ctx());
     return -1.0;
 }
 

@@ -73,7 +73,8 @@ static std::vector<DevBuf *> all_buffers(Ctx &c)
     return {&c.model[0], &c.model[1], &c.setup,   &c.coef,    &c.arena,   &c.acc,
             &c.rec,      &c.obs,      &c.src,     &c.tables,  &c.taper,   &c.shot_out,
             &c.gsum,     &c.scratch,  &c.misc[0], &c.misc[1], &c.misc[2], &c.misc[3],
-            &c.ck16,     &c.ckaux,    &c.fftspec, &c.fftwin,  &c.ffttmp,  &c.adjfull};
+            &c.ck16,     &c.ckaux,    &c.fftspec, &c.fftwin,  &c.ffttmp,  &c.adjfull,
+            &c.stage,    &c.ftables};
 }
 
 size_t held_bytes(const Ctx &c)
@@ -426,13 +427,26 @@ struct Tables {
     RcvTable rcv;
     InjTable inj;       // adjoint sources at the recording rate ([ntr][ns] per shot)
     InjTable inj_full;  // adjoint sources interpolated to every time step ([ntr][nt] per shot)
+    // Shots in lanes: the receivers the LDS stencil samples and injects itself
+    // (per column at most one row, in the stencil's interior, one receiver of
+    // a shot per cell), as time-major stage slots rcol[ix] * Sp + lane; the
+    // rest stays in the *_left tables of the separate kernels.
+    int nci = 0;                  // columns with such a receiver row
+    const int *rrow = nullptr;    // [Nx]
+    const int *rcol = nullptr;    // [Nx]
+    const long *rec_slot = nullptr;      // [nci*Sp] record trace offset or -1
+    const int *rec_sr = nullptr;         // [nci*Sp] sample rate of the slot's shot
+    const long *inj_slot = nullptr;      // [nci*Sp] adjoint-source trace offset or -1
+    const long *injfull_slot = nullptr;  // [nci*Sp] interpolated adjoint-source offset or -1
+    RcvTable rcv_left;
+    InjTable inj_left, inj_full_left;
 };
 
 // Receiver tables of the batch.  trace offsets index the batch record buffer
 // (shot-major, [ntr][ns] per shot).
 static Tables build_tables(const std::vector<ShotIn> &shots, const Geom &g, int nt,
                            std::vector<long> &roff, long &rec_total, std::vector<long> &foff,
-                           long &full_total)
+                           long &full_total, int Sp_lanes)
 {
     Ctx &c = ctx();
     const int S = (int)shots.size();
@@ -535,6 +549,43 @@ static Tables build_tables(const std::vector<ShotIn> &shots, const Geom &g, int 
         u_full.swap(nf);
     }
     const int nu = (int)u_cell.size();
+
+    // receivers the lanes stencil takes over (see Tables)
+    std::vector<int> rrow, rcol;
+    std::vector<char> e_fused(nent, 0), u_fused(nu, 0);
+    int nci = 0;
+    if (Sp_lanes > 0) {
+        std::vector<std::map<int, int>> rows(g.Nx);
+        for (int e = 0; e < nent; ++e) {
+            const int gx = e_cell[e] / g.ld, gz = e_cell[e] % g.ld;
+            if (gx >= g.ixs && gx <= g.ixe && gz >= g.izs && gz <= g.ize) rows[gx][gz]++;
+        }
+        rrow.assign(g.Nx, -1);
+        rcol.assign(g.Nx, -1);
+        for (int x = 0; x < g.Nx; ++x) {
+            int best = -1, cnt = 0;
+            for (auto &kv : rows[x])
+                if (kv.second > cnt) {
+                    cnt = kv.second;
+                    best = kv.first;
+                }
+            if (best >= 0) {
+                rrow[x] = best;
+                rcol[x] = nci++;
+            }
+        }
+        // one receiver per (shot, cell) in a slot: entries are in (cell, shot)
+        // order, so a repeat follows its first
+        for (int e = 0; e < nent; ++e) {
+            const int gx = e_cell[e] / g.ld, gz = e_cell[e] % g.ld;
+            const bool repeat = e > 0 && e_cell[e - 1] == e_cell[e] && e_shot[e - 1] == e_shot[e];
+            e_fused[e] = rrow[gx] == gz && !repeat;
+        }
+        for (int u = 0; u < nu; ++u) {
+            const int gx = u_cell[u] / g.ld, gz = u_cell[u] % g.ld;
+            u_fused[u] = rrow[gx] == gz && u_beg[u + 1] - u_beg[u] == 1;
+        }
+    }
     // one device block: ints then longs
     const size_t ni = (size_t)3 * nent + 2 * nu + nu + 1;
     const size_t nl = (size_t)nent + 2 * u_off.size();
@@ -568,6 +619,96 @@ static Tables build_tables(const std::vector<ShotIn> &shots, const Geom &g, int 
     t.inj.beg = ib + 3 * nent + 2 * nu;
     t.inj_full = t.inj;
     t.inj_full.trace_off = lb + nent + u_off.size();
+    t.rcv_left = t.rcv;
+    t.inj_left = t.inj;
+    t.inj_full_left = t.inj_full;
+    if (nci == 0) return t;
+
+    // the stencil's slots, and the left-over tables of the separate kernels
+    const long nslot = (long)nci * Sp_lanes;
+    std::vector<long> rslot(nslot, -1), islot(nslot, -1), fslot(nslot, -1);
+    std::vector<int> rsr(nslot, 1);
+    std::vector<int> le_shot, le_cell, le_sr;
+    std::vector<long> le_off;
+    for (int e = 0; e < nent; ++e) {
+        if (e_fused[e]) {
+            const long sl = (long)rcol[e_cell[e] / g.ld] * Sp_lanes + e_shot[e];
+            rslot[sl] = e_off[e];
+            rsr[sl] = e_sr[e];
+        } else {
+            le_shot.push_back(e_shot[e]);
+            le_cell.push_back(e_cell[e]);
+            le_sr.push_back(e_sr[e]);
+            le_off.push_back(e_off[e]);
+        }
+    }
+    std::vector<int> lu_shot, lu_cell, lu_beg;
+    std::vector<long> lu_off, lu_full;
+    for (int u = 0; u < nu; ++u) {
+        if (u_fused[u]) {
+            const long sl = (long)rcol[u_cell[u] / g.ld] * Sp_lanes + u_shot[u];
+            islot[sl] = u_off[u_beg[u]];
+            fslot[sl] = u_full[u_beg[u]];
+        } else {
+            lu_shot.push_back(u_shot[u]);
+            lu_cell.push_back(u_cell[u]);
+            lu_beg.push_back((int)lu_off.size());
+            for (int e = u_beg[u]; e < u_beg[u + 1]; ++e) {
+                lu_off.push_back(u_off[e]);
+                lu_full.push_back(u_full[e]);
+            }
+        }
+    }
+    lu_beg.push_back((int)lu_off.size());
+    const int nle = (int)le_cell.size(), nlu = (int)lu_cell.size();
+    // longs: 3 slot tables, left-over record offsets, injection offsets (both
+    // rates); ints: rrow, rcol, slot sample rates, left-over tables
+    const size_t fl = (size_t)3 * nslot + nle + 2 * lu_off.size();
+    const size_t fi = (size_t)2 * g.Nx + nslot + 3 * nle + 2 * nlu + nlu + 1;
+    char *fb = c.ftables.as<char>(fl * sizeof(long) + fi * sizeof(int) + 64);
+    long *fL = reinterpret_cast<long *>(fb);
+    int *fI = reinterpret_cast<int *>(fb + fl * sizeof(long));
+    std::vector<long> hl;
+    hl.reserve(fl);
+    hl.insert(hl.end(), rslot.begin(), rslot.end());
+    hl.insert(hl.end(), islot.begin(), islot.end());
+    hl.insert(hl.end(), fslot.begin(), fslot.end());
+    hl.insert(hl.end(), le_off.begin(), le_off.end());
+    hl.insert(hl.end(), lu_off.begin(), lu_off.end());
+    hl.insert(hl.end(), lu_full.begin(), lu_full.end());
+    std::vector<int> hi;
+    hi.reserve(fi);
+    hi.insert(hi.end(), rrow.begin(), rrow.end());
+    hi.insert(hi.end(), rcol.begin(), rcol.end());
+    hi.insert(hi.end(), rsr.begin(), rsr.end());
+    hi.insert(hi.end(), le_shot.begin(), le_shot.end());
+    hi.insert(hi.end(), le_cell.begin(), le_cell.end());
+    hi.insert(hi.end(), le_sr.begin(), le_sr.end());
+    hi.insert(hi.end(), lu_shot.begin(), lu_shot.end());
+    hi.insert(hi.end(), lu_cell.begin(), lu_cell.end());
+    hi.insert(hi.end(), lu_beg.begin(), lu_beg.end());
+    h2d(fL, hl.data(), hl.size() * sizeof(long));
+    h2d(fI, hi.data(), hi.size() * sizeof(int));
+    t.nci = nci;
+    t.rec_slot = fL;
+    t.inj_slot = fL + nslot;
+    t.injfull_slot = fL + 2 * nslot;
+    t.rrow = fI;
+    t.rcol = fI + g.Nx;
+    t.rec_sr = fI + 2 * g.Nx;
+    const int *li = fI + 2 * g.Nx + nslot;
+    t.rcv_left.n = nle;
+    t.rcv_left.trace_off = fL + 3 * nslot;
+    t.rcv_left.shot = li;
+    t.rcv_left.cell = li + nle;
+    t.rcv_left.sr = li + 2 * nle;
+    t.inj_left.n = nlu;
+    t.inj_left.trace_off = fL + 3 * nslot + nle;
+    t.inj_left.shot = li + 3 * nle;
+    t.inj_left.cell = li + 3 * nle + nlu;
+    t.inj_left.beg = li + 3 * nle + 2 * nlu;
+    t.inj_full_left = t.inj_left;
+    t.inj_full_left.trace_off = fL + 3 * nslot + nle + lu_off.size();
     return t;
 }
 
@@ -715,6 +856,8 @@ struct BatchState {
     double *acc = nullptr;     // [S][nx*nz]
     long nacc = 0;
     long cells_per_launch = 0;
+    float *stage = nullptr;    // [nt][nslot] receiver values of the lanes stencil
+    long nslot = 0;
     FdTile tile_m[3];          // launch shape per stencil mode (FD_FWD, FD_RECON, FD_ADJ)
     FdTile tile;               // the reconstruction sweep's (reported)
     float *F(int k) const { return arena + (long)k * KS; }
@@ -796,7 +939,12 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
 
     std::vector<ShotIn> si(S);
     for (int s = 0; s < S; ++s) si[s] = {recs[s], nullptr, s};
-    b.tb = build_tables(si, g, nt, b.roff, b.rec_total, b.foff, b.full_total);
+    b.tb = build_tables(si, g, nt, b.roff, b.rec_total, b.foff, b.full_total,
+                        lanes ? b.Sp : 0);
+    if (b.tb.nci > 0) {
+        b.nslot = (long)b.tb.nci * b.Sp;
+        b.stage = c.stage.as<float>((size_t)nt * b.nslot + 64);
+    }
     b.rec = c.rec.as<float>(b.rec_total + 64);
     HIPCHECK(hipMemsetAsync(b.rec, 0, b.rec_total * sizeof(float), c.st));
     if (adjoint) {
@@ -849,6 +997,16 @@ static void forward_loop(BatchState &b, bool save_ckpt, bool precond)
     a.acc = b.acc;
     a.acc_stride = b.ASS;
     a.acc_cell = b.ACS;
+    // the LDS stencil samples the receivers of its stage slots itself
+    const bool fused = b.tb.nci > 0 && b.tile_m[FD_FWD].kind == 3;
+    if (fused) {
+        a.rrow = b.tb.rrow;
+        a.rcol = b.tb.rcol;
+        a.stage = b.stage;
+        a.nslot = b.nslot;
+        a.rec_on = 1;
+    }
+    const RcvTable &rt = fused ? b.tb.rcv_left : b.tb.rcv;
     float *W[3] = {b.F(BatchPlan::A0), b.F(BatchPlan::A1), b.F(BatchPlan::A2)};
     const float *zero = b.F(BatchPlan::ZERO);
     const float *cur = zero, *prev = zero;
@@ -859,7 +1017,7 @@ static void forward_loop(BatchState &b, bool save_ckpt, bool precond)
             out = b.F(BatchPlan::CK0 + sc.ckslot[it + 1]);
         else
             out = pick_free(W, cur, prev);
-        launch_record(b.tb.rcv, cur, b.SS, b.CS, b.rec, it, c.st);
+        launch_record(rt, cur, b.SS, b.CS, b.rec, it, c.st);
         a.out = out;
         a.cur = cur;
         a.prev = prev;
@@ -870,6 +1028,8 @@ static void forward_loop(BatchState &b, bool save_ckpt, bool precond)
         prev = cur;
         cur = out;
     }
+    if (fused)
+        launch_stage_scatter(b.tb.rec_slot, b.tb.rec_sr, b.stage, b.rec, b.nslot, b.nt, c.st);
 }
 
 // The adjoint wavefield starts from zero (propagation.c:504-506); only the
@@ -919,6 +1079,18 @@ static void adjoint_loop(BatchState &b)
     ad.acc_stride = b.ASS;
     ad.acc_cell = b.ACS;
     const float *K = b.coef + CO_K * g.fsz;
+    // the LDS stencil of step it injects the adjoint sources of its stage
+    // slots for step it - 1 into its output; step nt - 1 and the left-over
+    // receivers keep the separate injection
+    const bool fused = b.tb.nci > 0 && b.tile_m[FD_ADJ].kind == 3;
+    if (fused) {
+        launch_stage_gather(b.adjfull ? b.tb.injfull_slot : b.tb.inj_slot,
+                            b.adjfull ? b.adjfull : b.rec, b.stage, b.nslot, nt, c.st);
+        ad.rrow = b.tb.rrow;
+        ad.rcol = b.tb.rcol;
+        ad.stage = b.stage;
+        ad.nslot = b.nslot;
+    }
     const float *cur = AW[0], *prev = AW[1];
     for (int it = nt - 1; it >= 0; --it) {
         if (!stored[it]) {
@@ -947,12 +1119,14 @@ static void adjoint_loop(BatchState &b)
                 where[t] = fout;
             }
         }
+        const bool left = fused && it < nt - 1;
         if (b.adjfull)
-            launch_inject(b.tb.inj_full, const_cast<float *>(cur), b.SS, b.CS, K, b.adjfull, it,
-                          g.dt, g.invdx, g.invdz, c.st);
+            launch_inject(left ? b.tb.inj_full_left : b.tb.inj_full, const_cast<float *>(cur),
+                          b.SS, b.CS, K, b.adjfull, it, g.dt, g.invdx, g.invdz, c.st);
         else
-            launch_inject(b.tb.inj, const_cast<float *>(cur), b.SS, b.CS, K, b.rec, it, g.dt,
-                          g.invdx, g.invdz, c.st);
+            launch_inject(left ? b.tb.inj_left : b.tb.inj, const_cast<float *>(cur), b.SS, b.CS,
+                          K, b.rec, it, g.dt, g.invdx, g.invdz, c.st);
+        ad.inj_on = (fused && it >= 1) ? 1 : 0;
         float *out = pick_free(AW, cur, prev);
         ad.out = out;
         ad.cur = cur;
@@ -997,14 +1171,17 @@ static std::vector<long> batch_key(int pass, const BatchState &b, bool f1, bool 
 {
     std::vector<long> k = {pass, (long)f1, (long)f2, (long)ctx().fma, (long)b.nt, (long)b.S, (long)b.Sp,
                            (long)b.lanes, (long)b.ck16, b.sc ? (long)b.sc->d : 0L,
-                           b.tb.rcv.n, b.tb.inj.n};
+                           b.tb.rcv.n, b.tb.inj.n, (long)b.tb.nci, b.nslot, b.tb.rcv_left.n,
+                           b.tb.inj_left.n};
     key_push_geom(k, b.g);
     for (const void *p : {(const void *)b.arena, (const void *)b.coef, (const void *)b.cell_dev,
                           (const void *)b.wav_dev, (const void *)b.rec, (const void *)b.acc,
                           (const void *)b.h16, (const void *)b.ckinv, (const void *)b.ckmx,
                           (const void *)b.tb.rcv.trace_off, (const void *)b.tb.rcv.shot,
                           (const void *)b.tb.inj.trace_off, (const void *)b.tb.inj.shot,
-                          (const void *)b.adjfull})
+                          (const void *)b.adjfull, (const void *)b.stage,
+                          (const void *)b.tb.rrow, (const void *)b.tb.rec_slot,
+                          (const void *)b.tb.rcv_left.trace_off, (const void *)b.tb.inj_left.beg})
         k.push_back((long)(uintptr_t)p);
     for (const FdTile &t : b.tile_m)
         for (long v : {(long)t.kind, (long)t.tz, (long)t.spb, (long)t.txm, (long)t.zfast,
@@ -1079,6 +1256,9 @@ static size_t per_shot_bytes(const Geom &g, const Schedule &sc, bool adjoint, bo
     size_t b = (size_t)fields_per_shot(sc, adjoint, ck16) * g.fsz * sizeof(float);
     if (adjoint && ck16) b += (size_t)sc.nck * g.fsz * sizeof(__half);
     b += (size_t)2 * rec_floats * sizeof(float) + (size_t)nt * sizeof(float);
+    // the lanes stencil's time-major receiver stage: at most one receiver row
+    // per column
+    b += (size_t)nt * g.Nx * sizeof(float);
     if (adjoint) b += (size_t)g.nx * g.nz * (sizeof(double) + 2 * sizeof(float));
     return b;
 }
@@ -1103,7 +1283,7 @@ static int choose_batch(int n, size_t per_shot, long fsz, bool &lanes)
             HIPCHECK(hipMemGetInfo(&fr, &tot));
             // buffers the engine already holds are reused
             size_t held = c.arena.cap + c.acc.cap + c.rec.cap + c.obs.cap + c.src.cap +
-                          c.misc[0].cap + c.misc[1].cap + c.ck16.cap;
+                          c.misc[0].cap + c.misc[1].cap + c.ck16.cap + c.stage.cap;
             budget = ((double)fr + (double)held) * c.memfrac;
         }
         Smax = std::max<long>(1, (long)(budget / (double)per_shot));

@@ -587,7 +587,8 @@ __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long f
                                              float q0, float q1, float q2, float q3, float q4,
                                              float q5, float q6, float xm1, float xp1, float xm2,
                                              float xp2, float xm3, float xp3, float p0v,
-                                             const float *k, double accv = 0.0, float uv = 0.f)
+                                             const float *k, double accv = 0.0, float uv = 0.f,
+                                             bool inj = false, float injv = 0.f)
 {
     const long i = colb + iz;
     const float pc = q3;
@@ -614,7 +615,16 @@ __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long f
         stored = stored + a.src_wav[(long)s * a.nt + a.it] * cf[CO_K * fsz + i] * a.dt *
                               a.dt * a.invdx * a.invdz;
     }
-    lput(P2, i, CS, s * 4u, stored);
+    if (MODE == FD_ADJ && inj) {
+        // the adjoint source of the next (earlier) step, injected where the
+        // separate injection would add it before that step's stencil
+        // (propagation.c:681); the imaging below sees the field without it,
+        // as it does there
+        lput(P2, i, CS, s * 4u,
+             stored - injv * cf[CO_K * fsz + i] * a.dt * a.dt * a.invdx * a.invdz);
+    } else {
+        lput(P2, i, CS, s * 4u, stored);
+    }
     if (MODE != FD_RECON && EPI != 1 && (EPI == 2 || a.acc_on)) {
         const int px = ix - a.pml, pz = iz - a.pml;
         if (px >= 0 && px < a.nx && pz >= 0 && pz < a.nz) {
@@ -828,6 +838,22 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
     }
     const int nh = w < 6 ? (6 - w + WPB - 1) / WPB : 0;
 
+    // receiver row of the column inside this run (wave-uniform), its stage
+    // slot, and (adjoint) the adjoint source value to inject there, loaded
+    // first so that waiting for it never waits for the row loads
+    int rr = -1;
+    long slot = 0;
+    float injv = 0.f;
+    if ((MODE == FD_FWD && a.rec_on) || (MODE == FD_ADJ && a.inj_on)) {
+        if (active) {
+            const int r = a.rrow[ix];
+            if (r >= iz0 && r <= iz1) {
+                rr = r;
+                slot = (long)a.rcol[ix] * CS + s;
+                if (MODE == FD_ADJ) injv = a.stage[(long)(a.it - 1) * a.nslot + slot];
+            }
+        }
+    }
     float q0 = lget(P1, colb + iz0 - 3, CS, vo), q1 = lget(P1, colb + iz0 - 2, CS, vo);
     float q2 = lget(P1, colb + iz0 - 1, CS, vo), q3 = lget(P1, colb + iz0, CS, vo);
     float q4 = lget(P1, colb + iz0 + 1, CS, vo), q5 = lget(P1, colb + iz0 + 2, CS, vo);
@@ -886,6 +912,9 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
                 if (k < nh) row[buf][hj[k]][lane] = ph[p][k];
             load_row(min(iz + PD, iz1), p);
             __syncthreads();
+            // the receiver sample of the forward: p1 at the receiver row
+            // (propagation.c:574), time-major in the stage
+            if (MODE == FD_FWD && iz == rr) a.stage[(long)a.it * a.nslot + slot] = q3;
             if (active && live) {
                 const float xm1 = row[buf][2 + w][lane], xp1 = row[buf][4 + w][lane];
                 const float xm2 = row[buf][1 + w][lane], xp2 = row[buf][5 + w][lane];
@@ -896,7 +925,8 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
                                     cf[CO_IRZ * fsz + i], cf[CO_IR * fsz + i]};
                 lanes_update<MODE, FMA, ACC ? 2 : 1>(a, cf, fsz, colb, ix, iz, s, scell, P2, U,
                                                      CS, q0, q1, q2, q3, q4, q5, q6, xm1, xp1,
-                                                     xm2, xp2, xm3, xp3, p0v, k, pa[p], pu[p]);
+                                                     xm2, xp2, xm3, xp3, p0v, k, pa[p], pu[p],
+                                                     MODE == FD_ADJ && iz == rr, injv);
             }
             load_acc(min(iz + PD, iz1), p);
             q0 = q1;
@@ -1184,6 +1214,75 @@ void launch_record(const RcvTable &t, const float *field0, long shot_stride, lon
 {
     if (t.n <= 0) return;
     record_kernel<<<(t.n + 255) / 256, 256, 0, st>>>(t, field0, shot_stride, cell_stride, rec, it);
+}
+
+// Time-major staging of receiver values for the lanes stencil: 64 x 64 tiles
+// (slots x time steps) transposed through LDS, so that both the trace-major
+// side (records, adjoint sources: consecutive samples of one trace) and the
+// time-major stage (consecutive slots of one step) are accessed in rows.
+__global__ __launch_bounds__(256) void stage_gather_kernel(const long *slot_off, const float *src,
+                                                           float *stage, long nslot, int nt)
+{
+    __shared__ float tile[64][65];
+    const long s0 = (long)blockIdx.x * 64;
+    const int t0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    // read: one trace per row, 64 consecutive samples
+    for (int j = ty; j < 64; j += 4) {
+        const long sl = s0 + j;
+        float v = 0.f;
+        if (sl < nslot && t0 + tx < nt) {
+            const long off = slot_off[sl];
+            if (off >= 0) v = src[off + t0 + tx];
+        }
+        tile[j][tx] = v;
+    }
+    __syncthreads();
+    // write: one step per row, 64 consecutive slots
+    for (int j = ty; j < 64; j += 4) {
+        const int t = t0 + j;
+        if (t < nt && s0 + tx < nslot) stage[(long)t * nslot + s0 + tx] = tile[tx][j];
+    }
+}
+
+void launch_stage_gather(const long *slot_off, const float *src, float *stage, long nslot, int nt,
+                         hipStream_t st)
+{
+    if (nslot <= 0 || nt <= 0) return;
+    const dim3 grid((unsigned)((nslot + 63) / 64), (unsigned)((nt + 63) / 64));
+    stage_gather_kernel<<<grid, 256, 0, st>>>(slot_off, src, stage, nslot, nt);
+}
+
+__global__ __launch_bounds__(256) void stage_scatter_kernel(const long *slot_off,
+                                                            const int *slot_sr, const float *stage,
+                                                            float *rec, long nslot, int nt)
+{
+    __shared__ float tile[64][65];
+    const long s0 = (long)blockIdx.x * 64;
+    const int t0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int j = ty; j < 64; j += 4) {
+        const int t = t0 + j;
+        tile[j][tx] = (t < nt && s0 + tx < nslot) ? stage[(long)t * nslot + s0 + tx] : 0.f;
+    }
+    __syncthreads();
+    for (int j = ty; j < 64; j += 4) {
+        const long sl = s0 + j;
+        const int t = t0 + tx;
+        if (sl >= nslot || t >= nt) continue;
+        const long off = slot_off[sl];
+        if (off < 0) continue;
+        const int sr = slot_sr[sl];
+        if (t % sr == 0) rec[off + t / sr] = tile[tx][j];
+    }
+}
+
+void launch_stage_scatter(const long *slot_off, const int *slot_sr, const float *stage, float *rec,
+                          long nslot, int nt, hipStream_t st)
+{
+    if (nslot <= 0 || nt <= 0) return;
+    const dim3 grid((unsigned)((nslot + 63) / 64), (unsigned)((nt + 63) / 64));
+    stage_scatter_kernel<<<grid, 256, 0, st>>>(slot_off, slot_sr, stage, rec, nslot, nt);
 }
 
 __global__ void inject_kernel(InjTable t, float *f0, long stride, long cstride, const float *K,

@@ -77,6 +77,16 @@ struct FdArgs {
     long acc_cell;          // doubles between cells
     const float *u;         // FD_ADJ: forward field at `it`, shot 0
     int acc_on;
+    // Receivers inside the shots-in-lanes LDS stencil (one receiver row per
+    // column): rrow[ix] is the row of column ix's receivers or -1, rcol[ix]
+    // its index among those columns; stage is time-major [nt][nslot] with
+    // slot = rcol[ix] * cell_stride + lane.
+    const int *rrow;
+    const int *rcol;
+    float *stage;
+    long nslot;
+    int rec_on;             // FD_FWD: stage[it][slot] = p1 at the receiver row
+    int inj_on;             // FD_ADJ: p2 at the receiver row -= stage[it-1][slot] * K dt^2/(dx dz)
 };
 
 void launch_fd(int mode, const FdArgs &a, FdTile tile, hipStream_t st);
@@ -111,6 +121,13 @@ struct RcvTable {
 };
 void launch_record(const RcvTable &t, const float *field0, long shot_stride, long cell_stride,
                    float *rec, int it, hipStream_t st);
+// stage[t][slot] = src[slot_off[slot] + t] (0 where slot_off < 0), t < nt
+void launch_stage_gather(const long *slot_off, const float *src, float *stage, long nslot, int nt,
+                         hipStream_t st);
+// rec[slot_off[slot] + t / sr[slot]] = stage[t][slot] for t % sr[slot] == 0
+// (slots with slot_off < 0 skipped), t < nt
+void launch_stage_scatter(const long *slot_off, const int *slot_sr, const float *stage, float *rec,
+                          long nslot, int nt, hipStream_t st);
 struct InjTable {
     int n;                  // unique (shot, cell) pairs
     const int *shot, *cell; // [n]

@@ -373,6 +373,72 @@ def test_lanes_layout_matches_arenas(L):
         assert np.array_equal(recs(pre_l), recs(pre_a)), name
 
 
+def irregular_shots(w, sample_rate):
+    """70 shots sharing one window (receivers in the first and last model
+    column) with irregular receivers: some shots miss columns 10-19, some have
+    a second receiver in the cells of columns 30-34, some a second receiver
+    row (depth 4 cells) on columns 40-49, one a third row on columns 60-62."""
+    from fwiedu.records import ShotSet
+    dx, dz = np.float32(w.dx), np.float32(w.dz)
+    sx = [np.float32(c) * dx for c in w.shot_columns()]
+    sz = [np.float32(w.shot_depth_cells) * dz] * w.nshot
+    gx, gz = [], []
+    for s in range(w.nshot):
+        cols = [c for c in range(w.nx) if not (s % 3 == 0 and 10 <= c < 20)]
+        deps = [1] * len(cols)
+        if s % 4 == 1:
+            cols += list(range(30, 35))
+            deps += [1] * 5
+        if s % 5 == 2:
+            cols += list(range(40, 50))
+            deps += [4] * 10
+        if s == 7:
+            cols += [60, 61, 62]
+            deps += [6] * 3
+        gx.append((np.asarray(cols, np.float32) * dx).astype(np.float32))
+        gz.append((np.asarray(deps, np.float32) * dz).astype(np.float32))
+    return ShotSet(sx, sz, gx, gz, w.src, w.nt, w.dt, w.dx, w.dz, w.nx, w.fpeak, sample_rate)
+
+
+@pytest.mark.parametrize("sr", [1, 2])
+def test_lanes_stencil_receivers_match_separate_kernels(L, sr):
+    """The shots-in-lanes LDS stencil samples the receivers of one row per
+    column and injects their adjoint sources itself (time-major stage); the
+    others (missing lanes, a second receiver of a shot in a cell, other rows)
+    stay with the separate kernels.  With irregular receivers, records,
+    adjoint sources and gradients are bit-identical to per-shot arenas, where
+    every receiver goes through the separate kernels; sampleRate 1 and 2 (the
+    interpolated adjoint sources)."""
+    w = synth.workload("tiny", nshot=70, freesurface=1, shot_depth_cells=1, nx=90)
+    say(f"case: lanes stencil receivers, irregular geometry, 70 shots, sampleRate {sr}")
+    out = {}
+    try:
+        for name, layout, lds in (("arenas", 0, 0), ("lds", 1, 1), ("lanes", 1, 0)):
+            L.fwiedu_set_layout(layout)
+            if layout:
+                L.fwiedu_set_lanes(4, 8, 2, lds)
+            obs = irregular_shots(w, sr)
+            vp, rho = Array2D(w.vp_true), Array2D(w.rho_true)
+            flib.propagate(L, w, vp, rho, obs, 0)
+            for s in range(w.nshot):
+                # a source-like perturbation, so the residual is not zero
+                tr = obs.trace(s)
+                tr *= np.float32(1.25)
+            pre, g = gradient(L, w, obs, True)
+            out[name] = (recs(obs), recs(pre), g)
+            assert flib.stat("layout") == layout, name
+    finally:
+        L.fwiedu_set_layout(-1)
+        L.fwiedu_set_tile(0, 0)
+    r_a, p_a, g_a = out["arenas"]
+    assert np.abs(g_a).max() > 0 and np.abs(r_a).max() > 0
+    for name in ("lds", "lanes"):
+        r_l, p_l, g_l = out[name]
+        assert np.array_equal(r_l, r_a), name
+        assert np.array_equal(p_l, p_a), name
+        assert np.array_equal(g_l, g_a), name
+
+
 def test_graph_replays_match_eager(L):
     """Time loops replayed from captured HIP graphs give the eager result bit
     for bit on every replay, with eager work and synchronisations between the
