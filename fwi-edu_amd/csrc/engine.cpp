@@ -74,7 +74,7 @@ static std::vector<DevBuf *> all_buffers(Ctx &c)
             &c.rec,      &c.obs,      &c.src,     &c.tables,  &c.taper,   &c.shot_out,
             &c.gsum,     &c.scratch,  &c.misc[0], &c.misc[1], &c.misc[2], &c.misc[3],
             &c.ck16,     &c.ckaux,    &c.fftspec, &c.fftwin,  &c.ffttmp,  &c.adjfull,
-            &c.stage,    &c.ftables};
+            &c.stage,    &c.ftables,  &c.accstage};
 }
 
 size_t held_bytes(const Ctx &c)
@@ -186,6 +186,8 @@ void build_coefs(const Geom &g, const float *vpw, const float *rhow, float fpeak
 void fd_launch(int mode, const FdArgs &a, long cells, FdTile tile)
 {
     Ctx &c = ctx();
+    if (tile.kind == 3 && mode != FD_RECON && a.acc_on && !a.accstage)
+        fail("<error>: internal: accumulating lanes stencil launch without its staging buffer");
     // every ktiming-th launch of each stencil mode, outside graph capture
     const bool timed = c.ktiming > 0 && !c.capturing && mode >= 0 && mode < 3 &&
                        (c.mode_launches[mode]++ % c.ktiming) == 0;
@@ -856,6 +858,8 @@ struct BatchState {
     double *acc = nullptr;     // [S][nx*nz]
     long nacc = 0;
     long cells_per_launch = 0;
+    float *accstage = nullptr; // [ACC_STAGES][nx*nz][Sp] staged accumulation terms (lanes)
+    long accstage_stride = 0;
     float *stage = nullptr;    // [nt][nslot] receiver values of the lanes stencil
     long nslot = 0;
     FdTile tile_m[3];          // launch shape per stencil mode (FD_FWD, FD_RECON, FD_ADJ)
@@ -950,6 +954,10 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
     if (adjoint) {
         b.acc = c.acc.as<double>((size_t)b.Sp * b.nacc);
         HIPCHECK(hipMemsetAsync(b.acc, 0, (size_t)b.Sp * b.nacc * sizeof(double), c.st));
+        if (lanes) {
+            b.accstage_stride = (long)b.Sp * b.nacc;
+            b.accstage = c.accstage.as<float>((size_t)ACC_STAGES * b.accstage_stride + 64);
+        }
     }
     b.cells_per_launch = interior_cells(g) * S;
     for (int m = 0; m < 3; ++m) {
@@ -983,6 +991,23 @@ static void ck_load(BatchState &b, int k, float *f)
                      b.lanes ? 1 : 0, b.ckinv + (long)k * b.Sp, ctx().st);
 }
 
+// Deferred accumulation of an accumulating launch: stage its term while a
+// staging slot is free and a later accumulation step follows, else add the
+// staged terms and its own.
+static void set_deferral(FdArgs &a, bool defer, int &staged, bool last)
+{
+    a.acc_stage = 0;
+    a.acc_n = 0;
+    if (!a.acc_on || !defer) return;
+    if (staged < ACC_STAGES && !last) {
+        a.acc_stage = 1;
+        a.acc_slot = staged++;
+    } else {
+        a.acc_n = staged;
+        staged = 0;
+    }
+}
+
 // propagation.c:539-608.  With save_ckpt the fields u(kd-1), u(kd) land in
 // their checkpoint slots; with precond the pseudo-Hessian accumulates in acc.
 // Kernel launches only, so a HIP graph can capture it.
@@ -1007,6 +1032,13 @@ static void forward_loop(BatchState &b, bool save_ckpt, bool precond)
         a.rec_on = 1;
     }
     const RcvTable &rt = fused ? b.tb.rcv_left : b.tb.rcv;
+    // the LDS stencil stages the pseudo-Hessian terms of ACC_STAGES steps and
+    // adds them with the next one's (one pass over the FP64 accumulator
+    // instead of ACC_STAGES + 1)
+    const bool defer = precond && b.accstage && b.tile_m[FD_FWD].kind == 3;
+    a.accstage = b.accstage;
+    a.accstage_stride = b.accstage_stride;
+    int staged = 0;
     float *W[3] = {b.F(BatchPlan::A0), b.F(BatchPlan::A1), b.F(BatchPlan::A2)};
     const float *zero = b.F(BatchPlan::ZERO);
     const float *cur = zero, *prev = zero;
@@ -1023,6 +1055,7 @@ static void forward_loop(BatchState &b, bool save_ckpt, bool precond)
         a.prev = prev;
         a.it = it;
         a.acc_on = (precond && it % MST == 0) ? 1 : 0;
+        set_deferral(a, defer, staged, it + MST > b.nt - 1);
         fd_launch(FD_FWD, a, b.cells_per_launch, b.tile_m[FD_FWD]);
         if (ck && b.ck16) ck_store(b, sc.ckslot[it + 1], out);
         prev = cur;
@@ -1091,6 +1124,11 @@ static void adjoint_loop(BatchState &b)
         ad.stage = b.stage;
         ad.nslot = b.nslot;
     }
+    // imaging terms staged as in the forward (the last imaging step is it = 0)
+    const bool defer = b.accstage && b.tile_m[FD_ADJ].kind == 3;
+    ad.accstage = b.accstage;
+    ad.accstage_stride = b.accstage_stride;
+    int staged = 0;
     const float *cur = AW[0], *prev = AW[1];
     for (int it = nt - 1; it >= 0; --it) {
         if (!stored[it]) {
@@ -1133,6 +1171,7 @@ static void adjoint_loop(BatchState &b)
         ad.prev = prev;
         ad.it = it;
         ad.acc_on = (it % MST == 0) ? 1 : 0;
+        set_deferral(ad, defer, staged, it < MST);
         ad.u = (it % MST == 0) ? where[it] : nullptr;
         if (ad.acc_on && !ad.u) {
             // an FP16 checkpoint time: restore the forward field for imaging
@@ -1180,6 +1219,7 @@ static std::vector<long> batch_key(int pass, const BatchState &b, bool f1, bool 
                           (const void *)b.tb.rcv.trace_off, (const void *)b.tb.rcv.shot,
                           (const void *)b.tb.inj.trace_off, (const void *)b.tb.inj.shot,
                           (const void *)b.adjfull, (const void *)b.stage,
+                          (const void *)b.accstage,
                           (const void *)b.tb.rrow, (const void *)b.tb.rec_slot,
                           (const void *)b.tb.rcv_left.trace_off, (const void *)b.tb.inj_left.beg})
         k.push_back((long)(uintptr_t)p);
@@ -1259,7 +1299,7 @@ static size_t per_shot_bytes(const Geom &g, const Schedule &sc, bool adjoint, bo
     // the lanes stencil's time-major receiver stage: at most one receiver row
     // per column
     b += (size_t)nt * g.Nx * sizeof(float);
-    if (adjoint) b += (size_t)g.nx * g.nz * (sizeof(double) + 2 * sizeof(float));
+    if (adjoint) b += (size_t)g.nx * g.nz * (sizeof(double) + (2 + ACC_STAGES) * sizeof(float));
     return b;
 }
 
@@ -1283,7 +1323,8 @@ static int choose_batch(int n, size_t per_shot, long fsz, bool &lanes)
             HIPCHECK(hipMemGetInfo(&fr, &tot));
             // buffers the engine already holds are reused
             size_t held = c.arena.cap + c.acc.cap + c.rec.cap + c.obs.cap + c.src.cap +
-                          c.misc[0].cap + c.misc[1].cap + c.ck16.cap + c.stage.cap;
+                          c.misc[0].cap + c.misc[1].cap + c.ck16.cap + c.stage.cap +
+                          c.accstage.cap;
             budget = ((double)fr + (double)held) * c.memfrac;
         }
         Smax = std::max<long>(1, (long)(budget / (double)per_shot));

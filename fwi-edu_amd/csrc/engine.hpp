@@ -113,6 +113,7 @@ struct Ctx {
     DevBuf fftspec, fftwin, ffttmp;   // trace filters
     DevBuf adjfull;               // adjoint sources interpolated to every time step
     DevBuf stage;                 // time-major receiver values of the lanes stencil
+    DevBuf accstage;              // staged accumulation terms of the lanes stencil
     DevBuf ftables;               // receiver-row maps and slot tables of the lanes stencil
     std::map<std::pair<int, int>, std::pair<void *, void *>> fft_plans;  // (n, batch) -> R2C, C2R
 };

@@ -577,9 +577,13 @@ __device__ __forceinline__ void lput(frsrc f, long cell, long CS, unsigned voff,
 // epilogue (source, free-surface mirror, accumulation) are those of fd_kernel.
 // EPI selects the accumulation epilogue: 0 when a.acc_on says so, loading
 // the accumulator (and the adjoint's forward field) here; 1 never (a launch
-// with a.acc_on == 0); 2 always, with the cell's accumulator (accv) and, in
-// the adjoint, forward-field value (uv) loaded by the caller rows ahead, with
-// the rest of the row.
+// with a.acc_on == 0); 2 always, with the cell's accumulator (accv), the
+// a.acc_n terms staged by earlier launches (tv[], oldest first) and, in the
+// adjoint, the forward-field value (uv) loaded by the caller rows ahead: the
+// accumulator becomes ((accv + t0) + t1 ...) + this step's term, the
+// reference's sequence of double additions; 3 always, writing this step's
+// term (a float: the reference adds its exact double image) to staging slot
+// a.acc_slot instead of the accumulator.
 template <int MODE, bool FMA, int EPI = 0>
 __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long fsz, long colb,
                                              int ix, int iz, unsigned s, int scell, frsrc P2,
@@ -588,7 +592,8 @@ __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long f
                                              float q5, float q6, float xm1, float xp1, float xm2,
                                              float xp2, float xm3, float xp3, float p0v,
                                              const float *k, double accv = 0.0, float uv = 0.f,
-                                             bool inj = false, float injv = 0.f)
+                                             bool inj = false, float injv = 0.f,
+                                             const float *tv = nullptr)
 {
     const long i = colb + iz;
     const float pc = q3;
@@ -625,18 +630,28 @@ __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long f
     } else {
         lput(P2, i, CS, s * 4u, stored);
     }
-    if (MODE != FD_RECON && EPI != 1 && (EPI == 2 || a.acc_on)) {
+    if (MODE != FD_RECON && EPI != 1 && (EPI >= 2 || a.acc_on)) {
         const int px = ix - a.pml, pz = iz - a.pml;
         if (px >= 0 && px < a.nx && pz >= 0 && pz < a.nz) {
-            double *acc = a.acc + ((long)px * a.nz + pz) * a.acc_cell + s;
-            const double old = EPI == 2 ? accv : *acc;
+            const long ai = ((long)px * a.nz + pz) * a.acc_cell + s;
+            float term;
             if (MODE == FD_FWD) {
                 const float qq = (stored - 2.0f * pc + p0v) * a.invdtdt;
-                *acc = old + (double)(qq * qq);
+                term = qq * qq;
             } else {
-                const float u = EPI == 2 ? uv : lget(U, i, CS, s * 4u);
-                const float qq = (stored - 2.0f * pc + p0v) * a.invdtdt * u;
-                *acc = old + (double)qq;
+                const float u = EPI >= 2 ? uv : lget(U, i, CS, s * 4u);
+                term = (stored - 2.0f * pc + p0v) * a.invdtdt * u;
+            }
+            if (EPI == 3) {
+                a.accstage[(long)a.acc_slot * a.accstage_stride + ai] = term;
+            } else {
+                double old = EPI == 2 ? accv : a.acc[ai];
+                if (EPI == 2) {
+#pragma unroll
+                    for (int j = 0; j < ACC_STAGES; ++j)
+                        if (j < a.acc_n) old = old + (double)tv[j];
+                }
+                a.acc[ai] = old + (double)term;
             }
         }
     }
@@ -792,7 +807,7 @@ static void launch_lanes_t(const FdArgs &a0, FdTile tl, hipStream_t st)
 // of 7 * WPB.  The LDS row is double buffered: one barrier per row.  Waves
 // whose column lies past ixe still load and publish their column (an active
 // neighbour may need it) but compute nothing.
-template <int MODE, int WPB, int PD, bool FMA, bool ACC>
+template <int MODE, int WPB, int PD, bool FMA, int ACC>
 __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
 {
     constexpr int NC = WPB + 6;                    // columns of the LDS row
@@ -858,16 +873,16 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
     float q2 = lget(P1, colb + iz0 - 1, CS, vo), q3 = lget(P1, colb + iz0, CS, vo);
     float q4 = lget(P1, colb + iz0 + 1, CS, vo), q5 = lget(P1, colb + iz0 + 2, CS, vo);
     // prefetched rows: own column z+3, p0 at z, halo columns at z; with ACC
-    // (the launches whose accumulation epilogue runs) also the cell's
-    // accumulator and (adjoint) forward-field value, loaded unconditionally
-    // at clamped in-range indices (the epilogue skips cells outside the
-    // model), so that the epilogue never waits for a load issued after the
-    // prefetched rows
-    float pn[PD], pp[PD], ph[PD][NH], pu[PD] = {};
+    // (the launches whose accumulation epilogue runs: 1 staging this step's
+    // term, 2 adding the staged terms and this step's to the accumulator)
+    // also the (adjoint) forward-field value and, with 2, the cell's
+    // accumulator and staged terms, loaded unconditionally at clamped
+    // in-range indices (the epilogue skips cells outside the model), so that
+    // the epilogue never waits for a load issued after the prefetched rows
+    float pn[PD], pp[PD], ph[PD][NH], pu[PD] = {}, pt[PD][ACC_STAGES] = {};
     double pa[PD] = {};
-    const double *accb = nullptr;
-    if (ACC)
-        accb = a.acc + (long)min(max(ix - a.pml, 0), a.nx - 1) * a.nz * a.acc_cell + s;
+    long acol = 0;
+    if (ACC) acol = (long)min(max(ix - a.pml, 0), a.nx - 1) * a.nz * a.acc_cell + s;
     auto load_row = [&](int iz, int p) {
         pn[p] = lget(P1, colb + iz + 3, CS, vo);
         pp[p] = lget(P0, colb + iz, CS, vo);
@@ -880,7 +895,13 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
     // the loop's back edge)
     auto load_acc = [&](int iz, int p) {
         if (ACC) {
-            pa[p] = accb[(long)min(max(iz - a.pml, 0), a.nz - 1) * a.acc_cell];
+            const long ai = acol + (long)min(max(iz - a.pml, 0), a.nz - 1) * a.acc_cell;
+            if (ACC == 2) {
+                pa[p] = a.acc[ai];
+#pragma unroll
+                for (int j = 0; j < ACC_STAGES; ++j)
+                    pt[p][j] = a.accstage[(long)j * a.accstage_stride + ai];
+            }
             if (MODE == FD_ADJ) pu[p] = lget(U, colb + iz, CS, vo);
         }
     };
@@ -923,10 +944,10 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
                 const float k[6] = {cf[CO_C1 * fsz + i],  cf[CO_C2 * fsz + i],
                                     cf[CO_A * fsz + i],   cf[CO_IRX * fsz + i],
                                     cf[CO_IRZ * fsz + i], cf[CO_IR * fsz + i]};
-                lanes_update<MODE, FMA, ACC ? 2 : 1>(a, cf, fsz, colb, ix, iz, s, scell, P2, U,
-                                                     CS, q0, q1, q2, q3, q4, q5, q6, xm1, xp1,
-                                                     xm2, xp2, xm3, xp3, p0v, k, pa[p], pu[p],
-                                                     MODE == FD_ADJ && iz == rr, injv);
+                lanes_update<MODE, FMA, ACC == 2 ? 2 : (ACC == 1 ? 3 : 1)>(
+                    a, cf, fsz, colb, ix, iz, s, scell, P2, U, CS, q0, q1, q2, q3, q4, q5, q6,
+                    xm1, xp1, xm2, xp2, xm3, xp3, p0v, k, pa[p], pu[p],
+                    MODE == FD_ADJ && iz == rr, injv, pt[p]);
             }
             load_acc(min(iz + PD, iz1), p);
             q0 = q1;
@@ -939,7 +960,7 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
     }
 }
 
-template <int MODE, int WPB, bool FMA, bool ACC>
+template <int MODE, int WPB, bool FMA, int ACC>
 static void launch_lanes_lds_p(const FdArgs &a, const dim3 &grid, int pd, hipStream_t st)
 {
     if (pd >= 4)
@@ -955,21 +976,24 @@ static void launch_lanes_lds_p(const FdArgs &a, const dim3 &grid, int pd, hipStr
 // The accumulation epilogue is a template choice: the launches without it
 // (the reconstruction sweep, four in five forward and adjoint steps) carry
 // none of its registers or loads.
+template <int MODE, int WPB, bool FMA>
+static void launch_lanes_lds_f(const FdArgs &a, const dim3 &grid, int pd, hipStream_t st)
+{
+    if (MODE == FD_RECON || !a.acc_on)
+        launch_lanes_lds_p<MODE, WPB, FMA, 0>(a, grid, pd, st);
+    else if (a.acc_stage)
+        launch_lanes_lds_p<MODE, WPB, FMA, MODE == FD_RECON ? 0 : 1>(a, grid, pd, st);
+    else
+        launch_lanes_lds_p<MODE, WPB, FMA, MODE == FD_RECON ? 0 : 2>(a, grid, pd, st);
+}
+
 template <int MODE, int WPB>
 static void launch_lanes_lds_w(const FdArgs &a, const dim3 &grid, int pd, hipStream_t st)
 {
-    const bool acc = MODE != FD_RECON && a.acc_on;
-    if (a.fma) {
-        if (acc)
-            launch_lanes_lds_p<MODE, WPB, true, MODE != FD_RECON>(a, grid, pd, st);
-        else
-            launch_lanes_lds_p<MODE, WPB, true, false>(a, grid, pd, st);
-    } else {
-        if (acc)
-            launch_lanes_lds_p<MODE, WPB, false, MODE != FD_RECON>(a, grid, pd, st);
-        else
-            launch_lanes_lds_p<MODE, WPB, false, false>(a, grid, pd, st);
-    }
+    if (a.fma)
+        launch_lanes_lds_f<MODE, WPB, true>(a, grid, pd, st);
+    else
+        launch_lanes_lds_f<MODE, WPB, false>(a, grid, pd, st);
 }
 
 template <int MODE>

@@ -14,6 +14,9 @@
 namespace fwiedu {
 
 enum FdMode { FD_FWD = 0, FD_RECON = 1, FD_ADJ = 2 };
+// accumulation steps staged (as float terms) before one pass over the FP64
+// accumulators adds them in order
+constexpr int ACC_STAGES = 3;
 
 // Geometry of one padded simulation window.
 struct Geom {
@@ -77,6 +80,14 @@ struct FdArgs {
     long acc_cell;          // doubles between cells
     const float *u;         // FD_ADJ: forward field at `it`, shot 0
     int acc_on;
+    // Deferred accumulation (shots-in-lanes LDS stencil): with acc_stage the
+    // launch writes its term (float) to staging slot acc_slot of accstage
+    // ([ACC_STAGES][nx*nz][lanes], slots accstage_stride floats apart);
+    // otherwise it adds the acc_n staged terms, oldest first, and its own to
+    // the accumulator.
+    float *accstage;
+    long accstage_stride;
+    int acc_stage, acc_slot, acc_n;
     // Receivers inside the shots-in-lanes LDS stencil (one receiver row per
     // column): rrow[ix] is the row of column ix's receivers or -1, rcol[ix]
     // its index among those columns; stage is time-major [nt][nslot] with
