@@ -89,10 +89,9 @@ def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
 C1_LANES_TRAFFIC = {
     "reconstruction": {(2, 8, 8, 1, 0): 447.5e6, (2, 8, 16, 1, 1): 345.5e6,
                        (2, 4, 8, 1, 0): 518.8e6, (2, 8, 8, 2, 0): 447.3e6,
-                       (3, 8, 16, 2, 0): 413.9e6, (3, 8, 16, 2, 1): 352.4e6,
                        (3, 8, 16, 4, 1): 355.1e6},
-    "forward": {(3, 8, 16, 2, 0): 469.1e6, (3, 8, 16, 2, 1): 409.7e6, (3, 8, 16, 4, 1): 413.8e6},
-    "adjoint": {(3, 8, 16, 2, 0): 483.4e6, (3, 8, 16, 2, 1): 424.7e6, (3, 8, 16, 4, 1): 433.7e6},
+    "forward": {(3, 8, 16, 4, 1): 393.3e6},
+    "adjoint": {(3, 8, 16, 4, 1): 413.4e6},
 }
 
 
