@@ -339,15 +339,17 @@ def test_every_stencil_shape_bit_identical(L, oracles, fs):
         L.fwiedu_set_xcd_remap(0)
 
 
-def test_lanes_layout_matches_arenas(L):
+@pytest.mark.parametrize("bits", [32, 16])
+def test_lanes_layout_matches_arenas(L, bits):
     """70 shots: two groups of 64 lanes, the second padded.  Records,
     per-shot-preconditioned gradients and the misfit of the shots-in-lanes
     layout are bit-identical to those of per-shot arenas, in one launch and
-    in batches of 33 (33 lanes of 64 used)."""
+    in batches of 33 (33 lanes of 64 used); FP32 and FP16 checkpoints."""
     w = synth.workload("tiny", nshot=70, freesurface=1, shot_depth_cells=1, nx=90)
-    say("case: lanes vs arenas tiny fs=1 70 shots")
+    say(f"case: lanes vs arenas tiny fs=1 70 shots, fp{bits} checkpoints")
     obs = observed(L, w, True)
     out = {}
+    L.fwiedu_set_checkpoint_bits(bits)
     try:
         for name, layout, batch in (("arenas", 0, 0), ("lanes", 1, 0), ("lanes33", 1, 33)):
             L.fwiedu_set_layout(layout)
@@ -366,6 +368,7 @@ def test_lanes_layout_matches_arenas(L):
         L.fwiedu_set_batch(0)
         L.fwiedu_set_tile(0, 0)
         L.fwiedu_set_tile_order(0)
+        L.fwiedu_set_checkpoint_bits(32)
     pre_a, g_a = out["arenas"]
     for name in ["lanes", "lanes33"] + [f"lanes_order{o}_{l}" for o in (0, 1, 2) for l in (0, 1)]:
         pre_l, g_l = out[name]
@@ -443,7 +446,8 @@ def test_graph_replays_match_eager(L):
     """Time loops replayed from captured HIP graphs give the eager result bit
     for bit on every replay, with eager work and synchronisations between the
     replays, for per-shot arenas (one shot per launch and 4 per launch) and for
-    shots in lanes, FP32 and FP16 checkpoints."""
+    shots in lanes (the LDS stencil: in-stencil receivers, staged
+    accumulation), FP32 and FP16 checkpoints."""
     w = synth.workload("tiny", nshot=4, freesurface=1, shot_depth_cells=1)
     w2 = synth.workload("tiny", nshot=2, freesurface=0)
     say("case: graph replays tiny fs=1 4 shots")
@@ -453,6 +457,10 @@ def test_graph_replays_match_eager(L):
         for batch, layout, bits in ((1, 0, 32), (0, 0, 32), (0, 1, 32), (0, 1, 16)):
             L.fwiedu_set_batch(batch)
             L.fwiedu_set_layout(layout)
+            if layout:
+                L.fwiedu_set_lanes(4, 8, 2, 1)
+            else:
+                L.fwiedu_set_tile(0, 0)
             L.fwiedu_set_checkpoint_bits(bits)
             L.fwiedu_set_graphs(0)
             pre_e, g_e = gradient(L, w, obs, True)
@@ -471,6 +479,7 @@ def test_graph_replays_match_eager(L):
         L.fwiedu_set_graphs(0)
         L.fwiedu_set_batch(0)
         L.fwiedu_set_layout(-1)
+        L.fwiedu_set_tile(0, 0)
         L.fwiedu_set_checkpoint_bits(32)
 
 
