@@ -89,9 +89,9 @@ def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
 C1_LANES_TRAFFIC = {
     "reconstruction": {(2, 8, 8, 1, 0): 447.5e6, (2, 8, 16, 1, 1): 345.5e6,
                        (2, 4, 8, 1, 0): 518.8e6, (2, 8, 8, 2, 0): 447.3e6,
-                       (3, 8, 16, 4, 1): 355.1e6},
-    "forward": {(3, 8, 16, 4, 1): 393.3e6},
-    "adjoint": {(3, 8, 16, 4, 1): 413.4e6},
+                       (3, 8, 16, 4, 1): 355.1e6, (3, 8, 16, 4, 0): 415.1e6},
+    "forward": {(3, 8, 16, 4, 1): 393.3e6, (3, 8, 16, 4, 0): 452.3e6},
+    "adjoint": {(3, 8, 16, 4, 1): 413.4e6, (3, 8, 16, 4, 0): 471.1e6},
 }
 
 
