@@ -291,13 +291,18 @@ def main():
     ap.add_argument("--serving-slots", type=int, default=4)
     ap.add_argument("--ckpt-bits", type=int, default=32, choices=(32, 16),
                     help="checkpoint storage: FP32 or FP16 with a per-snapshot scale")
+    ap.add_argument("--freq", type=float, default=0.0,
+                    help="band frequency of the adjoint-source filter (0 = no filter)")
+    ap.add_argument("--filter", type=int, default=0, choices=(0, 1),
+                    help="with --freq: 0 low-pass, 1 band-pass")
     ap.add_argument("--lanes", default="",
                     help="force the shots-in-lanes stencil shape WPB,ZRUN,PD,XCD[,LDS] "
                          "(default: autotuned)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(shots would shard across GPUs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported (the shots "
+              "of a gradient would be split across GPUs, one process each, with one FP64 "
+              "all-reduce of gradient and pseudo-Hessian per gradient)", file=sys.stderr)
         sys.exit(2)
     L = flib.load()
     if not L.fwiedu_device_ok():
@@ -312,6 +317,10 @@ def main():
     over = {"nshot": args.nshot} if args.nshot else {}
     if args.nt:
         over["nt"] = args.nt
+    if args.freq > 0:
+        # one band of the multiscale sequence: FFT filter of the adjoint sources
+        over["freq"] = args.freq
+        over["filter"] = args.filter
     w = synth.workload(args.workload, **over)
     log(f"workload {w.name}: {w.nz}x{w.nx} nt={w.nt} shots={w.nshot}")
     t0 = time.perf_counter()
@@ -378,7 +387,9 @@ def main():
         "config": {"workload": f"{w.name}: {w.nz}x{w.nx} @ {w.dx} m, nt {w.nt}, {w.nshot} shots x "
                                f"{len(w.receiver_columns())} receivers, pml {w.pml}, free surface "
                                f"{w.freesurface}, checkpointed adjoint gradient, precondition "
-                               f"{w.precondition}, model and records resident in HBM",
+                               f"{w.precondition}, model and records resident in HBM" +
+                               (f", adjoint sources {'low' if w.filter == 0 else 'band'}-pass "
+                                f"filtered at {w.freq:g} Hz" if w.freq > 0 else ""),
                    "shots_per_launch": S, "batch": int(st["batch"]),
                    "checkpoint_storage": f"fp{args.ckpt_bits}",
                    "stencil": stencil_shape(st)},
