@@ -303,6 +303,29 @@ def test_c1_subset_parity(L, oracles):
     assert abs(f_g - f_o) <= 1e-5 * f_o
 
 
+def test_c1_geometry_lanes_matches_oracle(L, oracles):
+    """The c1 grid (221 x 601, free surface, 601 receivers at depth 1) with 64
+    shots, nt 400: the shots-in-lanes LDS stencil with its autotuned shapes,
+    in-stencil receivers and staged accumulation, against the strict-order
+    oracle: records, adjoint sources and gradient bit for bit."""
+    w = synth.workload("c1", nshot=64, nt=400)
+    say(f"case: c1 geometry {w.nz}x{w.nx} nt={w.nt} shots=64, lanes vs strict oracle")
+    obs_o = observed(oracles["f32s"], w, False)
+    L.fwiedu_set_layout(1)
+    try:
+        obs_g = observed(L, w, True)
+        pre_g, g_g = gradient(L, w, obs_o, True)
+        assert flib.stat("layout") == 1
+        kinds = [flib.stat(f"tile{m}_kind") for m in range(3)]
+    finally:
+        L.fwiedu_set_layout(-1)
+    pre_o, g_o = gradient(oracles["f32s"], w, obs_o, False)
+    say("  stencil kinds", kinds)
+    assert np.array_equal(recs(obs_g), recs(obs_o))
+    assert np.array_equal(recs(pre_g), recs(pre_o))
+    assert np.array_equal(g_g, g_o)
+
+
 @pytest.mark.parametrize("fs", [0, 1])
 def test_every_stencil_shape_bit_identical(L, oracles, fs):
     """Every launch shape the autotuner may pick (square tiles of 32/64 rows in
