@@ -178,6 +178,7 @@ void build_coefs(const Geom &g, const float *vpw, const float *rhow, float fpeak
     launch_meal(vp, nullptr, D, Beta, g, nl, wdev, lnR, pd, sqrt3, fpeak, c.st);
     launch_coefs(vp, rho, D, Beta, coef, g, c.st);
     launch_grad_invrho(coef, g, c.st);
+    launch_coef_pack(coef, g, c.st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1465,7 +1466,7 @@ void propagate_core(const Call &P)
         const Record2D &r0 = P.obs[grp[0]];
         const int nxx = r0.ix_max - r0.ix_min + 1;
         Geom g = make_geom(nxx, nz, P.pml, P.fs, P.dx, P.dz, P.dt);
-        float *coef = c.coef.as<float>((size_t)CO_N * g.fsz);
+        float *coef = c.coef.as<float>((size_t)CO_ALLOC * g.fsz);
         const float *vpw = vpd + (long)r0.ix_min * nz, *rhow = rhod + (long)r0.ix_min * nz;
         build_coefs(g, vpw, rhow, r0.fpeak, coef);
         long recmax = 0;
@@ -1960,7 +1961,7 @@ extern "C" void iso_acoustic2d_propagation_engine(Record2D *record2d, Acoustic2d
     h2d(vpd, vpm[0], nmod * sizeof(float));
     h2d(rhod, rhom[0], nmod * sizeof(float));
     Geom g = make_geom(nx, nz, pmlThick, freesurface, dx, dz, dt);
-    float *coef = c.coef.as<float>((size_t)CO_N * g.fsz);
+    float *coef = c.coef.as<float>((size_t)CO_ALLOC * g.fsz);
     build_coefs(g, vpd, rhod, record2d->fpeak, coef);
     Schedule sc(nt);
     BatchState b;

@@ -584,7 +584,7 @@ __device__ __forceinline__ void lput(frsrc f, long cell, long CS, unsigned voff,
 // reference's sequence of double additions; 3 always, writing this step's
 // term (a float: the reference adds its exact double image) to staging slot
 // a.acc_slot instead of the accumulator.
-template <int MODE, bool FMA, int EPI = 0>
+template <int MODE, bool FMA, int EPI = 0, bool K8 = false>
 __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long fsz, long colb,
                                              int ix, int iz, unsigned s, int scell, frsrc P2,
                                              frsrc U, long CS,
@@ -604,7 +604,7 @@ __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long f
     float val = v;
     if (MODE == FD_FWD && (int)i == scell) {
         const double srcv = 1.0 * (double)a.src_wav[(long)s * a.nt + a.it] *
-                            (double)cf[CO_K * fsz + i] * (double)a.dt * (double)a.dt *
+                            (double)(K8 ? k[6] : cf[CO_K * fsz + i]) * (double)a.dt * (double)a.dt *
                             (double)a.invdx * (double)a.invdz;
         val = (float)((double)v + srcv);
     }
@@ -617,7 +617,7 @@ __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long f
             lput(P2, colb + (a.pml - kk), CS, s * 4u, -val);
     }
     if (MODE == FD_RECON && (int)i == scell) {
-        stored = stored + a.src_wav[(long)s * a.nt + a.it] * cf[CO_K * fsz + i] * a.dt *
+        stored = stored + a.src_wav[(long)s * a.nt + a.it] * (K8 ? k[6] : cf[CO_K * fsz + i]) * a.dt *
                               a.dt * a.invdx * a.invdz;
     }
     if (MODE == FD_ADJ && inj) {
@@ -626,7 +626,7 @@ __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long f
         // (propagation.c:681); the imaging below sees the field without it,
         // as it does there
         lput(P2, i, CS, s * 4u,
-             stored - injv * cf[CO_K * fsz + i] * a.dt * a.dt * a.invdx * a.invdz);
+             stored - injv * (K8 ? k[6] : cf[CO_K * fsz + i]) * a.dt * a.dt * a.invdx * a.invdz);
     } else {
         lput(P2, i, CS, s * 4u, stored);
     }
@@ -807,6 +807,9 @@ static void launch_lanes_t(const FdArgs &a0, FdTile tl, hipStream_t st)
 // of 7 * WPB.  The LDS row is double buffered: one barrier per row.  Waves
 // whose column lies past ixe still load and publish their column (an active
 // neighbour may need it) but compute nothing.
+typedef float float8v __attribute__((ext_vector_type(8)));
+typedef const __attribute__((address_space(4))) float8v cfloat8v;
+
 template <int MODE, int WPB, int PD, bool FMA, int ACC>
 __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
 {
@@ -833,6 +836,7 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
     const int ld = a.ld;
     const long fsz = (long)a.Nx * ld;
     cfloat *cf = (cfloat *)a.coef;
+    const cfloat8v *c8 = (const cfloat8v *)(a.coef + CO_N * fsz);
     const long fbytes = fsz * CS * 4;
     const frsrc P1 = field_rsrc(a.cur, fbytes), P0 = field_rsrc(a.prev, fbytes);
     const frsrc P2 = field_rsrc(a.out, fbytes);
@@ -941,10 +945,11 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
                 const float xm2 = row[buf][1 + w][lane], xp2 = row[buf][5 + w][lane];
                 const float xm3 = row[buf][0 + w][lane], xp3 = row[buf][6 + w][lane];
                 const long i = colb + iz;
-                const float k[6] = {cf[CO_C1 * fsz + i],  cf[CO_C2 * fsz + i],
-                                    cf[CO_A * fsz + i],   cf[CO_IRX * fsz + i],
-                                    cf[CO_IRZ * fsz + i], cf[CO_IR * fsz + i]};
-                lanes_update<MODE, FMA, ACC == 2 ? 2 : (ACC == 1 ? 3 : 1)>(
+                // the cell's coefficients in one scalar load (c1, c2, c3*K,
+                // rho^-1_x, rho^-1_z, rho^-1, K)
+                const float8v kv = c8[i];
+                const float k[7] = {kv[0], kv[1], kv[2], kv[3], kv[4], kv[5], kv[6]};
+                lanes_update<MODE, FMA, ACC == 2 ? 2 : (ACC == 1 ? 3 : 1), true>(
                     a, cf, fsz, colb, ix, iz, s, scell, P2, U, CS, q0, q1, q2, q3, q4, q5, q6,
                     xm1, xp1, xm2, xp2, xm3, xp3, p0v, k, pa[p], pu[p],
                     MODE == FD_ADJ && iz == rr, injv, pt[p]);
@@ -1218,6 +1223,24 @@ __global__ void grad_invrho_kernel(float *coef, int Nx, int Nz, int ld, long fsz
 void launch_grad_invrho(float *coef, const Geom &g, hipStream_t st)
 {
     grad_invrho_kernel<<<grid2(g, g.Nz), 64, 0, st>>>(coef, g.Nx, g.Nz, g.ld, g.fsz, g.dx, g.dz);
+}
+
+// The coefficients of a cell side by side, [fsz][8] after the CO_N arrays:
+// c1, c2, c3*K, rho^-1_x, rho^-1_z, rho^-1, K, 0 -- one 32-byte scalar load per
+// cell for the shots-in-lanes stencils instead of seven.
+__global__ void coef_pack_kernel(float *coef, long fsz)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fsz) return;
+    float *o = coef + CO_N * fsz + i * 8;
+#pragma unroll
+    for (int k = 0; k < CO_N; ++k) o[k] = coef[k * fsz + i];
+    o[7] = 0.f;
+}
+
+void launch_coef_pack(float *coef, const Geom &g, hipStream_t st)
+{
+    coef_pack_kernel<<<(unsigned)((g.fsz + 255) / 256), 256, 0, st>>>(coef, g.fsz);
 }
 
 // ---------------------------------------------------------------------------

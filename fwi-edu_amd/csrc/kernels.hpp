@@ -33,6 +33,9 @@ struct Geom {
 // the first six; K (bulk modulus) is read only at source and receiver cells.
 // CO_A holds coef3*K (propagation.c:71 evaluates that product first).
 enum { CO_C1 = 0, CO_C2, CO_A, CO_IRX, CO_IRZ, CO_IR, CO_K, CO_N };
+// floats per cell of a coefficient buffer: the CO_N arrays and their
+// cell-interleaved copy (8 per cell) that the shots-in-lanes stencils read
+constexpr int CO_ALLOC = CO_N + 8;
 constexpr int CO_STREAMED = 6;
 
 // Stencil launch shape.  kind 0: square tiles of tz (32 or 64) rows, spb shots
@@ -53,7 +56,7 @@ struct FdTile {
 };
 
 struct FdArgs {
-    const float *coef;      // CO_N * fsz
+    const float *coef;      // CO_ALLOC * fsz (see CO_ALLOC)
     float *out;             // shot 0 pointers
     const float *cur;
     const float *prev;
@@ -121,6 +124,9 @@ void launch_coefs(float *vp, float *rho, float *D, const float *Beta, float *coe
                   hipStream_t st);
 // d(1/rho)/dx, d(1/rho)/dz: 2nd-order everywhere, 12-point optimised inside
 void launch_grad_invrho(float *coef, const Geom &g, hipStream_t st);
+// the cell-interleaved copy [fsz][8] of the coefficients after the CO_N arrays
+// (CO_ALLOC * fsz floats in all)
+void launch_coef_pack(float *coef, const Geom &g, hipStream_t st);
 
 // ---- receivers -----------------------------------------------------------
 struct RcvTable {
