@@ -914,6 +914,14 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
         load_row(min(iz0 + p, iz1), p);
         load_acc(min(iz0 + p, iz1), p);
     }
+    // the cell's coefficients (c1, c2, c3*K, rho^-1_x, rho^-1_z, rho^-1, K),
+    // one 32-byte scalar load per row; with an even PD the next row's are
+    // loaded once this row's neighbours are read from LDS, so their latency
+    // passes during this row's arithmetic (a scalar load in flight would make
+    // every LDS wait a full wait, so none is issued before the LDS reads)
+    constexpr int CPD = (PD % 2 == 0) ? 2 : 1;
+    float8v kc[CPD];
+    if (CPD == 2) kc[0] = c8[colb + iz0];
     // PD rows per outer iteration, each with its own prefetch slot: slot p is
     // consumed and then reloaded with the row PD further down, so no register
     // is copied while its load is in flight and the wait before a row covers
@@ -944,10 +952,16 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
                 const float xm1 = row[buf][2 + w][lane], xp1 = row[buf][4 + w][lane];
                 const float xm2 = row[buf][1 + w][lane], xp2 = row[buf][5 + w][lane];
                 const float xm3 = row[buf][0 + w][lane], xp3 = row[buf][6 + w][lane];
-                const long i = colb + iz;
-                // the cell's coefficients in one scalar load (c1, c2, c3*K,
-                // rho^-1_x, rho^-1_z, rho^-1, K)
-                const float8v kv = c8[i];
+                float8v kv;
+                if (CPD == 2) {
+                    kv = kc[p & 1];
+                    // touch the neighbours first: the next row's coefficient
+                    // load goes after the LDS reads
+                    __builtin_amdgcn_sched_barrier(0);
+                    kc[(p + 1) & 1] = c8[colb + min(iz + 1, iz1)];
+                } else {
+                    kv = c8[colb + iz];
+                }
                 const float k[7] = {kv[0], kv[1], kv[2], kv[3], kv[4], kv[5], kv[6]};
                 lanes_update<MODE, FMA, ACC == 2 ? 2 : (ACC == 1 ? 3 : 1), true>(
                     a, cf, fsz, colb, ix, iz, s, scell, P2, U, CS, q0, q1, q2, q3, q4, q5, q6,
