@@ -81,27 +81,25 @@ def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
 
 # HBM bytes per launch of each stencil mode of the c1 gradient at 120 shots in
 # lanes (forward and adjoint: averaged over the gradient's launches, one in five
-# with its accumulation epilogue), per stencil shape (kernel kind 2 lanes /
-# 3 lanes with LDS row exchange, columns per block, rows per wave, rows in
-# flight, XCD order): FETCH_SIZE x 2 + WRITE_SIZE from rocprofv3 counter runs
-# scaled by a 512 MiB copy; the runs and the calibration are in
+# with its accumulation epilogue), per stencil shape (kernel kind 3: lanes with
+# LDS row exchange; columns per block, rows per wave, XCD order; the rows of
+# loads in flight change when a row's loads are issued, not which bytes are
+# loaded): FETCH_SIZE x 2 + WRITE_SIZE from rocprofv3 counter runs scaled by a
+# 512 MiB copy; the runs and the calibration are in
 # profiles/fd_lanes_c1_traffic.md.
 C1_LANES_TRAFFIC = {
-    "reconstruction": {(2, 8, 8, 1, 0): 447.5e6, (2, 8, 16, 1, 1): 345.5e6,
-                       (2, 4, 8, 1, 0): 518.8e6, (2, 8, 8, 2, 0): 447.3e6,
-                       (3, 8, 16, 4, 1): 355.1e6, (3, 8, 16, 4, 0): 415.1e6},
-    "forward": {(3, 8, 16, 4, 1): 393.3e6, (3, 8, 16, 4, 0): 452.3e6},
-    "adjoint": {(3, 8, 16, 4, 1): 413.4e6, (3, 8, 16, 4, 0): 471.1e6},
+    "forward": {(3, 8, 16, 1): 394.5e6, (3, 8, 16, 0): 442.9e6},
+    "reconstruction": {(3, 8, 16, 1): 356.4e6, (3, 8, 16, 0): 406.4e6},
+    "adjoint": {(3, 8, 16, 1): 411.2e6, (3, 8, 16, 0): 461.3e6},
 }
 
 
 def measured_traffic(w, st, mode):
     m = MODE_INDEX[mode]
     kind = int(st[f"tile{m}_kind"])
-    if w.name != "c1" or w.nshot != 120 or kind not in (2, 3) or int(st[f"tile{m}_zfast"]) != 0:
+    if w.name != "c1" or w.nshot != 120 or kind != 3 or int(st[f"tile{m}_zfast"]) != 0:
         return None
-    key = (kind, int(st[f"tile{m}_wpb"]), int(st[f"tile{m}_zrun"]), int(st[f"tile{m}_pd"]),
-           int(st[f"tile{m}_remap"]))
+    key = (kind, int(st[f"tile{m}_wpb"]), int(st[f"tile{m}_zrun"]), int(st[f"tile{m}_remap"]))
     return C1_LANES_TRAFFIC.get(mode, {}).get(key)
 
 
