@@ -887,12 +887,23 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
     double pa[PD] = {};
     long acol = 0;
     if (ACC) acol = (long)min(max(ix - a.pml, 0), a.nx - 1) * a.nz * a.acc_cell + s;
+    // byte offsets: column bases and the row stride, so that a row's loads
+    // cost one multiply and an add each
+    const int rs4 = (int)(CS * 4);
+    const int colb4 = (int)(colb * CS * 4);
+    int hcol4[NH];
+#pragma unroll
+    for (int k = 0; k < NH; ++k) hcol4[k] = (int)(hcol[k] * CS * 4);
+    auto ldb = [&](frsrc f, int boff) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(f, vo, boff, 0));
+    };
     auto load_row = [&](int iz, int p) {
-        pn[p] = lget(P1, colb + iz + 3, CS, vo);
-        pp[p] = lget(P0, colb + iz, CS, vo);
+        const int ro = iz * rs4;
+        pn[p] = ldb(P1, colb4 + ro + 3 * rs4);
+        pp[p] = ldb(P0, colb4 + ro);
 #pragma unroll
         for (int k = 0; k < NH; ++k)
-            if (k < nh) ph[p][k] = lget(P1, hcol[k] + iz, CS, vo);
+            if (k < nh) ph[p][k] = ldb(P1, hcol4[k] + ro);
     };
     // issued after the row that used the slot's previous values is done, so
     // that old and new value can share registers (no copy, hence no wait, at
