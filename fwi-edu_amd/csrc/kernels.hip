@@ -593,12 +593,8 @@ __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long f
                                              float xp2, float xm3, float xp3, float p0v,
                                              const float *k, double accv = 0.0, float uv = 0.f,
                                              bool inj = false, float injv = 0.f,
-                                             const float *tv = nullptr, bool srcw = true,
-                                             bool mirw = true)
+                                             const float *tv = nullptr)
 {
-    // srcw: a shot of this wave may have its source in this column; mirw:
-    // the row may be one of the free surface's mirror rows (both uniform,
-    // so that most rows skip those tests with one scalar branch)
     const long i = colb + iz;
     const float pc = q3;
     const float v = fd_update<FMA>(pc, p0v, q2, q4, q1, q5, q0, q6, xm1, xp1, xm2, xp2, xm3, xp3,
@@ -606,21 +602,21 @@ __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long f
                                    a.invdzdz);
 
     float val = v;
-    if (MODE == FD_FWD && srcw && (int)i == scell) {
+    if (MODE == FD_FWD && (int)i == scell) {
         const double srcv = 1.0 * (double)a.src_wav[(long)s * a.nt + a.it] *
                             (double)(K8 ? k[6] : cf[CO_K * fsz + i]) * (double)a.dt * (double)a.dt *
                             (double)a.invdx * (double)a.invdz;
         val = (float)((double)v + srcv);
     }
     float stored = val;
-    if (a.fs && mirw) {
+    if (a.fs) {
         const int kk = iz - a.pml;
         if (kk == 0)
             stored = -0.0f;
         else if (kk <= a.pml)
             lput(P2, colb + (a.pml - kk), CS, s * 4u, -val);
     }
-    if (MODE == FD_RECON && srcw && (int)i == scell) {
+    if (MODE == FD_RECON && (int)i == scell) {
         stored = stored + a.src_wav[(long)s * a.nt + a.it] * (K8 ? k[6] : cf[CO_K * fsz + i]) * a.dt *
                               a.dt * a.invdx * a.invdz;
     }
@@ -848,9 +844,6 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
     const unsigned vo = s * 4u;
     const int scell = (MODE != FD_ADJ && a.src_cell) ? a.src_cell[s] : -1;
     const long colb = (long)min(ix, a.Nx - 1) * ld;
-    const bool srcw = MODE != FD_ADJ &&
-                      __builtin_amdgcn_ballot_w64(scell >= 0 && scell / ld == ix) != 0;
-    const bool mirw = a.fs && iz0 <= 2 * a.pml;
     // halo columns carried by this wave: h = w, w + WPB, ... < 6; LDS column
     // j = h (left halo) or WPB + h (right halo)
     int hj[NH];
@@ -904,18 +897,13 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
     auto ldb = [&](frsrc f, int boff) {
         return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(f, vo, boff, 0));
     };
-    // the halo test, resolved at compile time for the waves without halo
-    // columns and (one halo column at most per wave) for those with one
-    auto carries = [&](auto halo, int k) {
-        return decltype(halo)::value && (NH == 1 || k < nh);
-    };
-    auto load_row = [&](int iz, int p, auto halo) {
+    auto load_row = [&](int iz, int p) {
         const int ro = iz * rs4;
         pn[p] = ldb(P1, colb4 + ro + 3 * rs4);
         pp[p] = ldb(P0, colb4 + ro);
 #pragma unroll
         for (int k = 0; k < NH; ++k)
-            if (carries(halo, k)) ph[p][k] = ldb(P1, hcol4[k] + ro);
+            if (k < nh) ph[p][k] = ldb(P1, hcol4[k] + ro);
     };
     // issued after the row that used the slot's previous values is done, so
     // that old and new value can share registers (no copy, hence no wait, at
@@ -932,82 +920,74 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
             if (MODE == FD_ADJ) pu[p] = lget(U, colb + iz, CS, vo);
         }
     };
-    // the march, in two versions: waves that carry halo columns and waves
-    // that do not (no per-row test of that)
-    auto march = [&](auto halo) {
+#pragma unroll
+    for (int p = 0; p < PD; ++p) {
+        load_row(min(iz0 + p, iz1), p);
+        load_acc(min(iz0 + p, iz1), p);
+    }
+    // the cell's coefficients (c1, c2, c3*K, rho^-1_x, rho^-1_z, rho^-1, K),
+    // one 32-byte scalar load per row; with an even PD the next row's are
+    // loaded once this row's neighbours are read from LDS, so their latency
+    // passes during this row's arithmetic (a scalar load in flight would make
+    // every LDS wait a full wait, so none is issued before the LDS reads)
+    constexpr int CPD = (PD % 2 == 0) ? 2 : 1;
+    float8v kc[CPD];
+    if (CPD == 2) kc[0] = c8[colb + iz0];
+    // PD rows per outer iteration, each with its own prefetch slot: slot p is
+    // consumed and then reloaded with the row PD further down, so no register
+    // is copied while its load is in flight and the wait before a row covers
+    // only that row's loads (a rotating slot array makes the compiler wait for
+    // every outstanding load at each row)
+#pragma unroll 1
+    for (int izb = iz0; izb <= iz1; izb += PD) {
 #pragma unroll
         for (int p = 0; p < PD; ++p) {
-            load_row(min(iz0 + p, iz1), p, halo);
-            load_acc(min(iz0 + p, iz1), p);
-        }
-        // the cell's coefficients (c1, c2, c3*K, rho^-1_x, rho^-1_z, rho^-1, K),
-        // one 32-byte scalar load per row; with an even PD the next row's are
-        // loaded once this row's neighbours are read from LDS, so their latency
-        // passes during this row's arithmetic (a scalar load in flight would make
-        // every LDS wait a full wait, so none is issued before the LDS reads)
-        constexpr int CPD = (PD % 2 == 0) ? 2 : 1;
-        float8v kc[CPD];
-        if (CPD == 2) kc[0] = c8[colb + iz0];
-        // PD rows per outer iteration, each with its own prefetch slot: slot p is
-        // consumed and then reloaded with the row PD further down, so no register
-        // is copied while its load is in flight and the wait before a row covers
-        // only that row's loads (a rotating slot array makes the compiler wait for
-        // every outstanding load at each row)
-#pragma unroll 1
-        for (int izb = iz0; izb <= iz1; izb += PD) {
+            // rows past iz1 (the last group of a short run) go through the
+            // loads and barriers but compute nothing: no early exit, so the
+            // prefetched values never merge from two paths (a merge copies a
+            // register whose load is in flight and has to wait for it)
+            const int iz = izb + p;
+            const bool live = iz <= iz1;
+            const int buf = iz & 1;
+            const float q6 = pn[p], p0v = pp[p];
+            row[buf][3 + w][lane] = q3;
 #pragma unroll
-            for (int p = 0; p < PD; ++p) {
-                // rows past iz1 (the last group of a short run) go through the
-                // loads and barriers but compute nothing: no early exit, so the
-                // prefetched values never merge from two paths (a merge copies a
-                // register whose load is in flight and has to wait for it)
-                const int iz = izb + p;
-                const bool live = iz <= iz1;
-                const int buf = iz & 1;
-                const float q6 = pn[p], p0v = pp[p];
-                row[buf][3 + w][lane] = q3;
-#pragma unroll
-                for (int k = 0; k < NH; ++k)
-                    if (carries(halo, k)) row[buf][hj[k]][lane] = ph[p][k];
-                load_row(min(iz + PD, iz1), p, halo);
-                __syncthreads();
-                // the receiver sample of the forward: p1 at the receiver row
-                // (propagation.c:574), time-major in the stage
-                if (MODE == FD_FWD && iz == rr) a.stage[(long)a.it * a.nslot + slot] = q3;
-                if (active && live) {
-                    const float xm1 = row[buf][2 + w][lane], xp1 = row[buf][4 + w][lane];
-                    const float xm2 = row[buf][1 + w][lane], xp2 = row[buf][5 + w][lane];
-                    const float xm3 = row[buf][0 + w][lane], xp3 = row[buf][6 + w][lane];
-                    float8v kv;
-                    if (CPD == 2) {
-                        kv = kc[p & 1];
-                        // touch the neighbours first: the next row's coefficient
-                        // load goes after the LDS reads
-                        __builtin_amdgcn_sched_barrier(0);
-                        kc[(p + 1) & 1] = c8[colb + min(iz + 1, iz1)];
-                    } else {
-                        kv = c8[colb + iz];
-                    }
-                    const float k[7] = {kv[0], kv[1], kv[2], kv[3], kv[4], kv[5], kv[6]};
-                    lanes_update<MODE, FMA, ACC == 2 ? 2 : (ACC == 1 ? 3 : 1), true>(
-                        a, cf, fsz, colb, ix, iz, s, scell, P2, U, CS, q0, q1, q2, q3, q4, q5, q6,
-                        xm1, xp1, xm2, xp2, xm3, xp3, p0v, k, pa[p], pu[p],
-                        MODE == FD_ADJ && iz == rr, injv, pt[p], srcw, mirw);
+            for (int k = 0; k < NH; ++k)
+                if (k < nh) row[buf][hj[k]][lane] = ph[p][k];
+            load_row(min(iz + PD, iz1), p);
+            __syncthreads();
+            // the receiver sample of the forward: p1 at the receiver row
+            // (propagation.c:574), time-major in the stage
+            if (MODE == FD_FWD && iz == rr) a.stage[(long)a.it * a.nslot + slot] = q3;
+            if (active && live) {
+                const float xm1 = row[buf][2 + w][lane], xp1 = row[buf][4 + w][lane];
+                const float xm2 = row[buf][1 + w][lane], xp2 = row[buf][5 + w][lane];
+                const float xm3 = row[buf][0 + w][lane], xp3 = row[buf][6 + w][lane];
+                float8v kv;
+                if (CPD == 2) {
+                    kv = kc[p & 1];
+                    // touch the neighbours first: the next row's coefficient
+                    // load goes after the LDS reads
+                    __builtin_amdgcn_sched_barrier(0);
+                    kc[(p + 1) & 1] = c8[colb + min(iz + 1, iz1)];
+                } else {
+                    kv = c8[colb + iz];
                 }
-                load_acc(min(iz + PD, iz1), p);
-                q0 = q1;
-                q1 = q2;
-                q2 = q3;
-                q3 = q4;
-                q4 = q5;
-                q5 = q6;
+                const float k[7] = {kv[0], kv[1], kv[2], kv[3], kv[4], kv[5], kv[6]};
+                lanes_update<MODE, FMA, ACC == 2 ? 2 : (ACC == 1 ? 3 : 1), true>(
+                    a, cf, fsz, colb, ix, iz, s, scell, P2, U, CS, q0, q1, q2, q3, q4, q5, q6,
+                    xm1, xp1, xm2, xp2, xm3, xp3, p0v, k, pa[p], pu[p],
+                    MODE == FD_ADJ && iz == rr, injv, pt[p]);
             }
+            load_acc(min(iz + PD, iz1), p);
+            q0 = q1;
+            q1 = q2;
+            q2 = q3;
+            q3 = q4;
+            q4 = q5;
+            q5 = q6;
         }
-    };
-    if (nh > 0)
-        march(std::true_type{});
-    else
-        march(std::false_type{});
+    }
 }
 
 template <int MODE, int WPB, bool FMA, int ACC>
