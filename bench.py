@@ -73,6 +73,7 @@ def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
                "acc_shot_cells_per_launch")]
     names += [f"tile{m}_{k}" for m in range(3) for k in
               ("kind", "tz", "spb", "txm", "zfast", "pd", "remap", "wpb", "zrun")]
+    names += [f"tile{m}{v}_{k}" for m in (0, 2) for v in (1, 2) for k in ("zrun", "pd")]
     stats = {k: flib.stat(k) for k in names}
     L.fwiedu_session_destroy(sess)
     L.fwiedu_set_batch(0)
@@ -88,9 +89,15 @@ def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
 # 512 MiB copy; the runs and the calibration are in
 # profiles/fd_lanes_c1_traffic.md.
 C1_LANES_TRAFFIC = {
-    "forward": {(3, 8, 16, 1): 394.5e6, (3, 8, 16, 0): 442.9e6},
-    "reconstruction": {(3, 8, 16, 1): 356.4e6, (3, 8, 16, 0): 406.4e6},
-    "adjoint": {(3, 8, 16, 1): 411.2e6, (3, 8, 16, 0): 461.3e6},
+    "forward": {(3, 8, 16, 1): 394.5e6, (3, 8, 16, 0): 442.9e6,
+                (3, 8, 20, 1): 385.2e6, (3, 8, 20, 0): 434.4e6,
+                (3, 8, 24, 1): 377.7e6, (3, 8, 24, 0): 430.2e6},
+    "reconstruction": {(3, 8, 16, 1): 356.4e6, (3, 8, 16, 0): 406.4e6,
+                       (3, 8, 20, 1): 348.1e6, (3, 8, 20, 0): 398.1e6,
+                       (3, 8, 24, 1): 339.9e6, (3, 8, 24, 0): 393.9e6},
+    "adjoint": {(3, 8, 16, 1): 411.2e6, (3, 8, 16, 0): 461.3e6,
+                (3, 8, 20, 1): 401.9e6, (3, 8, 20, 0): 452.8e6,
+                (3, 8, 24, 1): 396.5e6, (3, 8, 24, 0): 448.6e6},
 }
 
 
@@ -157,10 +164,16 @@ def tile_shape(st, m):
 
 
 def stencil_shape(st):
-    """The autotuned launch shape of each stencil mode of the last batch."""
+    """The autotuned launch shape of each stencil mode of the last batch, and
+    (forward, adjoint) the rows per wave and rows in flight of its launches that
+    stage their accumulation term or add the staged terms to the accumulator."""
     d = {"layout": "shots in lanes" if st["layout"] else "per-shot arenas"}
     for name, m in MODE_INDEX.items():
         d[name] = tile_shape(st, m)
+        if m != 1 and st.get(f"tile{m}_kind") == 3:
+            for v, tag in ((1, "staging"), (2, "adding")):
+                d[name][tag] = {"rows_per_wave": int(st[f"tile{m}{v}_zrun"]),
+                                "prefetch_rows": int(st[f"tile{m}{v}_pd"])}
     return d
 
 
@@ -282,7 +295,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-host", action="store_true",
                     help="skip the host-array (PCIe-inclusive) iso_acoustic2d_propagation timing")
-    ap.add_argument("--ktime-every", type=int, default=11)
+    ap.add_argument("--ktime-every", type=int, default=41,
+                    help="HIP-event timing of every Nth stencil launch of each mode (a timed "
+                         "launch leaves a few microseconds of idle device time around it; "
+                         "41 is prime to the accumulation period of 5)")
     ap.add_argument("--serving", type=int, default=48,
                     help="requests of the serving leg (0 = skip; 2000 = the full stream)")
     ap.add_argument("--serving-rate", type=float, default=3.0, help="Poisson arrivals per second")

@@ -417,7 +417,7 @@ extern "C" void fwiedu_set_lanes(int wpb, int zrun, int pd, int lds)
     const bool zok = zrun == 8 || zrun == 16 || zrun == 32 || zrun == 64 || zrun == 128 ||
                      zrun == 256;
     if (lds) {
-        if ((wpb != 4 && wpb != 8 && wpb != 16) || !zok || pd < 1 || pd > 4)
+        if ((wpb != 4 && wpb != 8 && wpb != 16) || zrun < 8 || zrun > 256 || pd < 1 || pd > 4)
             fail("<error>: fwiedu_set_lanes (LDS row exchange): 4, 8 or 16 columns per block, "
                  "8 to 256 rows per block, 1 to 4 rows of loads in flight");
     } else if ((wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8) || !zok || pd < 1 || pd > 3) {
@@ -473,10 +473,14 @@ extern "C" double fwiedu_stat(const char *name)
     if (n == "fd_cells_timed") return s.fd_cells_timed;
     if (n == "fd_cells_per_launch") return s.fd_timed ? s.fd_cells_timed / s.fd_timed : 0.0;
     if (n == "fd_shots_per_launch") return (double)s.shots_per_launch;
-    // launch shape per stencil mode m: "tile<m>_kind", "tile<m>_wpb", ...
-    if (n.size() > 6 && n.compare(0, 4, "tile") == 0 && n[4] >= '0' && n[4] <= '2' && n[5] == '_') {
-        const FdTile &t = s.tiles[n[4] - '0'];
-        const std::string k = n.substr(6);
+    // launch shape per stencil mode m: "tile<m>_kind", "tile<m>_wpb", ...; of
+    // its launches that stage (v = 1) or add to the accumulator (v = 2):
+    // "tile<m><v>_kind", ...
+    const bool tm = n.size() > 6 && n.compare(0, 4, "tile") == 0 && n[4] >= '0' && n[4] <= '2';
+    const bool tv = tm && n.size() > 7 && n[5] >= '1' && n[5] <= '2' && n[6] == '_';
+    if (tm && (n[5] == '_' || tv)) {
+        const FdTile &t = tv ? s.tiles_acc[n[4] - '0'][n[5] - '0'] : s.tiles[n[4] - '0'];
+        const std::string k = n.substr(tv ? 7 : 6);
         if (k == "kind") return t.kind;
         if (k == "tz") return t.tz;
         if (k == "spb") return t.spb;

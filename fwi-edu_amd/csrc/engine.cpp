@@ -249,11 +249,72 @@ void collect_timing()
 // (per-shot arenas: tile height, shots per block, block order, column march;
 // shots in lanes: kernel, columns per block, rows per wave, rows of loads in
 // flight, block order) on that mode's stencil over three distinct zero-filled
-// work fields of the batch, with no source and no accumulation epilogue (the
-// output is zero again, so nothing the passes read changes), and keeps the
-// fastest.  The modes are tuned apart because their epilogues change the
-// register budget: deeper prefetch pays in the reconstruction sweep and not
-// in the forward and adjoint stencils.
+// work fields of the batch, rotating as in a time loop, with no source and no
+// accumulation epilogue (the output is zero again, so nothing the passes read
+// changes), and keeps the fastest.  The modes are tuned apart because their
+// epilogues change the register budget: deeper prefetch pays in the
+// reconstruction sweep and not in the forward and adjoint stencils; the
+// launches with the accumulation epilogue are tuned apart again
+// (autotune_acc_tile).
+
+// Average time of one launch of shape t over reps launches (after one
+// untimed), the three work fields rotating as in a time loop: each launch
+// reads the field the previous one wrote.
+static float time_shape(int mode, FdArgs a, const FdTile &t, float *const f[3], int reps,
+                        hipEvent_t e0, hipEvent_t e1)
+{
+    Ctx &c = ctx();
+    auto go = [&](int r) {
+        a.cur = f[r % 3];
+        a.out = f[(r + 1) % 3];
+        a.prev = f[(r + 2) % 3];
+        launch_fd(mode, a, t, c.st);
+    };
+    go(0);
+    HIPCHECK(hipEventRecord(e0, c.st));
+    for (int r = 1; r <= reps; ++r) go(r);
+    HIPCHECK(hipEventRecord(e1, c.st));
+    HIPCHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    return ms / (float)reps;
+}
+
+// The fastest of the candidate shapes.  Every shape is timed over 9 launches
+// (3 once a launch takes more than 0.2 ms); the four fastest are then timed
+// again, twice each in alternating order over three times as many launches,
+// and the best of those averages wins, so that shapes within a percent of
+// each other are not told apart by one short sample.
+static FdTile fastest_shape(int mode, const FdArgs &a, const std::vector<FdTile> &cand,
+                            float *const f[3])
+{
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
+    HIPCHECK(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
+    if (cand.empty()) fail("<error>: internal: no launch shape to tune");
+    std::vector<std::pair<float, int>> ms;
+    int reps = 9;
+    for (size_t i = 0; i < cand.size(); ++i) {
+        const float t = time_shape(mode, a, cand[i], f, reps, e0, e1);
+        ms.push_back({t, (int)i});
+        if (reps == 9 && t > 0.2f) reps = 3;
+    }
+    std::sort(ms.begin(), ms.end());
+    const int top = std::min<int>(4, (int)ms.size());
+    std::vector<float> again(top, 1e30f);
+    for (int pass = 0; pass < 2; ++pass)
+        for (int j = 0; j < top; ++j) {
+            const int k = pass ? top - 1 - j : j;
+            again[k] = std::min(again[k],
+                                time_shape(mode, a, cand[ms[k].second], f, 3 * reps, e0, e1));
+        }
+    int best = 0;
+    for (int k = 1; k < top; ++k)
+        if (again[k] < again[best]) best = k;
+    HIPCHECK(hipEventDestroy(e0));
+    HIPCHECK(hipEventDestroy(e1));
+    return cand[ms[best].second];
+}
 
 FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur, const float *prev)
 {
@@ -282,15 +343,13 @@ FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur,
     a.acc = nullptr;
     a.acc_on = 0;
     a.u = nullptr;
-    hipEvent_t e0, e1;
-    HIPCHECK(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
-    HIPCHECK(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
     std::vector<FdTile> cand;
     if (lanes) {
         // the shapes that won somewhere among grids of 128^2 to 3000 x 1000
         // cells and 48 to 256 shots: the LDS kernel with 8 columns per block
-        // (16 were never faster), and for the smallest grids the plain lanes
-        // kernel with 4 columns and 8 rows per block
+        // (16 were never faster) and runs of 16 to 32 rows, and for the
+        // smallest grids the plain lanes kernel with 4 columns and 8 rows per
+        // block
         for (int rm : {0, 1})
             for (int sgo : {0, 2}) {
                 FdTile t;
@@ -303,7 +362,7 @@ FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur,
                 t.pd = 1;
                 cand.push_back(t);
             }
-        for (int zrun : {16, 32})
+        for (int zrun : {16, 20, 24, 32})
             for (int rm : {0, 1})
                 for (int order : {0, 2})
                     for (int pd : {2, 4}) {
@@ -343,30 +402,54 @@ FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur,
                 if (march_lds_bytes(base, t) > 65536) continue;
                 cand.push_back(t);
             }
-    FdTile best;
-    float best_ms = 1e30f;
-    // timed launches per shape: 8, or 3 when one launch of the geometry takes
-    // more than 0.2 ms (then 3 launches are as steady, and tuning a large
-    // grid costs a third)
-    int reps = 8;
-    for (const FdTile &t : cand) {
-        launch_fd(mode, a, t, c.st);
-        HIPCHECK(hipEventRecord(e0, c.st));
-        for (int r = 0; r < reps; ++r) launch_fd(mode, a, t, c.st);
-        HIPCHECK(hipEventRecord(e1, c.st));
-        HIPCHECK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
-        ms /= (float)reps;
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = t;
-        }
-        if (reps == 8 && ms > 0.2f) reps = 3;
-    }
-    HIPCHECK(hipEventDestroy(e0));
-    HIPCHECK(hipEventDestroy(e1));
+    float *const f[3] = {out, const_cast<float *>(cur), const_cast<float *>(prev)};
+    const FdTile best = fastest_shape(mode, a, cand, f);
     c.tuned[key] = best;
+    return best;
+}
+
+// The launches of the shots-in-lanes LDS stencil that carry the accumulation
+// epilogue (accv 1: stage this step's term, 2: add the staged terms and this
+// step's to the FP64 accumulator) hold more registers than the plain ones, so
+// their shape is tuned apart: rows per wave and rows in flight, with the
+// block orders of the mode's own shape.  base carries the batch's accumulator
+// and staging buffers, timed right after the accumulator is cleared and with
+// zero work fields: the staging slot 0 receives zeros (nothing is staged yet)
+// and the accumulator adds +0 terms to +0, so nothing a later launch reads
+// changes.
+FdTile autotune_acc_tile(int mode, int accv, const FdTile &mode_tile, const FdArgs &base,
+                         float *out, const float *cur, const float *prev)
+{
+    Ctx &c = ctx();
+    if (c.tile_req.tz > 0 || mode_tile.kind != 3 || mode == FD_RECON || !base.acc ||
+        !base.accstage)
+        return mode_tile;
+    const auto key = std::make_tuple(base.Nx, base.ld, base.ixs, base.ixe, base.izs, base.ize,
+                                     base.fs, ((base.nshot * 2 + base.fma) * 4 + mode) * 4 + accv,
+                                     base.cell_stride);
+    auto it = c.tuned_acc.find(key);
+    if (it != c.tuned_acc.end()) return it->second;
+    FdArgs a = base;
+    a.out = out;
+    a.cur = cur;
+    a.prev = prev;
+    a.src_cell = nullptr;
+    a.u = prev;
+    a.acc_on = 1;
+    a.acc_stage = accv == 1 ? 1 : 0;
+    a.acc_slot = 0;
+    a.acc_n = 0;
+    std::vector<FdTile> cand;
+    for (int zrun : {16, 20, 24, 32})
+        for (int pd : {2, 4}) {
+            FdTile t = mode_tile;
+            t.zrun = zrun;
+            t.pd = pd;
+            cand.push_back(t);
+        }
+    float *const f[3] = {out, const_cast<float *>(cur), const_cast<float *>(prev)};
+    const FdTile best = fastest_shape(mode, a, cand, f);
+    c.tuned_acc[key] = best;
     return best;
 }
 
@@ -777,42 +860,85 @@ static std::vector<float> taper_table(int &L, int n)
     return t;
 }
 
-// compute_adjoint_source2d (optimization.c:300-419) for filter == 2, scalars
-// on the host, residual and tapers on the device.  rec/obs/out are [ntr][ns].
+// compute_adjoint_source2d (optimization.c:300-419) for the shots of a batch:
+// scalars and taper tables on the host (every shot's tables in one upload, so
+// the shots need no host round trip each), residual, tapers and filters on
+// the device.  pre/obs/out + off[k] are shot k's [ntr][ns] records.
+static void adjoint_sources(const Record2D *const *r, int n, float **vp_full, float freq,
+                            int filter, const float *pre, const float *obs, float *out,
+                            const long *off)
+{
+    Ctx &c = ctx();
+    std::vector<float> tab;
+    std::vector<long> oT(n), oX(n);
+    std::vector<int> Lt(n), Lx(n);
+    std::vector<float> fr(n), kp(n);
+    for (int k = 0; k < n; ++k) {
+        const int ntr = r[k]->ntr, ns = r[k]->ns;
+        double vsum = 0.0;
+        for (int itr = 0; itr < ntr; ++itr) vsum += vp_full[r[k]->igx[itr]][r[k]->igz[itr]];
+        const float vavg = (float)(vsum / ntr);
+        fr[k] = freq == 0.0f ? r[k]->fpeak : freq;
+        kp[k] = fr[k] / vavg;
+        Lt[k] = (int)roundf((2.0f / fr[k]) / r[k]->dt) + 1;
+        Lx[k] = (int)roundf((2.0f / kp[k]) / r[k]->dtr) + 1;
+        const std::vector<float> cT = taper_table(Lt[k], ns), cX = taper_table(Lx[k], ntr);
+        oT[k] = (long)tab.size();
+        tab.insert(tab.end(), cT.begin(), cT.end());
+        oX[k] = (long)tab.size();
+        tab.insert(tab.end(), cX.begin(), cX.end());
+    }
+    // the table buffer is only rewritten by a later call, in stream order
+    float *tb = c.taper.as<float>(tab.size() + 64);
+    if (!tab.empty()) h2d(tb, tab.data(), tab.size() * sizeof(float));
+    for (int k = 0; k < n; ++k) {
+        const int ntr = r[k]->ntr, ns = r[k]->ns;
+        const float *tT = tb + oT[k], *tX = tb + oX[k];
+        const float *pk = pre + off[k], *ok = obs + off[k];
+        float *dk = out + off[k];
+        if (filter == 0 || filter == 1) {
+            // optimization.c:368-415: taper in t, filter in t (freq, 2 freq),
+            // taper in x, transpose, filter in x (kpass, 2 kpass), transpose back
+            launch_adjsrc(pk, ok, dk, ntr, ns, tT, Lt[k], tX, -1, c.st);
+            filter_traces_device(filter, dk, ns, ntr, fr[k], 2.0f * fr[k], r[k]->dt);
+            launch_taper(dk, ntr, ns, tT, -1, tX, Lx[k], c.st);
+            float *tr = c.ffttmp.as<float>((size_t)ntr * ns + 64);
+            launch_transpose(dk, tr, ntr, ns, c.st);
+            filter_traces_device(filter, tr, ntr, ns, kp[k], 2.0f * kp[k], r[k]->dtr);
+            launch_transpose(tr, dk, ns, ntr, c.st);
+            // optimization.c:417-418: the final taper in t and x
+            launch_taper(dk, ntr, ns, tT, Lt[k], tX, Lx[k], c.st);
+        } else {
+            launch_adjsrc(pk, ok, dk, ntr, ns, tT, Lt[k], tX, Lx[k], c.st);
+        }
+    }
+}
+
+// compute_adjoint_source2d of one shot; the result is complete on return.
 void adjoint_source_device(const Record2D *r, float **vp_full, float freq, int filter,
                            const float *pre_dev, const float *obs_dev, float *out_dev)
 {
-    Ctx &c = ctx();
-    const int ntr = r->ntr, ns = r->ns;
-    double vsum = 0.0;
-    for (int itr = 0; itr < ntr; ++itr) vsum += vp_full[r->igx[itr]][r->igz[itr]];
-    const float vavg = (float)(vsum / ntr);
-    if (freq == 0.0f) freq = r->fpeak;
-    const float kpass = freq / vavg;
-    int Lt = (int)roundf((2.0f / freq) / r->dt) + 1;
-    int Lx = (int)roundf((2.0f / kpass) / r->dtr) + 1;
-    std::vector<float> cT = taper_table(Lt, ns), cX = taper_table(Lx, ntr);
-    float *tb = c.taper.as<float>(cT.size() + cX.size() + 64);
-    if (!cT.empty()) h2d(tb, cT.data(), cT.size() * sizeof(float));
-    if (!cX.empty()) h2d(tb + cT.size(), cX.data(), cX.size() * sizeof(float));
-    const float *tT = tb, *tX = tb + cT.size();
-    if (filter == 0 || filter == 1) {
-        // optimization.c:368-415: taper in t, filter in t (freq, 2 freq), taper
-        // in x, transpose, filter in x (kpass, 2 kpass), transpose back
-        launch_adjsrc(pre_dev, obs_dev, out_dev, ntr, ns, tT, Lt, tX, -1, c.st);
-        filter_traces_device(filter, out_dev, ns, ntr, freq, 2.0f * freq, r->dt);
-        launch_taper(out_dev, ntr, ns, tT, -1, tX, Lx, c.st);
-        float *tr = c.ffttmp.as<float>((size_t)ntr * ns + 64);
-        launch_transpose(out_dev, tr, ntr, ns, c.st);
-        filter_traces_device(filter, tr, ntr, ns, kpass, 2.0f * kpass, r->dtr);
-        launch_transpose(tr, out_dev, ns, ntr, c.st);
-        // optimization.c:417-418: the final taper in t and x
-        launch_taper(out_dev, ntr, ns, tT, Lt, tX, Lx, c.st);
-    } else {
-        launch_adjsrc(pre_dev, obs_dev, out_dev, ntr, ns, tT, Lt, tX, Lx, c.st);
-    }
-    // the taper table must stay valid until the kernels ran
+    const long off = 0;
+    adjoint_sources(&r, 1, vp_full, freq, filter, pre_dev, obs_dev, out_dev, &off);
     sync();
+}
+
+// Device-to-device copies of n pieces (piece k: bytes[k] from src + soff[k] to
+// dst + doff[k], offsets in floats), pieces that continue each other on both
+// sides merged into one copy.
+static void copy_pieces(float *dst, const long *doff, const float *src, const long *soff,
+                        const long *len, int n)
+{
+    Ctx &c = ctx();
+    for (int k = 0; k < n;) {
+        long m = len[k];
+        int j = k + 1;
+        while (j < n && doff[j] == doff[k] + m && soff[j] == soff[k] + m) m += len[j++];
+        if (m > 0)
+            HIPCHECK(hipMemcpyAsync(dst + doff[k], src + soff[k], (size_t)m * sizeof(float),
+                                    hipMemcpyDeviceToDevice, c.st));
+        k = j;
+    }
 }
 
 // Copy a host record [ntr][ns] (contiguous block traces_p[0]) to / from the device.
@@ -869,6 +995,11 @@ struct BatchState {
     float *stage = nullptr;    // [nt][nslot] receiver values of the lanes stencil
     long nslot = 0;
     FdTile tile_m[3];          // launch shape per stencil mode (FD_FWD, FD_RECON, FD_ADJ)
+    FdTile tile_acc[3][3];     // [mode][0 plain, 1 staging, 2 adding to the accumulator]
+    const FdTile &tile_for(int mode, const FdArgs &a) const
+    {
+        return tile_acc[mode][!a.acc_on ? 0 : (a.acc_stage ? 1 : 2)];
+    }
     FdTile tile;               // the reconstruction sweep's (reported)
     float *F(int k) const { return arena + (long)k * KS; }
 };
@@ -970,6 +1101,17 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
         b.tile_m[m] = autotune_tile(m, base_args(g, coef, S, b.SS, b.CS), b.F(BatchPlan::A2),
                                     b.F(BatchPlan::A0), b.F(BatchPlan::A1));
         c.stats.tiles[m] = b.tile_m[m];
+        FdArgs aa = base_args(g, coef, S, b.SS, b.CS);
+        aa.acc = b.acc;
+        aa.acc_stride = b.ASS;
+        aa.acc_cell = b.ACS;
+        aa.accstage = b.accstage;
+        aa.accstage_stride = b.accstage_stride;
+        b.tile_acc[m][0] = b.tile_m[m];
+        for (int v = 1; v <= 2; ++v)
+            b.tile_acc[m][v] = autotune_acc_tile(m, v, b.tile_m[m], aa, b.F(BatchPlan::A2),
+                                                 b.F(BatchPlan::A0), b.F(BatchPlan::A1));
+        for (int v = 0; v < 3; ++v) c.stats.tiles_acc[m][v] = b.tile_acc[m][v];
     }
     b.tile = b.tile_m[FD_RECON];
     c.stats.tile_tz = b.tile.tz;
@@ -1062,7 +1204,7 @@ static void forward_loop(BatchState &b, bool save_ckpt, bool precond)
         a.it = it;
         a.acc_on = (precond && it % MST == 0) ? 1 : 0;
         set_deferral(a, defer, staged, it + MST > b.nt - 1);
-        fd_launch(FD_FWD, a, b.cells_per_launch, b.tile_m[FD_FWD]);
+        fd_launch(FD_FWD, a, b.cells_per_launch, b.tile_for(FD_FWD, a));
         if (ck && b.ck16) ck_store(b, sc.ckslot[it + 1], out);
         prev = cur;
         cur = out;
@@ -1184,7 +1326,7 @@ static void adjoint_loop(BatchState &b)
             ck_load(b, sc.ckslot[it], b.F(BatchPlan::RU));
             ad.u = b.F(BatchPlan::RU);
         }
-        fd_launch(FD_ADJ, ad, b.cells_per_launch, b.tile_m[FD_ADJ]);
+        fd_launch(FD_ADJ, ad, b.cells_per_launch, b.tile_for(FD_ADJ, ad));
         prev = cur;
         cur = out;
     }
@@ -1229,10 +1371,11 @@ static std::vector<long> batch_key(int pass, const BatchState &b, bool f1, bool 
                           (const void *)b.tb.rrow, (const void *)b.tb.rec_slot,
                           (const void *)b.tb.rcv_left.trace_off, (const void *)b.tb.inj_left.beg})
         k.push_back((long)(uintptr_t)p);
-    for (const FdTile &t : b.tile_m)
-        for (long v : {(long)t.kind, (long)t.tz, (long)t.spb, (long)t.txm, (long)t.zfast,
-                       (long)t.pd, (long)t.remap, (long)t.wpb, (long)t.zrun})
-            k.push_back(v);
+    for (const auto &tm : b.tile_acc)
+        for (const FdTile &t : tm)
+            for (long v : {(long)t.kind, (long)t.tz, (long)t.spb, (long)t.txm, (long)t.zfast,
+                           (long)t.pd, (long)t.remap, (long)t.wpb, (long)t.zrun})
+                k.push_back(v);
     return k;
 }
 
@@ -1494,18 +1637,20 @@ void propagate_core(const Call &P)
             }
             BatchState b;
             prepare_batch(b, g, coef, recs, nt, problem == 1, sc, lanes, ck16);
+            // the batch's records in the caller's device buffers (sessions)
+            std::vector<long> boff(nb), blen(nb);
+            for (int k = 0; k < nb; ++k) {
+                const int sidx = grp[b0 + k];
+                boff[k] = P.obs_off ? P.obs_off[sidx] : 0;
+                blen[k] = (long)P.obs[sidx].ntr * P.obs[sidx].ns;
+            }
             float *obsd = nullptr;
             if (problem != 0) {
                 obsd = c.obs.as<float>(b.rec_total + 64);
-                for (int k = 0; k < nb; ++k) {
-                    const int sidx = grp[b0 + k];
-                    if (P.obs_dev)
-                        HIPCHECK(hipMemcpyAsync(obsd + b.roff[k], P.obs_dev + P.obs_off[sidx],
-                                                (size_t)P.obs[sidx].ntr * P.obs[sidx].ns * sizeof(float),
-                                                hipMemcpyDeviceToDevice, c.st));
-                    else
-                        rec_h2d(obsd + b.roff[k], &P.obs[sidx]);
-                }
+                if (P.obs_dev)
+                    copy_pieces(obsd, b.roff.data(), P.obs_dev, boff.data(), blen.data(), nb);
+                else
+                    for (int k = 0; k < nb; ++k) rec_h2d(obsd + b.roff[k], &P.obs[grp[b0 + k]]);
             }
             const long nacc = b.nacc;
             float *gb = nullptr, *pb = nullptr;
@@ -1525,25 +1670,17 @@ void propagate_core(const Call &P)
                     for (int k = 0; k < nb; ++k) rec_d2h(recs[k], b.rec + b.roff[k]);
                 continue;
             }
-            for (int k = 0; k < nb; ++k)
-                adjoint_source_device(recs[k], P.vp, P.freq, P.filter, b.rec + b.roff[k],
-                                      obsd + b.roff[k], b.rec + b.roff[k]);
+            adjoint_sources(recs.data(), nb, P.vp, P.freq, P.filter, b.rec, obsd, b.rec,
+                            b.roff.data());
             if (mis) {
                 double *scr = c.scratch.as<double>(reduce_scratch_doubles() + (size_t)Nshot * 512 + Nshot + 64);
-                for (int k = 0; k < nb; ++k) {
+                for (int k = 0; k < nb; ++k)
                     launch_sum_sq(b.rec + b.roff[k], (long)recs[k]->ntr * recs[k]->ns, scr,
-                                  scr + 2 * 512, c.st);
-                    HIPCHECK(hipMemcpyAsync(mis + 2 * grp[b0 + k], scr + 2 * 512, 2 * sizeof(double),
-                                            hipMemcpyDeviceToDevice, c.st));
-                }
+                                  mis + 2 * grp[b0 + k], c.st);
             }
             if (P.records_to_host)
                 for (int k = 0; k < nb; ++k) rec_d2h(recs[k], b.rec + b.roff[k]);
-            if (P.pre_dev)
-                for (int k = 0; k < nb; ++k)
-                    HIPCHECK(hipMemcpyAsync(P.pre_dev + P.obs_off[grp[b0 + k]], b.rec + b.roff[k],
-                                            (size_t)recs[k]->ntr * recs[k]->ns * sizeof(float),
-                                            hipMemcpyDeviceToDevice, c.st));
+            if (P.pre_dev) copy_pieces(P.pre_dev, boff.data(), b.rec, b.roff.data(), blen.data(), nb);
             if (problem == 2) continue;
             bool full = false;
             for (int k = 0; k < nb; ++k) full |= recs[k]->sampleRate != 1 || recs[k]->ns != nt;
@@ -1560,14 +1697,13 @@ void propagate_core(const Call &P)
                 double *scr = c.scratch.as<double>((size_t)nb * 512 + nb + reduce_scratch_doubles());
                 launch_shot_precondition(gb, pb, nacc, nb, (int)nacc, scr, c.st);
             }
+            std::vector<long> go(nb), bo(nb), gl(nb, nacc);
             for (int k = 0; k < nb; ++k) {
-                const int sidx = grp[b0 + k];
-                HIPCHECK(hipMemcpyAsync(gout + (size_t)sidx * win_max, gb + (size_t)k * nacc,
-                                        nacc * sizeof(float), hipMemcpyDeviceToDevice, c.st));
-                if (P.precondition == 1)
-                    HIPCHECK(hipMemcpyAsync(pout + (size_t)sidx * win_max, pb + (size_t)k * nacc,
-                                            nacc * sizeof(float), hipMemcpyDeviceToDevice, c.st));
+                go[k] = (long)grp[b0 + k] * win_max;
+                bo[k] = (long)k * nacc;
             }
+            copy_pieces(gout, go.data(), gb, bo.data(), gl.data(), nb);
+            if (P.precondition == 1) copy_pieces(pout, go.data(), pb, bo.data(), gl.data(), nb);
         }
     }
     if (problem == 1) {

@@ -57,6 +57,7 @@ struct Stats {
     int tile_tz = 0, tile_spb = 0, tile_kind = 0, tile_txm = 0, tile_zfast = 0, tile_pd = 0, tile_remap = 0;
     int tile_wpb = 0, tile_zrun = 0, layout = 0, ckpt_bits = 32;   // the reconstruction sweep's
     FdTile tiles[3];              // launch shape per stencil mode of the last batch
+    FdTile tiles_acc[3][3];       // ... and of its launches with the accumulation epilogue
     long graphs_built = 0, graph_launches = 0;
 };
 
@@ -83,6 +84,7 @@ struct Ctx {
     bool capturing = false;
     std::map<std::vector<long>, GraphEntry> graph_cache;
     std::map<std::tuple<int, int, int, int, int, int, int, int, long>, FdTile> tuned;
+    std::map<std::tuple<int, int, int, int, int, int, int, int, long>, FdTile> tuned_acc;
     Stats stats;
     std::vector<hipEvent_t> ev;   // pool of events for sampled kernel timing
     size_t ev_used = 0;
@@ -142,6 +144,8 @@ void adjoint_source_device(const Record2D *r, float **vp_full, float freq, int f
 // Sampled timing of stencil launches.
 void fd_launch(int mode, const FdArgs &a, long cells, FdTile tile);
 FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur, const float *prev);
+FdTile autotune_acc_tile(int mode, int accv, const FdTile &mode_tile, const FdArgs &base,
+                         float *out, const float *cur, const float *prev);
 void collect_timing();
 void drop_graphs();
 

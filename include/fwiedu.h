@@ -164,9 +164,11 @@ void fwiedu_set_march(int txm, int spb, int pd);
 void fwiedu_set_xcd_remap(int on);
 /* Force the shots-in-lanes stencil (fields laid out [x][z][shot], the 64 lanes
  * of a wavefront are 64 shots of one cell): wpb columns per block, each wave
- * marching zrun (8 ... 256) rows with the loads of the next pd rows in flight.
- * lds = 0: x-neighbours loaded from memory (wpb 1, 2, 4, 8; pd 1 ... 3);
- * lds = 1: x-neighbours exchanged through LDS (wpb 4, 8, 16; pd 1 ... 4).
+ * marching zrun rows with the loads of the next pd rows in flight.
+ * lds = 0: x-neighbours loaded from memory (wpb 1, 2, 4, 8; zrun 8, 16, 32,
+ * 64, 128 or 256; pd 1 ... 3);
+ * lds = 1: x-neighbours exchanged through LDS (wpb 4, 8, 16; zrun 8 ... 256;
+ * pd 1 ... 4).
  * Applies to iso_acoustic2d_propagation and sessions; fwiedu_set_tile(0, 0)
  * returns to the autotuner. */
 void fwiedu_set_lanes(int wpb, int zrun, int pd, int lds);
@@ -206,7 +208,11 @@ void fwiedu_set_checkpoint_bits(int bits);
  * 2 adjoint: "fd<m>_timed", "fd<m>_ms_avg", "fd<m>_cells_per_launch",
  * "fd<m>_shot_cells_per_launch", "fd<m>_acc_shot_cells_per_launch" (shot-cells
  * of the sampled launches whose pseudo-Hessian or imaging epilogue ran, per
- * sampled launch).  Unknown names return -1. */
+ * sampled launch); the launch shape of mode m in the last batch, "tile<m>_kind",
+ * "tile<m>_wpb", "tile<m>_zrun", "tile<m>_pd", "tile<m>_remap", "tile<m>_zfast"
+ * ..., and of its launches that stage their accumulation term (v = 1) or add
+ * the staged terms to the FP64 accumulator (v = 2): "tile<m><v>_zrun", ...
+ * Unknown names return -1. */
 double fwiedu_stat(const char *name);
 void fwiedu_reset_stats(void);
 /* Device-resident gradient session: the model, the observed records and all
