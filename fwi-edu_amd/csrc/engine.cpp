@@ -281,10 +281,11 @@ static float time_shape(int mode, FdArgs a, const FdTile &t, float *const f[3], 
 }
 
 // The fastest of the candidate shapes.  Every shape is timed over 9 launches
-// (3 once a launch takes more than 0.2 ms); the four fastest are then timed
-// again, twice each in alternating order over three times as many launches,
-// and the best of those averages wins, so that shapes within a percent of
-// each other are not told apart by one short sample.
+// (3 once a launch takes more than 0.2 ms); the shapes within 3 % of the
+// fastest (four at most) are then timed again, twice each in alternating
+// order over three times as many launches, and the best of those averages
+// wins, so that shapes within a percent of each other are not told apart by
+// one short sample.
 static FdTile fastest_shape(int mode, const FdArgs &a, const std::vector<FdTile> &cand,
                             float *const f[3])
 {
@@ -300,7 +301,13 @@ static FdTile fastest_shape(int mode, const FdArgs &a, const std::vector<FdTile>
         if (reps == 9 && t > 0.2f) reps = 3;
     }
     std::sort(ms.begin(), ms.end());
-    const int top = std::min<int>(4, (int)ms.size());
+    int top = 1;
+    while (top < 4 && top < (int)ms.size() && ms[top].first <= 1.03f * ms[0].first) ++top;
+    if (top == 1) {
+        HIPCHECK(hipEventDestroy(e0));
+        HIPCHECK(hipEventDestroy(e1));
+        return cand[ms[0].second];
+    }
     std::vector<float> again(top, 1e30f);
     for (int pass = 0; pass < 2; ++pass)
         for (int j = 0; j < top; ++j) {
@@ -553,49 +560,27 @@ static Tables build_tables(const std::vector<ShotIn> &shots, const Geom &g, int 
         full_total += (long)shots[s].pre->ntr * nt;
         nent += shots[s].pre->ntr;
     }
-    std::vector<int> e_shot(nent), e_cell(nent), e_sr(nent);
+    std::vector<int> e_shot(nent), e_cell(nent), e_sr(nent), e_itr(nent);
     std::vector<long> e_off(nent);
-    // unique (shot, cell) for the injection, receiver order kept within a cell
-    std::vector<int> u_shot, u_cell, u_beg;
-    std::vector<long> u_off, u_full;
     int k = 0;
     for (int s = 0; s < S; ++s) {
         const Record2D *r = shots[s].pre;
-        std::map<int, std::vector<int>> bycell;
-        std::vector<int> order;
         for (int itr = 0; itr < r->ntr; ++itr, ++k) {
             const int gx = r->igx[itr] + g.pml, gz = r->igz[itr] + g.pml;
             if (gx < 0 || gx >= g.Nx || gz < 0 || gz >= g.Nz)
                 fail("<error>: receiver %d of shot %d lies outside the padded grid", itr, r->shotID);
-            const int cell = gx * g.ld + gz;
             e_shot[k] = s;
-            e_cell[k] = cell;
+            e_cell[k] = gx * g.ld + gz;
             e_sr[k] = r->sampleRate;
+            e_itr[k] = itr;
             e_off[k] = roff[s] + (long)itr * r->ns;
-            auto it = bycell.find(cell);
-            if (it == bycell.end()) {
-                order.push_back(cell);
-                bycell[cell] = {itr};
-            } else {
-                it->second.push_back(itr);
-            }
-        }
-        for (int cell : order) {
-            u_shot.push_back(s);
-            u_cell.push_back(cell);
-            u_beg.push_back((int)u_off.size());
-            for (int itr : bycell[cell]) {
-                u_off.push_back(roff[s] + (long)itr * r->ns);
-                u_full.push_back(foff[s] + (long)itr * nt);
-            }
         }
     }
-    u_beg.push_back((int)u_off.size());
     // Entries in (cell, shot) order: with shots in lanes the threads of one
     // wave then sample or update consecutive shots of one cell (one coalesced
     // segment) instead of cells one column apart.  Different (shot, cell)
     // pairs touch different field values, so their order is free; the
-    // receivers of one pair keep their order.
+    // receivers of one pair keep their order (the sort is stable).
     {
         std::vector<int> pe(nent);
         for (int i = 0; i < nent; ++i) pe[i] = i;
@@ -610,35 +595,23 @@ static Tables build_tables(const std::vector<ShotIn> &shots, const Geom &g, int 
         perm(e_shot);
         perm(e_cell);
         perm(e_sr);
+        perm(e_itr);
         perm(e_off);
-        const int n0 = (int)u_cell.size();
-        std::vector<int> pu(n0);
-        for (int i = 0; i < n0; ++i) pu[i] = i;
-        std::stable_sort(pu.begin(), pu.end(), [&](int x, int y) {
-            return u_cell[x] != u_cell[y] ? u_cell[x] < u_cell[y] : u_shot[x] < u_shot[y];
-        });
-        std::vector<int> ns(n0), nc(n0), nb;
-        std::vector<long> no, nf;
-        nb.reserve(n0 + 1);
-        no.reserve(u_off.size());
-        nf.reserve(u_full.size());
-        for (int i = 0; i < n0; ++i) {
-            const int j = pu[i];
-            ns[i] = u_shot[j];
-            nc[i] = u_cell[j];
-            nb.push_back((int)no.size());
-            for (int e = u_beg[j]; e < u_beg[j + 1]; ++e) {
-                no.push_back(u_off[e]);
-                nf.push_back(u_full[e]);
-            }
-        }
-        nb.push_back((int)no.size());
-        u_shot.swap(ns);
-        u_cell.swap(nc);
-        u_beg.swap(nb);
-        u_off.swap(no);
-        u_full.swap(nf);
     }
+    // unique (shot, cell) for the injection, in the same order, with the
+    // receivers of the pair in receiver order
+    std::vector<int> u_shot, u_cell, u_beg;
+    std::vector<long> u_off, u_full;
+    for (int e = 0; e < nent; ++e) {
+        if (e == 0 || e_cell[e] != e_cell[e - 1] || e_shot[e] != e_shot[e - 1]) {
+            u_shot.push_back(e_shot[e]);
+            u_cell.push_back(e_cell[e]);
+            u_beg.push_back((int)u_off.size());
+        }
+        u_off.push_back(e_off[e]);
+        u_full.push_back(foff[e_shot[e]] + (long)e_itr[e] * nt);
+    }
+    u_beg.push_back((int)u_off.size());
     const int nu = (int)u_cell.size();
 
     // receivers the lanes stencil takes over (see Tables)
@@ -646,25 +619,24 @@ static Tables build_tables(const std::vector<ShotIn> &shots, const Geom &g, int 
     std::vector<char> e_fused(nent, 0), u_fused(nu, 0);
     int nci = 0;
     if (Sp_lanes > 0) {
-        std::vector<std::map<int, int>> rows(g.Nx);
-        for (int e = 0; e < nent; ++e) {
-            const int gx = e_cell[e] / g.ld, gz = e_cell[e] % g.ld;
-            if (gx >= g.ixs && gx <= g.ixe && gz >= g.izs && gz <= g.ize) rows[gx][gz]++;
-        }
+        // per column the interior row with the most receivers (the first of
+        // equal counts): entries are in cell order, so a column's rows come
+        // in runs of increasing row
         rrow.assign(g.Nx, -1);
         rcol.assign(g.Nx, -1);
-        for (int x = 0; x < g.Nx; ++x) {
-            int best = -1, cnt = 0;
-            for (auto &kv : rows[x])
-                if (kv.second > cnt) {
-                    cnt = kv.second;
-                    best = kv.first;
-                }
-            if (best >= 0) {
-                rrow[x] = best;
-                rcol[x] = nci++;
+        std::vector<int> rcnt(g.Nx, 0);
+        for (int e = 0; e < nent;) {
+            int f = e + 1;
+            while (f < nent && e_cell[f] == e_cell[e]) ++f;
+            const int gx = e_cell[e] / g.ld, gz = e_cell[e] % g.ld;
+            if (gx >= g.ixs && gx <= g.ixe && gz >= g.izs && gz <= g.ize && f - e > rcnt[gx]) {
+                rcnt[gx] = f - e;
+                rrow[gx] = gz;
             }
+            e = f;
         }
+        for (int x = 0; x < g.Nx; ++x)
+            if (rrow[x] >= 0) rcol[x] = nci++;
         // one receiver per (shot, cell) in a slot: entries are in (cell, shot)
         // order, so a repeat follows its first
         for (int e = 0; e < nent; ++e) {
@@ -1097,7 +1069,14 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
         }
     }
     b.cells_per_launch = interior_cells(g) * S;
+    // the stencil modes the batch launches are tuned (without an adjoint pass,
+    // the forward alone)
     for (int m = 0; m < 3; ++m) {
+        if (!adjoint && m != FD_FWD) {
+            b.tile_m[m] = b.tile_m[FD_FWD];
+            for (int v = 0; v < 3; ++v) b.tile_acc[m][v] = b.tile_m[m];
+            continue;
+        }
         b.tile_m[m] = autotune_tile(m, base_args(g, coef, S, b.SS, b.CS), b.F(BatchPlan::A2),
                                     b.F(BatchPlan::A0), b.F(BatchPlan::A1));
         c.stats.tiles[m] = b.tile_m[m];
@@ -1113,7 +1092,7 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
                                                  b.F(BatchPlan::A0), b.F(BatchPlan::A1));
         for (int v = 0; v < 3; ++v) c.stats.tiles_acc[m][v] = b.tile_acc[m][v];
     }
-    b.tile = b.tile_m[FD_RECON];
+    b.tile = b.tile_m[adjoint ? FD_RECON : FD_FWD];
     c.stats.tile_tz = b.tile.tz;
     c.stats.tile_spb = b.tile.spb;
     c.stats.tile_kind = b.tile.kind;
