@@ -610,10 +610,15 @@ __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long f
     }
     float stored = val;
     if (a.fs) {
+        // free-surface mirror (propagation.c:94-100): of the rows above the
+        // surface only the three within the stencil's reach (pml-1..pml-3) are
+        // ever read again, by the next step's stencil; the reference's other
+        // mirror rows (pml-4..0) are dead stores in this layout, whose fields
+        // never leave the device, and are not written (they stay zero)
         const int kk = iz - a.pml;
         if (kk == 0)
             stored = -0.0f;
-        else if (kk <= a.pml)
+        else if (kk <= 3 && kk <= a.pml)
             lput(P2, colb + (a.pml - kk), CS, s * 4u, -val);
     }
     if (MODE == FD_RECON && (int)i == scell) {
