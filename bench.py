@@ -89,15 +89,15 @@ def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
 # 512 MiB copy; the runs and the calibration are in
 # profiles/fd_lanes_c1_traffic.md.
 C1_LANES_TRAFFIC = {
-    "forward": {(3, 8, 16, 1): 394.5e6, (3, 8, 16, 0): 442.9e6,
-                (3, 8, 20, 1): 385.2e6, (3, 8, 20, 0): 434.4e6,
-                (3, 8, 24, 1): 377.7e6, (3, 8, 24, 0): 430.2e6},
-    "reconstruction": {(3, 8, 16, 1): 356.4e6, (3, 8, 16, 0): 406.4e6,
-                       (3, 8, 20, 1): 348.1e6, (3, 8, 20, 0): 398.1e6,
-                       (3, 8, 24, 1): 339.9e6, (3, 8, 24, 0): 393.9e6},
-    "adjoint": {(3, 8, 16, 1): 411.2e6, (3, 8, 16, 0): 461.3e6,
-                (3, 8, 20, 1): 401.9e6, (3, 8, 20, 0): 452.8e6,
-                (3, 8, 24, 1): 396.5e6, (3, 8, 24, 0): 448.6e6},
+    "forward": {(3, 8, 16, 1): 380.1e6, (3, 8, 16, 0): 430.0e6,
+                (3, 8, 20, 1): 371.2e6, (3, 8, 20, 0): 421.5e6,
+                (3, 8, 24, 1): 363.7e6, (3, 8, 24, 0): 417.3e6},
+    "reconstruction": {(3, 8, 16, 1): 342.1e6, (3, 8, 16, 0): 393.6e6,
+                       (3, 8, 20, 1): 333.9e6, (3, 8, 20, 0): 385.2e6,
+                       (3, 8, 24, 1): 325.7e6, (3, 8, 24, 0): 381.1e6},
+    "adjoint": {(3, 8, 16, 1): 397.0e6, (3, 8, 16, 0): 448.5e6,
+                (3, 8, 20, 1): 387.8e6, (3, 8, 20, 0): 440.0e6,
+                (3, 8, 24, 1): 382.7e6, (3, 8, 24, 0): 435.8e6},
 }
 
 
