@@ -471,6 +471,88 @@ extern "C" void fwiedu_observe2d_from_json(Record2D *record2d, const char *jsonf
     }
 }
 
+// generate_shot_positions, observation.c:102-165: Nshot_in shots from fsx
+// every ds at depth sdepth, cut to those up to x[nx-1].  sx, sz are
+// allocated here (alloc1float); returns the shot count.
+extern "C" int generate_shot_positions(const float *x, int nx, const float *z, int nz, float fsx,
+                                       float ds, float sdepth, int Nshot_in, float **sx,
+                                       float **sz)
+{
+    if (nx < 1 || nz < 1) fail("Model domain arrays x or z are empty!\n");
+    if (fsx < x[0] || fsx > x[nx - 1]) fail("x-coordinate of first shot is out of range!\n");
+    if (sdepth < z[0] || sdepth > z[nz - 1]) fail("z-coordinate of shots is out of range!\n");
+    if (!(ds > 0.0f)) fail("<error>: the shot interval ds must be positive");
+    const int maxShots = (int)((x[nx - 1] - fsx) / ds) + 1;
+    const int n = Nshot_in < maxShots ? Nshot_in : maxShots;
+    if (n < 1) fail("<error>: no shot: Nshot = %d", Nshot_in);
+    *sx = alloc1f(n);
+    *sz = alloc1f(n);
+    for (int s = 0; s < n; ++s) {
+        (*sx)[s] = fsx + s * ds;
+        (*sz)[s] = sdepth;
+    }
+    return n;
+}
+
+// init_observe2d, observation.c:229-336: a fixed-offset spread per shot.
+// The relative offsets are those of the reference's two float loops, from
+// -offsmax up to -offsmin and from offsmin up to offsmax in steps of dtr, the
+// last negative one dropped when offsmin is 0 (that slot is the zero offset's;
+// :259-272); a shot keeps the receivers sx + offset inside [x[0], x[nx-1]]
+// at depth gdepth.  The reference sizes its offset table from
+// lroundf((offsmax - offsmin) / dtr) and reads every slot of it; where the
+// loops write fewer slots (offsmax - offsmin not a multiple of dtr) it reads
+// unwritten memory, and this version keeps only the offsets the loops give.
+extern "C" void init_observe2d(Record2D *record2d, int Nshot, float *x, int nx, float *z, int nz,
+                               float dt, int nt, float *sx, float *sz, float *src, float offsmin,
+                               float offsmax, float dtr, float gdepth, int sampleRate)
+{
+    if (!(dtr > 0.0f)) fail("<error>: the receiver interval dtr must be positive");
+    if (sampleRate < 1) fail("<error>: sampleRate must be >= 1");
+    (void)z;
+    (void)nz;
+    std::vector<float> offs;
+    for (float o = -offsmax; o <= -offsmin; o += dtr) offs.push_back(o);
+    if (offsmin == 0.0f && !offs.empty()) offs.pop_back();
+    for (float o = offsmin; o <= offsmax; o += dtr) offs.push_back(o);
+    const float fpeak = calculate_peak_frequency(src, nt, dt);
+    for (int s = 0; s < Nshot; ++s) {
+        Record2D &r = record2d[s];
+        r.shotID = s + 1;
+        r.sx = sx[s];
+        r.sz = sz[s];
+        int ntr = 0;
+        for (float o : offs) {
+            const float g = r.sx + o;
+            if (g >= x[0] && g <= x[nx - 1]) ++ntr;
+        }
+        if (ntr == 0) fail("the number of trace in shot %d is 0", s);
+        r.ntr = ntr;
+        r.gx = alloc1f(ntr);
+        r.gz = alloc1f(ntr);
+        int k = 0;
+        for (float o : offs) {
+            const float g = r.sx + o;
+            if (g >= x[0] && g <= x[nx - 1]) {
+                r.gx[k] = g;
+                r.gz[k] = gdepth;
+                ++k;
+            }
+        }
+        r.sampleRate = sampleRate;
+        r.ns = (nt - 1) / sampleRate + 1;
+        r.dt = dt * sampleRate;
+        // mean receiver spacing, summed from the last pair down (:318-323)
+        r.dtr = 0;
+        for (int i = ntr - 1; i > 0; --i) r.dtr += (r.gx[i] - r.gx[i - 1]);
+        r.dtr /= ntr - 1;
+        r.traces_p = alloc2f(r.ns, ntr);
+        r.src = alloc1f(nt);
+        memcpy(r.src, src, (size_t)nt * sizeof(float));
+        r.fpeak = fpeak;
+    }
+}
+
 // Shots in the "record" array of a JSON acquisition file.
 extern "C" int fwiedu_json_shot_count(const char *jsonfile)
 {

@@ -1,14 +1,16 @@
-/* acoustic2d_modeling: forward modeling of all shots of a JSON acquisition on
- * the GPU through libfwiedu's iso_acoustic2d_propagation (problem 0), the
- * flow of the reference driver (main/acoustic2d_modeling.c:110-388): models
- * and wavelet from float32 files, Gardner density or a density file, water
- * rows, JSON geometry, grid geometry, modeling, the SU gather
- * "shots_p_<Nshot>_<ns>_<traces>_<dt>ms.su".
+/* acoustic2d_modeling: forward modeling of all shots of an acquisition on the
+ * GPU through libfwiedu's iso_acoustic2d_propagation (problem 0), the flow of
+ * the reference driver (main/acoustic2d_modeling.c:110-388): models and
+ * wavelet from float32 files, Gardner density or a density file, water rows,
+ * the geometry from a JSON file or from parameters, grid geometry, modeling,
+ * the SU gather "shots_p_<Nshot>_<ns>_<traces>_<dt>ms.su".
  *
- * Parameters (key=value): nx nz nt dx dz dt srcfile vpfile jsonfile
- * [rhofile] [Gardner=1] [freesurface=0] [pmlThick=30] [waterdepth=0]
- * [vp_water=1500] [rho_water=1000] [sampleRate=1] [prefix=shots_p].
- * Geometry from parameters (fsx, ds, ...) is not in this build: use jsonfile.
+ * Parameters (key=value): nx nz nt dx dz dt srcfile vpfile [rhofile]
+ * [Gardner=1] [freesurface=0] [pmlThick=30] [waterdepth=0] [vp_water=1500]
+ * [rho_water=1000] [sampleRate=1] [prefix=shots_p], and the geometry:
+ * jsonfile=, or, without it, the spread of the reference's parameters (in
+ * cells; :191-246) [Nshot=99999] [fsx=1] [ds=1] [sdepth=2] [offsmin=0]
+ * [offsmax=99999] [dtr=1] [gdepth=2].
  */
 #include <math.h>
 #include <time.h>
@@ -36,7 +38,6 @@ int main(int argc, char **argv)
     const char *rhofile = arg_string("rhofile", 0);
     const char *jsonfile = arg_string("jsonfile", 0);
     const char *prefix = arg_string("prefix", 0);
-    if (!jsonfile) die("this build reads the geometry from a JSON file: specify", "jsonfile=");
     if (!gardner && !rhofile) die("didn't specify data file of density model:", "rhofile=");
 
     float *x = (float *)malloc(sizeof(float) * nx), *z = (float *)malloc(sizeof(float) * nz);
@@ -53,10 +54,29 @@ int main(int argc, char **argv)
     set_water(vp, vpw, wd, nx);
     set_water(rho, rhow, wd, nx);
 
-    const int Nshot = fwiedu_json_shot_count(jsonfile);
-    Record2D *obs = (Record2D *)calloc(Nshot, sizeof(Record2D));
+    int Nshot;
+    Record2D *obs;
+    if (jsonfile) {
+        Nshot = fwiedu_json_shot_count(jsonfile);
+        obs = (Record2D *)calloc(Nshot, sizeof(Record2D));
+        fwiedu_observe2d_from_json(obs, jsonfile, Nshot, x, nx, z, nz, dt, nt, src, sampleRate);
+    } else {
+        /* the reference's defaults, in cells, then in metres (:196-246) */
+        const int nshot_req = arg_int("Nshot", 99999, 0);
+        const float fsx = arg_float("fsx", 1.0f, 0) * dx, ds = arg_float("ds", 1.0f, 0) * dx;
+        const float sdepth = arg_float("sdepth", 2.0f, 0) * dz;
+        const float offsmin = arg_float("offsmin", 0.0f, 0) * dx;
+        const float offsmax = arg_float("offsmax", 99999.0f, 0) * dx;
+        const float dtr = arg_float("dtr", 1.0f, 0) * dx, gdepth = arg_float("gdepth", 2.0f, 0) * dz;
+        float *sx, *sz;
+        Nshot = generate_shot_positions(x, nx, z, nz, fsx, ds, sdepth, nshot_req, &sx, &sz);
+        obs = (Record2D *)calloc(Nshot, sizeof(Record2D));
+        init_observe2d(obs, Nshot, x, nx, z, nz, dt, nt, sx, sz, src, offsmin, offsmax, dtr, gdepth,
+                       sampleRate);
+        free(sx);
+        free(sz);
+    }
     Record2D *pre = (Record2D *)calloc(Nshot, sizeof(Record2D));
-    fwiedu_observe2d_from_json(obs, jsonfile, Nshot, x, nx, z, nz, dt, nt, src, sampleRate);
     create_grid_observe2d(obs, Nshot, dx, dz, nx);
     for (int s = 0; s < Nshot; ++s) {
         pre[s] = obs[s];  /* header copy: geometry pointers shared, fresh traces */

@@ -111,6 +111,12 @@ def load():
     _sig(L, "fwiedu_observe2d_from_json", None,
          [R, c_char_p, c_int, c_float_p, c_int, c_float_p, c_int, c_float, c_int, c_float_p, c_int])
     _sig(L, "fwiedu_json_shot_count", c_int, [c_char_p])
+    _sig(L, "generate_shot_positions", c_int,
+         [c_float_p, c_int, c_float_p, c_int, c_float, c_float, c_float, c_int,
+          POINTER(c_float_p), POINTER(c_float_p)])
+    _sig(L, "init_observe2d", None,
+         [R, c_int, c_float_p, c_int, c_float_p, c_int, c_float, c_int, c_float_p, c_float_p,
+          c_float_p, c_float, c_float, c_float, c_float, c_int])
     _sig(L, "fwiedu_stat", c_double, [c_char_p])
     _sig(L, "fwiedu_reset_stats", None, [])
     _sig(L, "fwiedu_release", None, [])
@@ -138,6 +144,7 @@ EXPORTED = [
     "fwiedu_server_stat", "fwiedu_server_destroy",
     "calculate_peak_frequency", "create_grid_observe2d", "write_acoustic2d_traces",
     "get_shots_number", "read_gather2d_from_sufile", "fwiedu_observe2d_from_json",
+    "generate_shot_positions", "init_observe2d",
     "fwiedu_json_shot_count",
 ]
 

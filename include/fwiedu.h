@@ -272,6 +272,18 @@ void fwiedu_observe2d_from_json(Record2D *record2d, const char *jsonfile, int Ns
                                 int sampleRate);
 /* Shots in the "record" array of a JSON acquisition file. */
 int fwiedu_json_shot_count(const char *jsonfile);
+/* generate_shot_positions (observation.h:84-89, observation.c:102-165): shots
+ * from fsx every ds at depth sdepth, at most Nshot_in of them and none past
+ * x[nx-1]; *sx, *sz allocated (alloc1float); returns the shot count. */
+int generate_shot_positions(const float *x, int nx, const float *z, int nz, float fsx, float ds,
+                            float sdepth, int Nshot_in, float **sx, float **sz);
+/* init_observe2d (observation.h:127-129, observation.c:229-336): receivers at
+ * the offsets -offsmax..-offsmin and offsmin..offsmax (step dtr) from each
+ * shot, inside the model, at depth gdepth; gx, gz, traces_p and src allocated
+ * with the SU conventions. */
+void init_observe2d(Record2D *record2d, int Nshot, float *x, int nx, float *z, int nz, float dt,
+                    int nt, float *sx, float *sz, float *src, float offsmin, float offsmax,
+                    float dtr, float gdepth, int sampleRate);
 
 /* ---- Serving loop (no counterpart in the reference) ------------------------
  * A job is one iso_acoustic2d_propagation call of problem 0 (modeling: records

@@ -3,7 +3,9 @@ on the library's C ABI, run as separate processes on files:
 
   * modeling: float32 model files, wavelet file and a JSON acquisition in,
     SU gather out; its traces are bit-identical to iso_acoustic2d_propagation
-    called in-process on the same model and geometry;
+    called in-process on the same model and geometry, and the same acquisition
+    given by the geometry parameters (fsx, ds, sdepth, offsmin, offsmax, dtr,
+    gdepth) writes the same file;
   * FWI: that gather, a smoothed start model and one iteration; the written
     model and gradient are bit-identical to fwiedu_session_iterate called
     in-process on the same inputs, and the final predicted gather is written.
@@ -93,6 +95,19 @@ def test_modeling_and_fwi_drivers(L, tmp_path):
     L.read_gather2d_from_sufile(back, os.path.join(d, su[0]).encode(), n, w.nt)
     for s in range(n):
         assert np.array_equal(traces(back[s]), traces(recs[s])), s
+
+    # the same acquisition from the reference's geometry parameters (shots
+    # every 16 cells from cell 8 at depth 1, the full spread at depth 1): the
+    # same gather file, byte for byte
+    cols = list(w.shot_columns())
+    assert cols == [8, 24, 40]
+    dp = os.path.join(d, "params")
+    os.mkdir(dp)
+    run([os.path.join(BIN, "acoustic2d_modeling")] + common +
+        ["vpfile=../vp.bin", "rhofile=../rho.bin", "srcfile=../src.bin", "Nshot=3", "fsx=8",
+         "ds=16", "sdepth=1", "offsmin=0", f"offsmax={w.nx - 1}", "dtr=1", "gdepth=1"], dp)
+    with open(os.path.join(d, su[0]), "rb") as fa, open(os.path.join(dp, su[0]), "rb") as fb:
+        assert fa.read() == fb.read()
 
     # FWI driver, one iteration, against fwiedu_session_iterate in-process
     run([os.path.join(BIN, "acoustic2d_fwi")] + common +

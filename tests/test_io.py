@@ -7,6 +7,13 @@ Pins:
     json module; ns, dt and the mean receiver spacing of init_observe2d_from_JSON
     (observation.c:509-517); fpeak of the reference's shipped Ricker wavelet
     (bin 60 of 6001 samples at 1 ms: 9.99833 Hz).
+  * geometry from parameters (generate_shot_positions, init_observe2d,
+    observation.c:102-165, 229-336): the demo's acquisition file is the
+    fixed full spread of its modeling.sh parameters (offsmin 0, offsmax 600,
+    dtr 1, gdepth 2 cells at 12.5 m) with shots every 20 cells from cell 0 at
+    depth 2: the parameter path reproduces its first 3 shots exactly; shot
+    counts cut at the model's edge; a spread cut by the model's edge; the
+    reference loops' dropped slot when offsmax is not a multiple of dtr.
   * grid geometry: floorf(coord / d) in float and the +-30 window
     (observation.c:376-418), recomputed with numpy float32.
   * SU: the byte positions of SEG-Y rev 1 (bytes 0-179: tracl 0, tracr 4,
@@ -75,6 +82,65 @@ def test_json_geometry(L):
         assert abs(r.dtr - 12.5) < 1e-5
         assert abs(r.fpeak - 9.99833) < 1e-4
     assert [recs[s].sx for s in range(3)] == [0.0, 250.0, 500.0]
+
+
+def param_records(L, nx, nz, dx, dz, fsx, ds, sdepth, nshot, offsmin, offsmax, dtr, gdepth,
+                   sample_rate=1):
+    """The modeling driver's parameter geometry (cells times dx / dz, as
+    acoustic2d_modeling.c:240-246) through the library."""
+    f32 = np.float32
+    x = (np.arange(nx) * f32(dx)).astype(f32)
+    z = (np.arange(nz) * f32(dz)).astype(f32)
+    src = ricker()
+    sx, sz = ctypes.POINTER(ctypes.c_float)(), ctypes.POINTER(ctypes.c_float)()
+    n = L.generate_shot_positions(fptr(x), nx, fptr(z), nz, f32(fsx) * f32(dx), f32(ds) * f32(dx),
+                                  f32(sdepth) * f32(dz), nshot, ctypes.byref(sx), ctypes.byref(sz))
+    recs = (Record2D * n)()
+    L.init_observe2d(recs, n, fptr(x), nx, fptr(z), nz, 0.001, src.size, sx, sz, fptr(src),
+                     f32(offsmin) * f32(dx), f32(offsmax) * f32(dx), f32(dtr) * f32(dx),
+                     f32(gdepth) * f32(dz), sample_rate)
+    return n, recs
+
+
+def test_parameter_geometry_reproduces_the_demo_acquisition(L):
+    d, jrecs, (nx, nz, dx, dz) = json_records(L, 2)
+    n, recs = param_records(L, nx, nz, dx, dz, fsx=0, ds=20, sdepth=2, nshot=3, offsmin=0,
+                            offsmax=600, dtr=1, gdepth=2, sample_rate=2)
+    assert n == 3
+    for s in range(3):
+        a, b = recs[s], jrecs[s]
+        for k in ("shotID", "sx", "sz", "ntr", "sampleRate", "ns", "dt", "dtr", "fpeak"):
+            assert getattr(a, k) == getattr(b, k), (s, k)
+        for k in ("gx", "gz"):
+            assert np.array_equal(np.ctypeslib.as_array(getattr(a, k), (a.ntr,)),
+                                  np.ctypeslib.as_array(getattr(b, k), (b.ntr,))), (s, k)
+
+
+def test_parameter_geometry_edges(L):
+    f32 = np.float32
+    # the reference's defaults (fsx 1, ds 1, 99999 shots) on 201 columns: the
+    # shots stop at the last column, 200 of them
+    n, recs = param_records(L, 201, 101, 10.0, 10.0, fsx=1, ds=1, sdepth=2, nshot=99999,
+                            offsmin=0, offsmax=99999, dtr=1, gdepth=2)
+    assert n == 200
+    assert [recs[i].sx for i in (0, 1, 199)] == [10.0, 20.0, 2000.0]
+    assert all(recs[i].ntr == 201 and recs[i].shotID == i + 1 for i in range(n))
+    # a spread of offsets 0..50 cells each side, cut by the model's edges
+    n, recs = param_records(L, 201, 101, 10.0, 10.0, fsx=30, ds=70, sdepth=2, nshot=5,
+                            offsmin=0, offsmax=50, dtr=1, gdepth=3)
+    assert n == 3 and [recs[i].sx for i in range(3)] == [300.0, 1000.0, 1700.0]
+    for i, lo, hi in ((0, 0, 80), (1, 50, 150), (2, 120, 200)):
+        gx = np.ctypeslib.as_array(recs[i].gx, (recs[i].ntr,))
+        assert np.array_equal(gx, (np.arange(lo, hi + 1) * f32(10.0)).astype(f32)), i
+        assert np.all(np.ctypeslib.as_array(recs[i].gz, (recs[i].ntr,)) == f32(30.0))
+        assert recs[i].dtr == f32(10.0)
+    # offsets 0..2.5 cells in steps of 1: the first loop gives -2.5, -1.5,
+    # -0.5 and its last slot goes to the zero offset (observation.c:259-272),
+    # so the spread is -2.5, -1.5, 0, 1, 2 cells from the shot
+    n, recs = param_records(L, 201, 101, 10.0, 10.0, fsx=100, ds=1, sdepth=2, nshot=1,
+                            offsmin=0, offsmax=2.5, dtr=1, gdepth=2)
+    gx = np.ctypeslib.as_array(recs[0].gx, (recs[0].ntr,))
+    assert np.array_equal(gx, np.array([975, 985, 1000, 1010, 1020], f32))
 
 
 def test_create_grid_observe2d(L):
