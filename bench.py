@@ -224,6 +224,30 @@ def serving_leg(args):
     return out
 
 
+def large_leg(L, args):
+    """Stencil launches on a working set beyond the 256 MB Infinity Cache: the
+    400 x 1200 multiscale workload (nt 5000 cut to --large-nt steps: a launch's
+    work and bytes do not depend on nt), 256 shots in one batch, one warm-up
+    gradient (launch-shape tuning) and one timed gradient.  Evidence for the
+    stencil's HBM fraction at this size; not the bench value."""
+    w = synth.workload("c2", nt=args.large_nt)
+    log(f"large leg {w.name}: {w.nz}x{w.nx} nt={w.nt} shots={w.nshot}")
+    obs = observed(L, w)
+    times, st, _ = timed_gradients(L, w, obs, 1, 1, 0, 7)
+    modes = stencil_modes(w, st)
+    for name, m in modes.items():
+        log(f"large leg {name}: {m['avg_launch_us']:.1f} us, {m['frac']:.3f} of HBM peak")
+    dom = max(modes, key=lambda m: modes[m]["time_share"])
+    return {"workload": f"{w.name}: {w.nz}x{w.nx} @ {w.dx} m, {w.nshot} shots, free surface "
+                        f"{w.freesurface}, gradient of nt {w.nt} (a launch's bytes and time do "
+                        f"not depend on nt)",
+            "working_set_bytes_per_launch": modes[dom]["bytes_per_launch"],
+            "shots_per_launch": int(st["fd_shots_per_launch"]),
+            "stencil": stencil_shape(st), "stencil_modes": modes,
+            "roofline": {"bound": "hbm", "achieved": modes[dom]["achieved"], "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": modes[dom]["frac"], "mode": dom}}
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -309,6 +333,9 @@ def main():
                     help="band frequency of the adjoint-source filter (0 = no filter)")
     ap.add_argument("--filter", type=int, default=0, choices=(0, 1),
                     help="with --freq: 0 low-pass, 1 band-pass")
+    ap.add_argument("--large-nt", type=int, default=400,
+                    help="time steps of the 400x1200, 256-shot stencil leg beyond the Infinity "
+                         "Cache (0 = skip)")
     ap.add_argument("--lanes", default="",
                     help="force the shots-in-lanes stencil shape WPB,ZRUN,PD,XCD[,LDS] "
                          "(default: autotuned)")
@@ -378,6 +405,7 @@ def main():
         log(f"plain: {ps:.2f} shots/s")
 
     serve = serving_leg(args) if args.serving > 0 else None
+    large = large_leg(L, args) if args.large_nt > 0 and not args.lanes else None
 
     if args.copy_mb:
         copy_calibration(args.copy_mb)
@@ -427,6 +455,7 @@ def main():
                      "avg_launch_us": modes[dom]["avg_launch_us"],
                      "launches_timed": modes[dom]["launches_timed"]},
         "stencil_modes": modes,
+        "beyond_infinity_cache": large,
         "cpu_baseline": cpu,
         "device_bytes": st["device_bytes"],
     }
