@@ -100,3 +100,32 @@ def test_invalid_arguments_exit_like_err(pml, fs, problem, msg):
     assert r.returncode == 1, (r.stdout, r.stderr)
     assert msg in r.stderr
     assert "returned" not in r.stdout
+
+
+BAD_GEOMETRY = """
+import ctypes
+import numpy as np
+from fwiedu import lib as flib
+from fwiedu.records import fptr
+L = flib.load()
+x = (np.arange(201) * np.float32(10.0)).astype(np.float32)
+z = (np.arange(101) * np.float32(10.0)).astype(np.float32)
+sx, sz = ctypes.POINTER(ctypes.c_float)(), ctypes.POINTER(ctypes.c_float)()
+L.generate_shot_positions(fptr(x), 201, fptr(z), 101, {fsx}, 10.0, {sdepth}, 5,
+                          ctypes.byref(sx), ctypes.byref(sz))
+print('returned')
+"""
+
+
+@pytest.mark.parametrize("fsx,sdepth,msg", [
+    (2001.0, 20.0, "x-coordinate of first shot is out of range"),
+    (-10.0, 20.0, "x-coordinate of first shot is out of range"),
+    (100.0, 1001.0, "z-coordinate of shots is out of range"),
+])
+def test_shot_positions_out_of_the_model_exit_like_err(fsx, sdepth, msg):
+    """generate_shot_positions (observation.c:138-148) ends the program with
+    the reference's messages for shots outside the model."""
+    r = _run_child(BAD_GEOMETRY.format(fsx=fsx, sdepth=sdepth))
+    assert r.returncode == 1, (r.stdout, r.stderr)
+    assert msg in r.stderr
+    assert "returned" not in r.stdout
