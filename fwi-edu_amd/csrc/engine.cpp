@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <map>
+#include <thread>
 #include <tuple>
 
 #include "engine.hpp"
@@ -65,6 +66,7 @@ void ensure_ready()
         fail("<error>: no HIP device available (libfwiedu runs the FWI path on the GPU only)");
     HIPCHECK(hipGetDevice(&c.device));
     HIPCHECK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&c.cst, hipStreamNonBlocking));
     c.ready = true;
 }
 
@@ -923,6 +925,52 @@ static void rec_d2h(Record2D *r, const float *src)
     d2h(r->traces_p[0], src, (size_t)r->ntr * r->ns * sizeof(float));
 }
 
+// Copies between a batch's device records and the caller's pageable record
+// arrays, on the context's copy stream from a helper thread, so that they run
+// while the calling thread enqueues, and the device runs, the stencil
+// launches: the observed records go up during the forward pass (which does not
+// read them) and the adjoint sources come down during the adjoint pass (which
+// only reads them).  The copies start once the work enqueued on the compute
+// stream before `after` is done; join() returns when they have completed.
+// Not used while the time loops are replayed from HIP graphs: capturing a loop
+// on the compute stream beside another thread's copies makes the capture fail.
+struct RecordCopies {
+    struct Piece {
+        void *dst;
+        const void *src;
+        size_t bytes;
+    };
+    std::thread th;
+    hipEvent_t after = nullptr;
+
+    void start(Ctx &c, std::vector<Piece> pieces, hipMemcpyKind kind)
+    {
+        join();
+        HIPCHECK(hipEventCreateWithFlags(&after, hipEventDisableTiming));
+        HIPCHECK(hipEventRecord(after, c.st));
+        for (const Piece &p : pieces)
+            (kind == hipMemcpyHostToDevice ? c.stats.h2d_bytes : c.stats.d2h_bytes) +=
+                (double)p.bytes;
+        const int dev = c.device;
+        hipStream_t cs = c.cst;
+        hipEvent_t ev = after;
+        th = std::thread([dev, cs, ev, kind, pieces] {
+            HIPCHECK(hipSetDevice(dev));
+            HIPCHECK(hipStreamWaitEvent(cs, ev, 0));
+            for (const Piece &p : pieces)
+                HIPCHECK(hipMemcpyAsync(p.dst, p.src, p.bytes, kind, cs));
+            HIPCHECK(hipStreamSynchronize(cs));
+        });
+    }
+    void join()
+    {
+        if (th.joinable()) th.join();
+        if (after) HIPCHECK(hipEventDestroy(after));
+        after = nullptr;
+    }
+    ~RecordCopies() { join(); }
+};
+
 // ---------------------------------------------------------------------------
 // One batch of S shots on the device.
 
@@ -1624,12 +1672,26 @@ void propagate_core(const Call &P)
                 blen[k] = (long)P.obs[sidx].ntr * P.obs[sidx].ns;
             }
             float *obsd = nullptr;
+            RecordCopies up, down;
             if (problem != 0) {
                 obsd = c.obs.as<float>(b.rec_total + 64);
-                if (P.obs_dev)
+                if (P.obs_dev) {
                     copy_pieces(obsd, b.roff.data(), P.obs_dev, boff.data(), blen.data(), nb);
-                else
+                } else if (c.graphs) {
+                    // a time loop captured into a graph must not run beside
+                    // another thread's copies: these go first, on the stream
                     for (int k = 0; k < nb; ++k) rec_h2d(obsd + b.roff[k], &P.obs[grp[b0 + k]]);
+                } else {
+                    // uploaded while the forward pass runs; joined before the
+                    // adjoint sources read them
+                    std::vector<RecordCopies::Piece> pc(nb);
+                    for (int k = 0; k < nb; ++k) {
+                        const Record2D &r = P.obs[grp[b0 + k]];
+                        pc[k] = {obsd + b.roff[k], r.traces_p[0],
+                                 (size_t)r.ntr * r.ns * sizeof(float)};
+                    }
+                    up.start(c, std::move(pc), hipMemcpyHostToDevice);
+                }
             }
             const long nacc = b.nacc;
             float *gb = nullptr, *pb = nullptr;
@@ -1649,6 +1711,7 @@ void propagate_core(const Call &P)
                     for (int k = 0; k < nb; ++k) rec_d2h(recs[k], b.rec + b.roff[k]);
                 continue;
             }
+            up.join();
             adjoint_sources(recs.data(), nb, P.vp, P.freq, P.filter, b.rec, obsd, b.rec,
                             b.roff.data());
             if (mis) {
@@ -1657,8 +1720,17 @@ void propagate_core(const Call &P)
                     launch_sum_sq(b.rec + b.roff[k], (long)recs[k]->ntr * recs[k]->ns, scr,
                                   mis + 2 * grp[b0 + k], c.st);
             }
-            if (P.records_to_host)
+            if (P.records_to_host && c.graphs) {
                 for (int k = 0; k < nb; ++k) rec_d2h(recs[k], b.rec + b.roff[k]);
+            } else if (P.records_to_host) {
+                // downloaded while the adjoint pass runs (it only reads
+                // b.rec); `down` is joined when this batch's scope ends
+                std::vector<RecordCopies::Piece> pc(nb);
+                for (int k = 0; k < nb; ++k)
+                    pc[k] = {recs[k]->traces_p[0], b.rec + b.roff[k],
+                             (size_t)recs[k]->ntr * recs[k]->ns * sizeof(float)};
+                down.start(c, std::move(pc), hipMemcpyDeviceToHost);
+            }
             if (P.pre_dev) copy_pieces(P.pre_dev, boff.data(), b.rec, b.roff.data(), blen.data(), nb);
             if (problem == 2) continue;
             bool full = false;

@@ -71,6 +71,7 @@ struct Ctx {
     bool ready = false;
     int device = 0;
     hipStream_t st = nullptr;
+    hipStream_t cst = nullptr;    // host-copy stream (records to and from the caller's arrays)
     int batch_req = 0;
     double memfrac = 0.85;
     int ktiming = 0;

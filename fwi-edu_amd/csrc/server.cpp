@@ -314,6 +314,7 @@ extern "C" void fwiedu_server_destroy(void *server)
             sync();
             release_buffers(*w.ctx);
             (void)hipStreamDestroy(w.ctx->st);
+            (void)hipStreamDestroy(w.ctx->cst);
         }
         delete w.ctx;
     }
