@@ -206,6 +206,40 @@ def test_batched_equals_plain_and_deterministic(L, oracles):
         assert np.array_equal(r[0], res[0][0]) and np.array_equal(r[1], res[0][1])
 
 
+def test_host_arrays_equal_resident_session(L):
+    """A gradient on the caller's host arrays (observed records uploaded and
+    adjoint sources downloaded by a helper thread while the stencils run) is
+    bit-identical to the same gradient of a resident session (records already
+    on the device, no helper copies), in one batch and in three batches that
+    reuse the record buffers (2 + 2 + 1 shots)."""
+    w = synth.workload("tiny", nshot=5, freesurface=1, shot_depth_cells=1, nx=128, nt=400)
+    say("case: host arrays vs session, tiny 128 columns, 5 shots, batches of 2 and 5")
+    obs = observed(L, w, True)
+    vp, rho = Array2D(w.vp_start), Array2D(w.rho_start)
+    out = {}
+    try:
+        for b in (2, 0):
+            L.fwiedu_set_batch(b)
+            pre, g = gradient(L, w, obs, True)
+            out[f"host{b}"] = (recs(pre), g)
+            sess = L.fwiedu_session_create(vp.pp, rho.pp, obs.recs, w.nshot, w.nz, w.nx, w.dx,
+                                           w.dz, w.dt, w.nt, w.pml, w.freesurface, w.freq,
+                                           float(w.filter), w.precondition)
+            gs = Array2D(np.zeros((w.nx, w.nz), np.float32))
+            try:
+                L.fwiedu_session_gradient(sess, gs.pp)
+            finally:
+                L.fwiedu_session_destroy(sess)
+            out[f"session{b}"] = gs.a.copy()
+    finally:
+        L.fwiedu_set_batch(0)
+    assert np.any(out["host0"][1] != 0)
+    for b in (2, 0):
+        assert np.array_equal(out[f"host{b}"][1], out[f"session{b}"]), b
+    assert np.array_equal(out["host2"][0], out["host0"][0])
+    assert np.array_equal(out["host2"][1], out["host0"][1])
+
+
 def test_shot_windows_grouping(L, oracles):
     """Shots whose receiver spans give different simulation windows
     (create_grid_observe2d's +-30-cell window) run as separate groups; the
