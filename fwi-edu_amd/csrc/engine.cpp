@@ -932,8 +932,8 @@ static void rec_d2h(Record2D *r, const float *src)
 // read them) and the adjoint sources come down during the adjoint pass (which
 // only reads them).  The copies start once the work enqueued on the compute
 // stream before `after` is done; join() returns when they have completed.
-// Not used while the time loops are replayed from HIP graphs: capturing a loop
-// on the compute stream beside another thread's copies makes the capture fail.
+// Not used while the time loops are replayed from HIP graphs: a loop captured
+// on the compute stream while another thread copied has ended with a HIP error.
 struct RecordCopies {
     struct Piece {
         void *dst;
