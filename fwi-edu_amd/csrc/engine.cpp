@@ -996,7 +996,7 @@ struct BatchState {
     __half *h16 = nullptr;        // FP16 slots: slot k of shot s, cell i at k*KS16 + s*SS16 + i*CS16
     long KS16 = 0, SS16 = 0, CS16 = 1;
     float *ckinv = nullptr;       // [nck][Sp] inverse scales
-    unsigned *ckmx = nullptr;     // [Sp] scratch
+    unsigned *ckmx = nullptr;     // [ck_mx_words(Sp)] scratch
     int ring0 = 0;                // first imaging-ring field
     int *cell_dev = nullptr;
     float *wav_dev = nullptr;
@@ -1070,7 +1070,7 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
             b.SS16 = (long)sc.nck * g.fsz;
             b.CS16 = 1;
         }
-        char *aux = c.ckaux.as<char>(((size_t)sc.nck + 1) * b.Sp * 4 + 256);
+        char *aux = c.ckaux.as<char>(((size_t)sc.nck * b.Sp + ck_mx_words(b.Sp)) * 4 + 256);
         b.ckinv = reinterpret_cast<float *>(aux);
         b.ckmx = reinterpret_cast<unsigned *>(aux + (size_t)sc.nck * b.Sp * 4);
     }

@@ -1944,28 +1944,58 @@ void launch_fill_u32(unsigned *p, int n, unsigned v, hipStream_t st)
 }
 
 // max |u| per shot, as the bit pattern of a non-negative float (ordered like
-// the float), into mx[s] (cleared to 0 beforehand).  lanes: a wave reads one
-// cell of 64 shots; otherwise block y is one shot and the block strides cells.
-__global__ void ck_maxabs_kernel(const float *f, long SS, long CS, long ncell, int nshot, int lanes,
-                                 unsigned *mx)
+// the float).  Each block reduces its cells in registers and LDS and takes its
+// maximum into copy blockIdx.x % CK_MX_COPIES of the shot words (mxs words
+// per copy, cleared to 0 beforehand), so that the blocks' atomics meet rarely;
+// ck_maxabs_fold_kernel then takes the maximum over the copies into copy 0.
+// The maximum is exact whatever the order.  lanes: a wave reads one cell of 64
+// shots at a time; otherwise block y is one shot and the block strides cells.
+__global__ __launch_bounds__(256) void ck_maxabs_kernel(const float *f, long SS, long CS,
+                                                        long ncell, int nshot, int lanes, long mxs,
+                                                        unsigned *mx)
 {
-    float m = 0.0f;
-    int s;
+    __shared__ float red[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned *dst = mx + (long)(blockIdx.x % CK_MX_COPIES) * mxs;
+    float m0 = 0.0f, m1 = 0.0f;
     if (lanes) {
-        s = blockIdx.y * 64 + (threadIdx.x & 63);
-        const long w0 = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-        const long nw = (long)gridDim.x * (blockDim.x >> 6);
-        for (long i = w0; i < ncell; i += nw) m = fmaxf(m, fabsf(f[s * SS + i * CS]));
+        const int s = blockIdx.y * 64 + lane;
+        const long nw = (long)gridDim.x * 4;
+        long i = (long)blockIdx.x * 4 + w;
+        // two loads in flight per wave
+        for (; i + nw < ncell; i += 2 * nw) {
+            m0 = fmaxf(m0, fabsf(f[s * SS + i * CS]));
+            m1 = fmaxf(m1, fabsf(f[s * SS + (i + nw) * CS]));
+        }
+        if (i < ncell) m0 = fmaxf(m0, fabsf(f[s * SS + i * CS]));
+        red[w][lane] = fmaxf(m0, m1);
+        __syncthreads();
+        if (w == 0) {
+            const float m = fmaxf(fmaxf(red[0][lane], red[1][lane]), fmaxf(red[2][lane], red[3][lane]));
+            if (s < nshot) atomicMax(&dst[s], __float_as_uint(m));
+        }
     } else {
-        s = blockIdx.y;
+        const int s = blockIdx.y;
         for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < ncell;
              i += (long)gridDim.x * blockDim.x)
-            m = fmaxf(m, fabsf(f[s * SS + i * CS]));
-        // wave maximum, then one atomic per wave
-        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-        if ((threadIdx.x & 63) != 0) return;
+            m0 = fmaxf(m0, fabsf(f[s * SS + i * CS]));
+        for (int off = 32; off > 0; off >>= 1) m0 = fmaxf(m0, __shfl_xor(m0, off));
+        if (lane == 0) red[w][0] = m0;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const float m = fmaxf(fmaxf(red[0][0], red[1][0]), fmaxf(red[2][0], red[3][0]));
+            atomicMax(&dst[s], __float_as_uint(m));
+        }
     }
-    if (s < nshot) atomicMax(&mx[s], __float_as_uint(m));
+}
+
+__global__ void ck_maxabs_fold_kernel(unsigned *mx, long mxs, int nshot)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nshot) return;
+    unsigned m = mx[s];
+    for (int k = 1; k < CK_MX_COPIES; ++k) m = max(m, mx[(long)k * mxs + s]);
+    mx[s] = m;
 }
 
 __device__ __forceinline__ float ck_scale_of(unsigned bits)
@@ -2016,7 +2046,7 @@ __global__ void ck_unpack_kernel(const __half *h, long SS16, long CS16, float *f
 static dim3 ck_grid(long ncell, int nshot, int lanes, int reduce)
 {
     if (lanes) {
-        const long nb = reduce ? std::min<long>((ncell + 3) / 4, 4096) : (ncell + 3) / 4;
+        const long nb = reduce ? std::min<long>((ncell + 3) / 4, 1024) : (ncell + 3) / 4;
         return dim3((unsigned)nb, (unsigned)((nshot + 63) / 64));
     }
     const long nb = reduce ? std::min<long>((ncell + 255) / 256, 512) : (ncell + 255) / 256;
@@ -2026,9 +2056,11 @@ static dim3 ck_grid(long ncell, int nshot, int lanes, int reduce)
 void launch_ck_pack(const float *f, long SS, long CS, __half *h, long SS16, long CS16, long ncell,
                     int nshot, int lanes, unsigned *mx, float *inv, hipStream_t st)
 {
-    launch_fill_u32(mx, nshot, 0u, st);
+    const long mxs = ck_mx_stride(nshot);
+    launch_fill_u32(mx, (int)ck_mx_words(nshot), 0u, st);
     ck_maxabs_kernel<<<ck_grid(ncell, nshot, lanes, 1), 256, 0, st>>>(f, SS, CS, ncell, nshot,
-                                                                        lanes, mx);
+                                                                        lanes, mxs, mx);
+    ck_maxabs_fold_kernel<<<(nshot + 63) / 64, 64, 0, st>>>(mx, mxs, nshot);
     ck_pack_kernel<<<ck_grid(ncell, nshot, lanes, 0), 256, 0, st>>>(f, SS, CS, h, SS16, CS16,
                                                                       ncell, nshot, lanes, mx, inv);
 }

@@ -213,8 +213,13 @@ void launch_fill_u32(unsigned *p, int n, unsigned v, hipStream_t st);
 // ---- FP16 checkpoints ------------------------------------------------------
 // Pack the FP32 field f of nshot shots (strides SS, CS, ncell cells each) into
 // FP16 h (strides SS16, CS16) with a per-shot power-of-two scale; inv[s] gets
-// the inverse scale.  mx: nshot words of scratch.  lanes selects the thread
-// mapping of the layout (shot fastest or cell fastest).
+// the inverse scale.  mx: ck_mx_words(nshot) words of scratch.  lanes selects
+// the thread mapping of the layout (shot fastest or cell fastest).
+// The per-shot maxima are reduced into CK_MX_COPIES copies of the shot words,
+// each copy on its own cache lines, so that the blocks' atomics spread out.
+constexpr int CK_MX_COPIES = 32;
+inline long ck_mx_stride(int nshot) { return (long)(nshot + 31) / 32 * 32; }
+inline long ck_mx_words(int nshot) { return CK_MX_COPIES * ck_mx_stride(nshot); }
 void launch_ck_pack(const float *f, long SS, long CS, __half *h, long SS16, long CS16, long ncell,
                     int nshot, int lanes, unsigned *mx, float *inv, hipStream_t st);
 void launch_ck_unpack(const __half *h, long SS16, long CS16, float *f, long SS, long CS,
