@@ -145,7 +145,8 @@ def stencil_modes(w, st):
 KERNELS = {0: "fd_kernel (square tiles, per-shot arenas)",
            1: "fd_march_kernel (column march, per-shot arenas)",
            2: "fd_lanes_kernel (shots in lanes)",
-           3: "fd_lanes_lds_kernel (shots in lanes, LDS row exchange)"}
+           3: "fd_lanes_lds_kernel (shots in lanes, LDS row exchange)",
+           4: "fd_ring_kernel (shots in lanes, LDS row exchange, register ring)"}
 
 
 def tile_shape(st, m):

@@ -416,7 +416,11 @@ extern "C" void fwiedu_set_lanes(int wpb, int zrun, int pd, int lds)
     Ctx &c = ctx();
     const bool zok = zrun == 8 || zrun == 16 || zrun == 32 || zrun == 64 || zrun == 128 ||
                      zrun == 256;
-    if (lds) {
+    if (lds == 2) {
+        if (wpb != 8 || zrun < 8 || zrun > 256 || pd < 2 || pd > 5)
+            fail("<error>: fwiedu_set_lanes (register ring): 8 columns per block, 8 to 256 rows "
+                 "per block, 2 to 5 rows of loads in flight");
+    } else if (lds) {
         if ((wpb != 4 && wpb != 8 && wpb != 16) || zrun < 8 || zrun > 256 || pd < 1 || pd > 4)
             fail("<error>: fwiedu_set_lanes (LDS row exchange): 4, 8 or 16 columns per block, "
                  "8 to 256 rows per block, 1 to 4 rows of loads in flight");
@@ -429,7 +433,7 @@ extern "C" void fwiedu_set_lanes(int wpb, int zrun, int pd, int lds)
     c.tile_req.remap = rm;
     c.tile_req.zfast = zf;
     c.tile_req.tz = 64;
-    c.tile_req.kind = lds ? 3 : 2;
+    c.tile_req.kind = lds == 2 ? 4 : (lds ? 3 : 2);
     c.tile_req.wpb = wpb;
     c.tile_req.zrun = zrun;
     c.tile_req.pd = pd;

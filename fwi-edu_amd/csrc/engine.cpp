@@ -189,7 +189,7 @@ void build_coefs(const Geom &g, const float *vpw, const float *rhow, float fpeak
 void fd_launch(int mode, const FdArgs &a, long cells, FdTile tile)
 {
     Ctx &c = ctx();
-    if (tile.kind == 3 && mode != FD_RECON && a.acc_on && !a.accstage)
+    if (tile.kind >= 3 && mode != FD_RECON && a.acc_on && !a.accstage)
         fail("<error>: internal: accumulating lanes stencil launch without its staging buffer");
     // every ktiming-th launch of each stencil mode, outside graph capture
     const bool timed = c.ktiming > 0 && !c.capturing && mode >= 0 && mode < 3 &&
@@ -385,6 +385,22 @@ FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur,
                         t.pd = pd;
                         cand.push_back(t);
                     }
+        // the register-ring LDS stencil: runs of whole unrolled groups (10
+        // rows with 2 or 3 rows of loads in flight, 12 with 4 or 5)
+        for (int pd : {2, 3, 4, 5})
+            for (int k : {2, 3})
+                for (int rm : {0, 1})
+                    for (int order : {0, 2}) {
+                        FdTile t;
+                        t.tz = 64;
+                        t.kind = 4;
+                        t.zfast = order;
+                        t.wpb = 8;
+                        t.zrun = k * ring_unroll(pd);
+                        t.remap = rm;
+                        t.pd = pd;
+                        cand.push_back(t);
+                    }
     }
     for (int tz : {64, 32})
         for (int spb : {1, 2, 4, 8})
@@ -430,7 +446,7 @@ FdTile autotune_acc_tile(int mode, int accv, const FdTile &mode_tile, const FdAr
                          float *out, const float *cur, const float *prev)
 {
     Ctx &c = ctx();
-    if (c.tile_req.tz > 0 || mode_tile.kind != 3 || mode == FD_RECON || !base.acc ||
+    if (c.tile_req.tz > 0 || mode_tile.kind < 3 || mode == FD_RECON || !base.acc ||
         !base.accstage)
         return mode_tile;
     const auto key = std::make_tuple(base.Nx, base.ld, base.ixs, base.ixe, base.izs, base.ize,
@@ -451,7 +467,12 @@ FdTile autotune_acc_tile(int mode, int accv, const FdTile &mode_tile, const FdAr
     std::vector<FdTile> cand;
     for (int zrun : {16, 20, 24, 32})
         for (int pd : {2, 4}) {
+            // the register-ring stencil has no accumulation epilogue: its
+            // accumulating launches run the LDS stencil of the same block
+            // shape and order
             FdTile t = mode_tile;
+            t.kind = 3;
+            t.wpb = 8;
             t.zrun = zrun;
             t.pd = pd;
             cand.push_back(t);
@@ -1198,7 +1219,7 @@ static void forward_loop(BatchState &b, bool save_ckpt, bool precond)
     a.acc_stride = b.ASS;
     a.acc_cell = b.ACS;
     // the LDS stencil samples the receivers of its stage slots itself
-    const bool fused = b.tb.nci > 0 && b.tile_m[FD_FWD].kind == 3;
+    const bool fused = b.tb.nci > 0 && b.tile_m[FD_FWD].kind >= 3;
     if (fused) {
         a.rrow = b.tb.rrow;
         a.rcol = b.tb.rcol;
@@ -1210,7 +1231,7 @@ static void forward_loop(BatchState &b, bool save_ckpt, bool precond)
     // the LDS stencil stages the pseudo-Hessian terms of ACC_STAGES steps and
     // adds them with the next one's (one pass over the FP64 accumulator
     // instead of ACC_STAGES + 1)
-    const bool defer = precond && b.accstage && b.tile_m[FD_FWD].kind == 3;
+    const bool defer = precond && b.accstage && b.tile_m[FD_FWD].kind >= 3;
     a.accstage = b.accstage;
     a.accstage_stride = b.accstage_stride;
     int staged = 0;
@@ -1290,7 +1311,7 @@ static void adjoint_loop(BatchState &b)
     // the LDS stencil of step it injects the adjoint sources of its stage
     // slots for step it - 1 into its output; step nt - 1 and the left-over
     // receivers keep the separate injection
-    const bool fused = b.tb.nci > 0 && b.tile_m[FD_ADJ].kind == 3;
+    const bool fused = b.tb.nci > 0 && b.tile_m[FD_ADJ].kind >= 3;
     if (fused) {
         launch_stage_gather(b.adjfull ? b.tb.injfull_slot : b.tb.inj_slot,
                             b.adjfull ? b.adjfull : b.rec, b.stage, b.nslot, nt, c.st);
@@ -1300,7 +1321,7 @@ static void adjoint_loop(BatchState &b)
         ad.nslot = b.nslot;
     }
     // imaging terms staged as in the forward (the last imaging step is it = 0)
-    const bool defer = b.accstage && b.tile_m[FD_ADJ].kind == 3;
+    const bool defer = b.accstage && b.tile_m[FD_ADJ].kind >= 3;
     ad.accstage = b.accstage;
     ad.accstage_stride = b.accstage_stride;
     int staged = 0;
@@ -1483,6 +1504,16 @@ static size_t per_shot_bytes(const Geom &g, const Schedule &sc, bool adjoint, bo
 // multiple of 64; it is used when at least 48 shots share a launch (below
 // that a third of the lanes or more would idle) unless a layout or a stencil
 // shape is forced.
+// The lanes stencils address a field through 32-bit buffer offsets and form
+// row and column byte offsets in signed 32-bit arithmetic: one field of a
+// padded batch (fsz cells x Sp lanes x 4 bytes) must stay below 2 GiB.  The
+// largest multiple of 64 shots that does (0 if not even 64 do).
+long lanes_max_shots(long fsz)
+{
+    if (fsz <= 0) return 0;
+    return (0x7FFFFFFFL / (fsz * 4)) / 64 * 64;
+}
+
 static int choose_batch(int n, size_t per_shot, long fsz, bool &lanes)
 {
     Ctx &c = ctx();
@@ -1520,13 +1551,11 @@ static int choose_batch(int n, size_t per_shot, long fsz, bool &lanes)
             lanes = false;
     }
     if (lanes) {
-        // the lanes stencils address a field with 32-bit buffer offsets: one
-        // field of the padded batch must stay below 4 GiB
-        const long max_lanes = (0xFFFFFFFFL / (fsz * 4)) / 64 * 64;
+        const long max_lanes = lanes_max_shots(fsz);
         if (max_lanes < 64) {
             if (c.tile_req.tz > 0 && c.tile_req.kind >= 2)
                 fail("<error>: the shots-in-lanes stencil cannot address this grid (one field of 64 "
-                     "shots exceeds 4 GiB)");
+                     "shots reaches 2 GiB)");
             lanes = false;
         } else if ((S + 63) / 64 * 64 > max_lanes) {
             S = max_lanes;

@@ -14,6 +14,8 @@
 #include <float.h>
 #include <math.h>
 
+#include <type_traits>
+
 #pragma clang fp contract(off)
 
 namespace fwiedu {
@@ -1053,10 +1055,255 @@ static void launch_lanes_lds_t(const FdArgs &a0, FdTile tl, hipStream_t st)
         launch_lanes_lds_w<MODE, 4>(a, grid, tl.pd, st);
 }
 
+// Shots in lanes, LDS row exchange, register ring (launch kind 4).  The same
+// block shape and arithmetic as fd_lanes_lds_kernel with WPB = 8 (8 adjacent
+// columns per block, one wave per column, the row's x-neighbours through a
+// double-buffered LDS row, one barrier per row), built so that a row costs
+// little beside its arithmetic:
+//  * the rows a wave holds (the 7-row z-stencil of its column and the D rows
+//    of loads in flight) live in a ring of U registers indexed by
+//    (row - first row) mod U, and the row loop is unrolled U rows deep, so
+//    every value keeps its register from load to last use and no register is
+//    copied at the loop's back edge (a copy of a value whose load is in flight
+//    waits for it); U is even (the LDS buffer of a row is its parity) and
+//    at least D + 7 (the slot a load fills is free when it is issued);
+//  * every field access of a wave goes through a buffer resource based at its
+//    own column (one for p1 three rows down, whose rows are the z+3 loads), so
+//    all loads of a row share one scalar row offset, advanced by one add per
+//    row; the store's resource is based D rows up so that it uses the same
+//    offset;
+//  * the coefficients of row r + 1 (one 32-byte scalar load of the
+//    cell-interleaved copy) come from a group base plus an immediate offset,
+//    loaded during row r;
+//  * blocks whose rows and columns hold no free-surface row, source or
+//    receiver take a loop with none of those tests; the others run the
+//    epilogue of lanes_update, as fd_lanes_lds_kernel does.
+// Rows past a run's end (the last group of a run) still load, publish and
+// pass the barrier, computing nothing.
+template <int D>
+struct RingShape {
+    static constexpr int U = ((D + 8) / 2) * 2;
+    static_assert(U >= D + 7 && U % 2 == 0, "ring too small");
+};
+
+__device__ __forceinline__ frsrc col_rsrc(const float *base, long off_floats, long bytes_left)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base + off_floats), (short)0,
+                                             (int)(unsigned)bytes_left, 0x00020000);
+}
+
+template <int MODE, int D>
+__global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
+{
+    constexpr int WPB = 8;
+    constexpr int NC = WPB + 8;  // 6 halo columns and 2 spare (see below)
+    constexpr int U = RingShape<D>::U;
+    __shared__ float row[2][NC][64];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int b = blockIdx.x;
+    if (a.remap) {
+        const int per = (int)((gridDim.x + 7) >> 3);
+        b = (b & 7) * per + (b >> 3);
+        if (b >= a.lz_nblk) return;  // the whole block
+    }
+    int zr, sg, xg;
+    lanes_block(a, b, zr, sg, xg);
+    const int x0 = a.ixs + xg * WPB;
+    const int ix = x0 + w;
+    const bool active = ix <= a.ixe;
+    const int iz0 = a.izs + zr * a.lz_run;
+    const int iz1 = min(a.ize, iz0 + a.lz_run - 1);
+    const int nrow = iz1 - iz0 + 1;
+    const unsigned s = (unsigned)(sg * 64 + lane);
+    const unsigned vo = s * 4u;
+    const long CS = a.cell_stride;
+    const int ld = a.ld;
+    const long fsz = (long)a.Nx * ld;
+    const long fbytes = fsz * CS * 4;
+    const int rs4 = (int)(CS * 4);
+    const long colb = (long)min(ix, a.Nx - 1) * ld;
+    const long cb = colb * CS * 4;
+    // column resources: row r of the column at byte offset r * rs4 (P1z: row
+    // r + 3; P2: row r - D, so that the store of row r uses the load offset
+    // of row r + D)
+    const frsrc P1c = col_rsrc(a.cur, colb * CS, fbytes - cb);
+    const frsrc P1z = col_rsrc(a.cur, (colb + 3) * CS, fbytes - cb - 3L * rs4);
+    const frsrc P0c = col_rsrc(a.prev, colb * CS, fbytes - cb);
+    const frsrc P2c = col_rsrc(a.out, (colb - D) * CS, fbytes - cb + (long)D * rs4);
+    // halo column of this wave: waves 0..5 carry the six halo columns (LDS
+    // columns 0..2 and WPB+3..WPB+5); waves 6 and 7 run the same loads
+    // through a resource of zero range (no memory request, value 0) into the
+    // two spare LDS columns that nothing reads, so that every wave runs the
+    // same instructions and no branch splits the count of loads in flight (a
+    // join of two such paths makes the compiler wait for every outstanding
+    // load)
+    const int hw = w < 6 ? w : w - 6;
+    const int hj = w < 3 ? w : WPB + w;
+    const int hc = hw < 3 ? x0 - 3 + hw : x0 + WPB + (hw - 3);
+    const long hcolb = (long)min(max(hc, 0), a.Nx - 1) * ld;
+    const frsrc Hc = col_rsrc(a.cur, hcolb * CS, w < 6 ? fbytes - hcolb * CS * 4 : 0);
+    auto ldc = [&](frsrc f, int boff) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(f, vo, boff, 0));
+    };
+
+    // does the block touch a free-surface row, a source or a receiver?
+    const int scell = (MODE != FD_ADJ && a.src_cell) ? a.src_cell[s] : -1;
+    int rr = -1;
+    long slot = 0;
+    float injv = 0.f;
+    if ((MODE == FD_FWD && a.rec_on) || (MODE == FD_ADJ && a.inj_on)) {
+        if (active) {
+            const int r = a.rrow[ix];
+            if (r >= iz0 && r <= iz1) {
+                rr = r;
+                slot = (long)a.rcol[ix] * CS + s;
+                if (MODE == FD_ADJ) injv = a.stage[(long)(a.it - 1) * a.nslot + slot];
+            }
+        }
+    }
+    const bool src_here = active && scell >= colb + iz0 && scell <= colb + iz1;
+    const int wave_special = (rr >= 0) || __any(src_here) || (a.fs && iz0 <= a.pml + 3);
+    const bool special = __syncthreads_or(wave_special) != 0;
+
+    // rings: V holds p1 row iz0 - 3 + j in slot j mod U, P and H (p0 and the
+    // halo column) row iz0 + j in slot j mod U
+    float V[U], P[U], H[U];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) V[j] = ldc(P1c, (iz0 - 3 + j) * rs4);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        V[(6 + d) % U] = ldc(P1z, (iz0 + d) * rs4);
+        P[d % U] = ldc(P0c, (iz0 + d) * rs4);
+        H[d % U] = ldc(Hc, (iz0 + d) * rs4);
+    }
+    const cfloat8v *c8 = (const cfloat8v *)(a.coef + CO_N * fsz);
+    cfloat *cf = (cfloat *)a.coef;
+    const frsrc P2 = field_rsrc(a.out, fbytes);
+    int rl = (iz0 + D) * rs4;  // byte offset of the row whose loads are issued next
+    float8v kc[2];
+    kc[0] = c8[colb + iz0];
+
+    // one group of U rows; CHECK: rows past the run's end compute nothing;
+    // ACT: the wave's column is inside the stencil (a compile-time choice on
+    // the plain path, so that its loop has no branch at all and the waits for
+    // loads in flight are exact)
+    auto group = [&](int g, auto check_t, auto special_t, auto act_t) {
+        constexpr bool CHECK = decltype(check_t)::value;
+        constexpr bool SPECIAL = decltype(special_t)::value;
+        constexpr bool ACT = decltype(act_t)::value;
+        const cfloat8v *cg = c8 + colb + iz0 + g * U;  // the group's coefficients
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int rel = g * U + u;
+            const bool live = !CHECK || rel < nrow;
+            row[u & 1][3 + w][lane] = V[(u + 3) % U];
+            row[u & 1][hj][lane] = H[u % U];
+            V[(u + 6 + D) % U] = ldc(P1z, rl);
+            P[(u + D) % U] = ldc(P0c, rl);
+            H[(u + D) % U] = ldc(Hc, rl);
+            __syncthreads();
+            const int iz = iz0 + rel;
+            if (SPECIAL && MODE == FD_FWD && iz == rr) a.stage[(long)a.it * a.nslot + slot] = V[(u + 3) % U];
+            if (ACT && (!SPECIAL || active) && live) {
+                const float xm1 = row[u & 1][2 + w][lane], xp1 = row[u & 1][4 + w][lane];
+                const float xm2 = row[u & 1][1 + w][lane], xp2 = row[u & 1][5 + w][lane];
+                const float xm3 = row[u & 1][0 + w][lane], xp3 = row[u & 1][6 + w][lane];
+                // this row's coefficients were loaded a row ago; the next
+                // row's load goes after the LDS reads (a scalar load in
+                // flight makes every LDS wait a full wait)
+                const float8v kv = kc[u & 1];
+                __builtin_amdgcn_sched_barrier(0);
+                kc[(u + 1) & 1] = cg[u + 1];
+                const float q0 = V[u % U], q1 = V[(u + 1) % U], q2 = V[(u + 2) % U];
+                const float q3 = V[(u + 3) % U], q4 = V[(u + 4) % U], q5 = V[(u + 5) % U];
+                const float q6 = V[(u + 6) % U];
+                if (SPECIAL) {
+                    const float k[7] = {kv[0], kv[1], kv[2], kv[3], kv[4], kv[5], kv[6]};
+                    lanes_update<MODE, false, 1, true>(
+                        a, cf, fsz, colb, ix, iz, s, scell, P2, P2, CS, q0, q1, q2, q3, q4, q5,
+                        q6, xm1, xp1, xm2, xp2, xm3, xp3, P[u % U], k, 0.0, 0.f,
+                        MODE == FD_ADJ && iz == rr, injv, nullptr);
+                } else {
+                    const float v = fd_update<false>(q3, P[u % U], q2, q4, q1, q5, q0, q6, xm1,
+                                                     xp1, xm2, xp2, xm3, xp3, kv[0], kv[1], kv[2],
+                                                     kv[3], kv[4], kv[5], a.invdx, a.invdz,
+                                                     a.invdxdx, a.invdzdz);
+                    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), P2c, vo, rl,
+                                                          0);
+                }
+            }
+            rl += rs4;
+        }
+    };
+    const int nfull = nrow / U;
+    using T = std::true_type;
+    using F = std::false_type;
+    if (special) {
+#pragma unroll 1
+        for (int g = 0; g < nfull; ++g) group(g, F(), T(), T());
+        if (nfull * U < nrow) group(nfull, T(), T(), T());
+    } else if (active) {
+#pragma unroll 1
+        for (int g = 0; g < nfull; ++g) group(g, F(), F(), T());
+        if (nfull * U < nrow) group(nfull, T(), F(), T());
+    } else {
+#pragma unroll 1
+        for (int g = 0; g < nfull; ++g) group(g, F(), F(), F());
+        if (nfull * U < nrow) group(nfull, T(), F(), F());
+    }
+}
+
+template <int MODE>
+static void launch_ring_t(const FdArgs &a0, FdTile tl, hipStream_t st)
+{
+    FdArgs a = a0;
+    const int H = a.ize - a.izs + 1;
+    a.lz_run = tl.zrun;
+    a.lz_nrun = (H + tl.zrun - 1) / tl.zrun;
+    a.lz_ngrp = (a.nshot + 63) / 64;
+    const int nxg = (a.ixe - a.ixs + 1 + 7) / 8;
+    const long nb = (long)nxg * a.lz_nrun * a.lz_ngrp;
+    a.lz_nblk = (int)nb;
+    a.lz_nxg = nxg;
+    a.lz_order = tl.zfast;
+    const dim3 grid((unsigned)(a.remap ? (nb + 7) / 8 * 8 : nb)), block(512);
+    if (tl.pd >= 5)
+        fd_go(fd_ring_kernel<MODE, 5>, grid, block, 0, st, a);
+    else if (tl.pd == 4)
+        fd_go(fd_ring_kernel<MODE, 4>, grid, block, 0, st, a);
+    else if (tl.pd == 3)
+        fd_go(fd_ring_kernel<MODE, 3>, grid, block, 0, st, a);
+    else
+        fd_go(fd_ring_kernel<MODE, 2>, grid, block, 0, st, a);
+}
+
+int ring_unroll(int pd)
+{
+    const int d = pd >= 5 ? 5 : (pd < 2 ? 2 : pd);
+    return ((d + 8) / 2) * 2;
+}
+
 void launch_fd(int mode, const FdArgs &a0, FdTile tl, hipStream_t st)
 {
     FdArgs a = a0;
     a.remap = tl.remap;
+    if (tl.kind == 4 && mode != FD_RECON && a.acc_on) {
+        // no accumulation epilogue in the ring stencil: the LDS stencil of
+        // the same block shape runs these launches
+        tl.kind = 3;
+        tl.wpb = 8;
+        tl.pd = min(tl.pd, 4);
+    }
+    if (tl.kind == 4) {
+        if (mode == FD_FWD)
+            launch_ring_t<FD_FWD>(a, tl, st);
+        else if (mode == FD_RECON)
+            launch_ring_t<FD_RECON>(a, tl, st);
+        else
+            launch_ring_t<FD_ADJ>(a, tl, st);
+        return;
+    }
     if (tl.kind == 3) {
         if (mode == FD_FWD)
             launch_lanes_lds_t<FD_FWD>(a, tl, st);

@@ -42,17 +42,20 @@ constexpr int CO_STREAMED = 6;
 // per block (1, 2, 4, 8).  kind 1: column march over txm columns per block,
 // spb shots per block (1, 2, 4), pd columns of loads in flight.  kind 2: shots
 // in wavefront lanes (fields laid out [x][z][shot]), wpb columns per block,
-// each wave marching zrun rows.
+// each wave marching zrun rows.  kind 3: kind 2 with the row's x-neighbours
+// exchanged through LDS (wpb 4, 8 or 16), pd rows of loads in flight (1-4).
+// kind 4: kind 3 with 8 columns per block and the rows a wave holds in a
+// register ring (pd 2-5 rows in flight, unrolled ring_unroll(pd) rows deep).
 struct FdTile {
     int tz = 64;
     int spb = 1;
     int kind = 0;
     int txm = 0;
-    int zfast = 0;  // kind 0: tile order; kinds 2, 3: block order 0, 1, 2 (see lanes_block)
-    int pd = 1;  // columns prefetched ahead by the march (1, 2, 4)
+    int zfast = 0;  // kind 0: tile order; kinds 2-4: block order 0, 1, 2 (see lanes_block)
+    int pd = 1;     // kind 1: columns prefetched ahead; kinds 2-4: rows of loads in flight
     int remap = 0;  // XCD-aware block order
-    int wpb = 4;    // kind 2: waves (= adjacent columns) per block
-    int zrun = 32;  // kind 2: rows marched by one wave
+    int wpb = 4;    // kinds 2-4: waves (= adjacent columns) per block
+    int zrun = 32;  // kinds 2-4: rows marched by one wave
 };
 
 struct FdArgs {
@@ -109,6 +112,11 @@ void launch_fd(int mode, const FdArgs &a, FdTile tile, hipStream_t st);
 void set_stencil_events(hipEvent_t start, hipEvent_t stop);
 hipError_t take_stencil_launch_error();
 int fd_tile_rows_padded(int tz, int izs, int ize);
+// rows per unrolled group of the register-ring stencil (launch kind 4) with
+// pd rows of loads in flight
+int ring_unroll(int pd);
+// largest multiple of 64 shots whose lanes field of fsz cells stays below 2 GiB
+long lanes_max_shots(long fsz);
 int march_lds_bytes(const FdArgs &a, FdTile tile);
 
 // ---- set-up --------------------------------------------------------------

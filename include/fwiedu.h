@@ -168,7 +168,10 @@ void fwiedu_set_xcd_remap(int on);
  * lds = 0: x-neighbours loaded from memory (wpb 1, 2, 4, 8; zrun 8, 16, 32,
  * 64, 128 or 256; pd 1 ... 3);
  * lds = 1: x-neighbours exchanged through LDS (wpb 4, 8, 16; zrun 8 ... 256;
- * pd 1 ... 4).
+ * pd 1 ... 4);
+ * lds = 2: the same with the rows each wave holds in a register ring (wpb 8;
+ * zrun 8 ... 256, best a multiple of the ring's unroll, 10 rows for pd 2 or
+ * 3 and 12 for pd 4 or 5; pd 2 ... 5).
  * Applies to iso_acoustic2d_propagation and sessions; fwiedu_set_tile(0, 0)
  * returns to the autotuner. */
 void fwiedu_set_lanes(int wpb, int zrun, int pd, int lds);

@@ -377,6 +377,10 @@ def test_every_stencil_shape_bit_identical(L, oracles, fs):
                for rm in (0, 1) for pd in (1, 2, 3)]
     shapes += [("lanes", wpb, zrun, (rm, pd, 1)) for wpb in (4, 8, 16) for zrun in (8, 32, 256)
                for rm in (0, 1) for pd in (1, 2, 3, 4)]
+    # the register-ring LDS stencil: runs shorter than, equal to and several
+    # times its unrolled group (10 or 12 rows), and one run per column
+    shapes += [("lanes", 8, zrun, (rm, pd, 2)) for zrun in (8, 10, 12, 24, 30, 256)
+               for rm in (0, 1) for pd in (2, 3, 4, 5)]
     try:
         for kind, p, spb, zf in shapes:
             if kind == "tile":
@@ -415,10 +419,14 @@ def test_lanes_layout_matches_arenas(L, bits):
             assert flib.stat("layout") == layout, name
         L.fwiedu_set_batch(0)
         for order in (0, 1, 2):
-            # forced shapes in every block order, without and with LDS
-            for lds in (0, 1):
+            # forced shapes in every block order, without and with LDS, and
+            # the register-ring LDS stencil
+            for lds in (0, 1, 2):
                 L.fwiedu_set_tile_order(order)
-                L.fwiedu_set_lanes(4, 8, 2, lds)
+                if lds == 2:
+                    L.fwiedu_set_lanes(8, 20, 3, 2)
+                else:
+                    L.fwiedu_set_lanes(4, 8, 2, lds)
                 out[f"lanes_order{order}_{lds}"] = gradient(L, w, obs, True)
     finally:
         L.fwiedu_set_layout(-1)
@@ -427,7 +435,7 @@ def test_lanes_layout_matches_arenas(L, bits):
         L.fwiedu_set_tile_order(0)
         L.fwiedu_set_checkpoint_bits(32)
     pre_a, g_a = out["arenas"]
-    for name in ["lanes", "lanes33"] + [f"lanes_order{o}_{l}" for o in (0, 1, 2) for l in (0, 1)]:
+    for name in ["lanes", "lanes33"] + [f"lanes_order{o}_{l}" for o in (0, 1, 2) for l in (0, 1, 2)]:
         pre_l, g_l = out[name]
         assert np.array_equal(g_l, g_a), name
         assert np.array_equal(recs(pre_l), recs(pre_a)), name
@@ -473,9 +481,12 @@ def test_lanes_stencil_receivers_match_separate_kernels(L, sr):
     say(f"case: lanes stencil receivers, irregular geometry, 70 shots, sampleRate {sr}")
     out = {}
     try:
-        for name, layout, lds in (("arenas", 0, 0), ("lds", 1, 1), ("lanes", 1, 0)):
+        for name, layout, lds in (("arenas", 0, 0), ("lds", 1, 1), ("lanes", 1, 0),
+                                  ("ring", 1, 2)):
             L.fwiedu_set_layout(layout)
-            if layout:
+            if lds == 2:
+                L.fwiedu_set_lanes(8, 10, 2, 2)
+            elif layout:
                 L.fwiedu_set_lanes(4, 8, 2, lds)
             obs = irregular_shots(w, sr)
             vp, rho = Array2D(w.vp_true), Array2D(w.rho_true)
@@ -492,7 +503,7 @@ def test_lanes_stencil_receivers_match_separate_kernels(L, sr):
         L.fwiedu_set_tile(0, 0)
     r_a, p_a, g_a = out["arenas"]
     assert np.abs(g_a).max() > 0 and np.abs(r_a).max() > 0
-    for name in ("lds", "lanes"):
+    for name in ("lds", "lanes", "ring"):
         r_l, p_l, g_l = out[name]
         assert np.array_equal(r_l, r_a), name
         assert np.array_equal(p_l, p_a), name
@@ -511,10 +522,12 @@ def test_graph_replays_match_eager(L):
     obs = observed(L, w, True)
     obs2 = observed(L, w2, True)
     try:
-        for batch, layout, bits in ((1, 0, 32), (0, 0, 32), (0, 1, 32), (0, 1, 16)):
+        for batch, layout, bits in ((1, 0, 32), (0, 0, 32), (0, 1, 32), (0, 1, 16), (0, 2, 32)):
             L.fwiedu_set_batch(batch)
-            L.fwiedu_set_layout(layout)
-            if layout:
+            L.fwiedu_set_layout(min(layout, 1))
+            if layout == 2:
+                L.fwiedu_set_lanes(8, 12, 4, 2)  # the register-ring stencil
+            elif layout:
                 L.fwiedu_set_lanes(4, 8, 2, 1)
             else:
                 L.fwiedu_set_tile(0, 0)
