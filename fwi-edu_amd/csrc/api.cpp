@@ -284,7 +284,10 @@ extern "C" void alloc_acoustic2dCheckPoint(Acoustic2dCheckPoint *chkpt, int NZ, 
         if (it % d == 0 || (it + 1) % d == 0) nc++;
     chkpt->nCheckPoint = nc;
     chkpt->ntPoint = nc + d - 2;
-    chkpt->memCost = 1.0f * (float)(chkpt->ntPoint * NZ * NX * (int)sizeof(float)) / powf(1024.0f, 3);
+    // checkpoint.c:296: the int product times FSIZE (a size_t), so the byte
+    // count itself is formed in size_t (3.3e9 bytes at 3080 x 1080, nt 8000)
+    chkpt->memCost =
+        1.0f * (float)((size_t)(chkpt->ntPoint * NZ * NX) * sizeof(float)) / powf(1024.0f, 3);
     free_storage(chkpt->storage_p);
     const int np = chkpt->ntPoint;
     float ***s = (float ***)malloc((size_t)np * sizeof(float **));
@@ -369,7 +372,7 @@ extern "C" float calculate_memory_cost_checkpoint(int nt, int NX, int NZ)
     for (int it = 0; it < nt; ++it)
         if (it % d == 0 || (it + 1) % d == 0) nc++;
     const int np = nc + d - 2;
-    return 1.0f * (float)(np * NZ * NX * (int)sizeof(float)) / powf(1024.0f, 3);
+    return 1.0f * (float)((size_t)(np * NZ * NX) * sizeof(float)) / powf(1024.0f, 3);
 }
 
 // ---------------------------------------------------------------------------
@@ -459,6 +462,14 @@ extern "C" void fwiedu_set_memory_fraction(double f)
     ctx().memfrac = (f > 0.0 && f <= 0.98) ? f : 0.85;
 }
 extern "C" void fwiedu_set_kernel_timing(int every) { ctx().ktiming = every < 0 ? 0 : every; }
+extern "C" long fwiedu_lanes_max_shots(int nz, int nx, int pmlThick)
+{
+    if (nz <= 0 || nx <= 0 || pmlThick < 0)
+        fail("<error>: fwiedu_lanes_max_shots: nz %d, nx %d, pmlThick %d", nz, nx, pmlThick);
+    const Geom g = make_geom(nx, nz, pmlThick, 0, 1.0f, 1.0f, 1.0f);
+    return lanes_max_shots(g.fsz);
+}
+
 extern "C" void fwiedu_set_checkpoint_bits(int bits)
 {
     if (bits != 32 && bits != 16) fail("<error>: checkpoint bits must be 32 or 16 (got %d)", bits);

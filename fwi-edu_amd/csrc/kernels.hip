@@ -550,8 +550,9 @@ typedef const __attribute__((address_space(4))) float cfloat;
 // Field access in the lanes layout through buffer instructions: the row
 // offset (cell * CS * 4 bytes) is wave-uniform and goes in the scalar offset,
 // the lane (shot) offset is a 32-bit vector offset, so a load or store costs
-// no vector address arithmetic.  A field must span less than 4 GiB (checked by
-// the host before it picks this layout).
+// no vector address arithmetic.  Byte offsets are formed in signed 32-bit
+// arithmetic, so a field must span less than 2 GiB (the host caps the shots
+// of a lanes batch accordingly, lanes_max_shots).
 typedef __amdgpu_buffer_rsrc_t frsrc;
 __device__ __forceinline__ frsrc field_rsrc(const float *p, long bytes)
 {

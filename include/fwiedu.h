@@ -218,6 +218,11 @@ void fwiedu_set_checkpoint_bits(int bits);
  * Unknown names return -1. */
 double fwiedu_stat(const char *name);
 void fwiedu_reset_stats(void);
+/* Shots per launch the shots-in-lanes layout can hold on an nz x nx model with
+ * pmlThick absorbing cells: the largest multiple of 64 whose field (padded
+ * grid x shots x 4 bytes) stays below 2 GiB, 0 if not even 64 fit (the layout
+ * is then not used).  Host arithmetic only; needs no device. */
+long fwiedu_lanes_max_shots(int nz, int nx, int pmlThick);
 /* Device-resident gradient session: the model, the observed records and all
  * wavefield state stay in HBM between calls, so repeated gradients (FWI
  * iterations, serving) move no traces over PCIe.  Same arguments and meaning as

@@ -469,7 +469,8 @@ EXPORT float oracle_memory_cost_checkpoint(int nt, int NX, int NZ)
 {
     int d, nc, ntp;
     ckpt_plan(nt, &d, &nc, &ntp);
-    return 1.0f * (float)(ntp * NZ * NX * (int)sizeof(float)) / powf(1024.0f, 3);
+    /* the int product times FSIZE, a size_t (checkpoint.c:296) */
+    return 1.0f * (float)((size_t)(ntp * NZ * NX) * sizeof(float)) / powf(1024.0f, 3);
 }
 
 /* ------------------------------------------------------------------------- */

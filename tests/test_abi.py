@@ -36,6 +36,35 @@ def test_library_exports_every_header_symbol():
     assert b"gfx950" in L.fwiedu_version()
 
 
+def test_memory_cost_past_32_bit_byte_counts(oracles):
+    """calculate_memory_cost_checkpoint at 1000 x 3000 with pml 40 and nt 8000:
+    251 checkpoint fields of 3080 x 1080 cells are 3.3e9 bytes, past a 32-bit
+    int; the reference forms the byte count as a size_t (checkpoint.c:296)."""
+    L = flib.load()
+    d, nc, ntp, _ = oracles["f32"].checkpoint_plan(8000)
+    want = ntp * 3080 * 1080 * 4 / 1024 ** 3
+    assert L.calculate_memory_cost_checkpoint(8000, 3080, 1080) == pytest.approx(want, rel=1e-6)
+    assert abs(want - 3.1095) < 1e-3
+
+
+@pytest.mark.parametrize("nz,nx,pml,want", [(221, 601, 40, 2432), (400, 1200, 40, 768),
+                                            (1000, 3000, 40, 128), (2000, 6000, 40, 0)])
+def test_lanes_batch_bound(nz, nx, pml, want):
+    """The shots-in-lanes layout keeps one field (padded cells x shots x 4 bytes,
+    the column height rounded up to 64 cells) below 2 GiB, in whole groups of 64
+    shots: the stencil's byte offsets are signed 32-bit.  Closed form against
+    the library's bound; 1000 x 3000 holds 128 shots (so its 64-shot workload
+    fits in one launch), 2000 x 6000 not even 64."""
+    L = flib.load()
+    Nx, Nz = nx + 2 * pml, nz + 2 * pml
+    ld = (Nz + 63) // 64 * 64
+    fbytes = Nx * ld * 4
+    n = (2 ** 31 - 1) // fbytes // 64 * 64
+    assert n == want
+    assert n * fbytes < 2 ** 31 and (n + 64) * fbytes >= 2 ** 31 - 1
+    assert L.fwiedu_lanes_max_shots(nz, nx, pml) == want
+
+
 def test_host_checkpoint_bookkeeping(oracles):
     """alloc_acoustic2dCheckPoint & co. (checkpoint.c:29-299) against the oracle's plan."""
     L = flib.load()

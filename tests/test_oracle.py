@@ -207,6 +207,14 @@ def test_checkpoint_plan_brute_force(oracles):
 def test_memory_cost(oracles):
     d, nc, ntp, _ = oracles["f32"].checkpoint_plan(4001)
     assert abs(oracles["f32"].memory_cost_checkpoint(4001, 681, 301) - ntp * 681 * 301 * 4 / 1024 ** 3) < 1e-6
+    # 1000 x 3000 with pml 40, nt 8000: 251 fields of 3080 x 1080 cells, 3.3e9
+    # bytes -- past a 32-bit int byte count (checkpoint.c:296 multiplies the
+    # int cell count by FSIZE, a size_t)
+    d, nc, ntp, _ = oracles["f32"].checkpoint_plan(8000)
+    assert (d, ntp) == (126, 251)
+    want = ntp * 3080 * 1080 * 4 / 1024 ** 3
+    assert abs(want - 3.1095) < 1e-3
+    assert oracles["f32"].memory_cost_checkpoint(8000, 3080, 1080) == pytest.approx(want, rel=1e-6)
 
 
 def _run(o, w, vp, rho, problem, obs, pre=None, grad=None, fullstore=False, precondition=1):
@@ -495,3 +503,176 @@ def test_interpolation_band_limited(oracles):
     w[ns - keep + 1:] = r
     want = np.real(np.fft.ifft(X * w))
     assert np.abs(b.a[0] - want).max() < 1e-5
+
+
+# --------------------------------------------------------------------------- adjoint source
+def _one_shot_record(pre, obs_traces, igx, igz, dt, dtr, fpeak, ix_min):
+    """A Record2D per array (ntr x ns traces) sharing the geometry, as
+    copy_record2d_header does (observation.c:173-226)."""
+    from fwiedu.records import Record2D, iptr
+    keep = []
+    out = []
+    for tr in (pre, obs_traces, np.zeros_like(pre)):
+        r = Record2D()
+        a = Array2D(tr)
+        keep.append(a)
+        r.ntr, r.ns = tr.shape
+        r.traces_p = a.pp
+        r.igx, r.igz = iptr(igx), iptr(igz)
+        r.dt, r.dtr, r.fpeak = dt, dtr, fpeak
+        r.ix_min = ix_min
+        out.append(r)
+    return out, keep
+
+
+def _taper_window(L, n):
+    """taper_traces' cosine half-windows (common.c:796-895) as a length-n
+    multiplier, L clamped to n/3 (integer division), in FP32."""
+    f32 = np.float32
+    L = min(L, n // 3)
+    w = np.ones(n, np.float32)
+    if L <= 0:
+        return w
+    i = np.arange(L + 1)
+    coef = (f32(0.5) * (f32(1) + np.cos(f32(np.pi) * i.astype(np.float32) / f32(L)))).astype(np.float32)
+    w[:L + 1] = coef[::-1]
+    w[n - L - 1:] = coef
+    return w
+
+
+def _round_half_away(x):
+    return int(np.floor(float(x) + 0.5))
+
+
+def _adjoint_geometry(nt):
+    """A shot whose window starts at column 100 of a 300-column model, 150
+    receivers at window-local columns 0..149 and depth 2 cells; vp grows with
+    x, so the mean velocity taken the reference's way (window-local igx
+    indexing the FULL model, optimization.c:352-354) differs from the mean
+    over the receivers' own columns."""
+    nx, nz, ix_min, ntr = 300, 20, 100, 150
+    vp = (1500.0 + 20.0 * np.arange(nx)[:, None] + 3.0 * np.arange(nz)[None, :]).astype(np.float32)
+    igx = np.arange(ntr, dtype=np.int32)
+    igz = np.full(ntr, 2, np.int32)
+    dt, dtr, fpeak = np.float32(0.002), np.float32(25.0), np.float32(7.3)
+    rng = np.random.default_rng(11)
+    obs = (rng.integers(-64, 64, (ntr, nt)) / np.float32(16)).astype(np.float32)
+    return vp, igx, igz, dt, dtr, fpeak, ix_min, obs
+
+
+def _taper_lengths(vp, igx, igz, dt, dtr, freq):
+    f32 = np.float32
+    vavg = f32(vp[igx, igz].astype(np.float64).sum() / igx.size)
+    kpass = f32(freq / vavg)
+    Lt = _round_half_away(f32(f32(2.0) / freq) / dt) + 1
+    Lx = _round_half_away(f32(f32(2.0) / kpass) / dtr) + 1
+    return vavg, kpass, Lt, Lx
+
+
+@pytest.mark.parametrize("nt", [600, 300])
+def test_adjoint_source_no_filter_closed_form(oracles, nt):
+    """compute_adjoint_source2d, filter 2 (optimization.c:336-361, 418): a
+    residual of +1 everywhere (pre = obs + 1; the sign is pre - obs) comes out
+    as the product of the time and offset cosine tapers, whose lengths are
+    round(2 / freq / dt) + 1 and round(2 / kpass / dtr) + 1 with freq = fpeak
+    (freq 0 given), kpass = freq / vp_avg and vp_avg the mean of vp[igx][igz]
+    over the receivers, igx window-local but indexing the full model; both
+    lengths clamped to a third of the samples / traces.  With nt 600 the time
+    taper is 138 samples (137.0 rounds up; floor would give 137), with nt 300
+    it is clamped to 100; the offset taper is 34 traces where the receivers'
+    own columns would give a mean of 4996 m/s and a taper of 50."""
+    o = oracles["f32"]
+    vp, igx, igz, dt, dtr, fpeak, ix_min, obs = _adjoint_geometry(nt)
+    pre = (obs + np.float32(1)).astype(np.float32)
+    (rp, ro, ra), keep = _one_shot_record(pre, obs, igx, igz, dt, dtr, fpeak, ix_min)
+    vpa = Array2D(vp)
+    o.adjoint_source(ctypes.byref(rp), ctypes.byref(ro), vpa.pp, 0.0, 2, ctypes.byref(ra))
+    out = keep[2].a
+    vavg, kpass, Lt, Lx = _taper_lengths(vp, igx, igz, dt, dtr, fpeak)
+    assert abs(float(vavg) - 2996.0) < 1e-3
+    assert (Lt, Lx) == (138, 34)
+    # the window-correct mean gives another taper: the quirk is observable
+    _, _, _, Lx_win = _taper_lengths(vp, igx + ix_min, igz, dt, dtr, fpeak)
+    assert min(Lx_win, igx.size // 3) != Lx
+    wt = _taper_window(Lt, nt)
+    wx = _taper_window(Lx, igx.size)
+    want = (np.ones((igx.size, nt), np.float32) * wt[None, :]) * wx[:, None]
+    assert np.abs(out - want).max() <= 2e-7
+    assert np.all(out[:, 0] == 0) and np.all(out[0, :] == 0)
+
+
+def _window_filter_np(x, fpass, fcut, d, band):
+    """apply_traces_lowpass / _bandpass (common.c:584-794): per row, rfft,
+    multiply by the cosine window built in FP32 from f = i / (n d), irfft."""
+    f32 = np.float32
+    n = x.shape[1]
+    df = f32(f32(1.0) / (f32(n) * f32(d)))
+    f = (np.arange(n).astype(np.float32) * df).astype(np.float32)
+    hi = f32(0.5) * (f32(1) + np.cos(f32(np.pi) * (f - f32(fpass)) / f32(fcut - fpass)))
+    if band:
+        lo = f32(0.5) * (f32(1) + np.cos(np.pi * (f32(fpass) - f) / f32(fpass)))
+    else:
+        lo = np.ones(n, np.float32)
+    win = np.where(f < fpass, lo, np.where(f <= fcut, hi, 0.0)).astype(np.float64)
+    X = np.fft.rfft(x.astype(np.float64), axis=1) * win[: n // 2 + 1]
+    return np.fft.irfft(X, n=n, axis=1)
+
+
+@pytest.mark.parametrize("filt", [0, 1])
+def test_adjoint_source_filter_chain(oracles, filt):
+    """compute_adjoint_source2d with filter 0 (low-pass) and 1 (band-pass),
+    optimization.c:368-418: residual, time taper, filter along t (freq,
+    2 freq, dt), offset taper, transpose, filter along the traces (kpass,
+    2 kpass, dtr), transpose back, time and offset tapers.  Restated with
+    numpy's FFT in that order; the same steps with the first taper after the
+    time filter, or without the transposed filter, miss it by far more than
+    the tolerance."""
+    o = oracles["f64"]
+    nt = 160
+    vp, igx, igz, dt, dtr, fpeak, ix_min, obs = _adjoint_geometry(nt)
+    igx, igz = igx[:60].copy(), igz[:60].copy()
+    obs = obs[:60].copy()
+    rng = np.random.default_rng(3)
+    pre = (obs + rng.standard_normal(obs.shape)).astype(np.float32)
+    freq = np.float32(20.0)
+    (rp, ro, ra), keep = _one_shot_record(pre, obs, igx, igz, dt, dtr, fpeak, ix_min)
+    vpa = Array2D(vp)
+    o.adjoint_source(ctypes.byref(rp), ctypes.byref(ro), vpa.pp, float(freq), filt, ctypes.byref(ra))
+    out = keep[2].a.astype(np.float64)
+    vavg, kpass, Lt, Lx = _taper_lengths(vp, igx, igz, dt, dtr, freq)
+    ntr = igx.size
+    wt, wx = _taper_window(Lt, nt), _taper_window(Lx, ntr)
+    band = filt == 1
+
+    def chain(order_ok=True, spatial=True):
+        r = (pre - obs).astype(np.float32).astype(np.float64)
+        if order_ok:
+            r = r * wt[None, :]
+            r = _window_filter_np(r, freq, 2 * freq, dt, band)
+        else:
+            r = _window_filter_np(r, freq, 2 * freq, dt, band)
+            r = r * wt[None, :]
+        r = r * wx[:, None]
+        if spatial:
+            r = _window_filter_np(r.T, kpass, 2 * kpass, dtr, band).T
+        return r * wt[None, :] * wx[:, None]
+
+    want = chain()
+    assert rel(out, want) < 2e-6
+    assert rel(chain(order_ok=False), want) > 1e-3
+    assert rel(chain(spatial=False), want) > 1e-3
+
+
+def test_set_water_layer(oracles):
+    """set_waterlayer_value (common.c:898-924): rows 0..waterdepth-1 of every
+    column take the value, the rest is untouched; waterdepth 0 changes
+    nothing."""
+    o = oracles["f32"]
+    nx, nz = 7, 12
+    m = np.random.default_rng(4).uniform(1000, 3000, (nx, nz)).astype(np.float32)
+    for wd in (0, 1, 5):
+        a = Array2D(m.copy())
+        o.set_water(a.pp, 1480.0, wd, nx, nz)
+        want = m.copy()
+        want[:, :wd] = np.float32(1480.0)
+        assert np.array_equal(a.a, want), wd
