@@ -1,0 +1,255 @@
+"""GPU parity at the large workloads (400 x 1200 multiscale, 1000 x 3000 high
+frequency, the serving stream's largest geometry), within the time of a GPU
+test run: full grids and full nt, with few shots where the CPU oracle has to
+follow.
+
+Tolerances (relative L2):
+  * against the strict-order FP32 oracle (the kernels' own operation order):
+    records and gradients bit-identical in the shots-in-lanes layout; the
+    one-shot-per-launch records, adjoint sources and gradients <= 1e-6 (the
+    stencil sweeps are bit-exact, FP64 sums run in another order);
+  * against the FP64 oracle, FP32 arithmetic: records <= 5e-5, FP32-stored
+    gradients <= 2e-4, FP16-stored gradients <= 2e-3 -- about four times the
+    figures measured at these sizes (records 7.0e-6 / 8.5e-6, gradients
+    1.44e-5 / 5.31e-5 with FP32 and 5.17e-4 / 5.54e-4 with FP16 storage);
+  * a low-pass band of the multiscale sequence (filter 0 at 3 Hz): the
+    library's hipFFT adjoint source against the same chain restated with
+    numpy's FFT in FP64 on the library's own records, <= 1e-5.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from fwiedu import lib as flib
+from fwiedu import serving
+from fwiedu import synth
+from fwiedu.records import Array2D
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    n = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (n if n > 0 else 1.0))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def L():
+    lib = flib.load()
+    assert lib.fwiedu_device_ok() == 1, "no HIP device"
+    return lib
+
+
+def recs(shotset):
+    return np.concatenate([t.ravel() for t in shotset.all_traces()])
+
+
+def observed(o_or_lib, w, is_lib):
+    obs = w.shots()
+    vp, rho = Array2D(w.vp_true), Array2D(w.rho_true)
+    if is_lib:
+        flib.propagate(o_or_lib, w, vp, rho, obs, 0)
+    else:
+        o_or_lib.propagation(vp.pp, rho.pp, obs.recs, w.nshot, w.nz, w.nx, w.dx, w.dz, w.dt, w.nt,
+                             w.pml, w.freesurface, w.freq, float(w.filter), w.precondition, 0,
+                             obs.recs, None)
+    return obs
+
+
+def gradient(o_or_lib, w, obs, is_lib, bits=32):
+    vp, rho = Array2D(w.vp_start), Array2D(w.rho_start)
+    pre = obs.header_copy()
+    g = Array2D(np.zeros((w.nx, w.nz), np.float32))
+    if is_lib:
+        o_or_lib.fwiedu_set_checkpoint_bits(bits)
+        try:
+            flib.propagate(o_or_lib, w, vp, rho, obs, 1, pre, g)
+        finally:
+            o_or_lib.fwiedu_set_checkpoint_bits(32)
+    else:
+        o_or_lib.propagation(vp.pp, rho.pp, obs.recs, w.nshot, w.nz, w.nx, w.dx, w.dz, w.dt, w.nt,
+                             w.pml, w.freesurface, w.freq, float(w.filter), w.precondition, 1,
+                             pre.recs, g.pp)
+    return pre, g.a.copy()
+
+
+@pytest.mark.parametrize("name", ["c2", "c4"])
+def test_full_size_one_shot_against_fp32_and_fp64(L, oracles, name):
+    """400 x 1200 (nt 5000) and 1000 x 3000 (nt 8000, checkpoint spacing 126,
+    251 slots): one shot, full grid and full nt.  At 400 x 1200 records,
+    adjoint source and gradient against the strict FP32 oracle; at both sizes
+    records and the gradients with FP32 and FP16 checkpoint storage against
+    the FP64 oracle (observed data from the FP64 oracle's modeling).  The
+    strict FP32 oracle is left out at 1000 x 3000 for time: its 1.0e11
+    cell-updates would take minutes of host time (FP32 denormals in the
+    absorbing layers run through the host's slow path)."""
+    w = synth.workload(name, nshot=1)
+    say(f"case: {name} one shot {w.nz}x{w.nx} nt={w.nt}")
+    errs = {}
+    if name == "c2":
+        obs_s = observed(oracles["f32s"], w, False)
+        errs["records vs f32s"] = rel(recs(observed(L, w, True)), recs(obs_s))
+        pre_g, g_g = gradient(L, w, obs_s, True, 32)
+        pre_s, g_s = gradient(oracles["f32s"], w, obs_s, False)
+        errs["gradient vs f32s"] = rel(g_g, g_s)
+        errs["adjoint source vs f32s"] = rel(recs(pre_g), recs(pre_s))
+    obs_d = observed(oracles["f64"], w, False)
+    errs["records vs f64"] = rel(recs(observed(L, w, True)), recs(obs_d))
+    pre32, g32 = gradient(L, w, obs_d, True, 32)
+    pre16, g16 = gradient(L, w, obs_d, True, 16)
+    assert np.array_equal(recs(pre16), recs(pre32))
+    _, g_d = gradient(oracles["f64"], w, obs_d, False)
+    errs["gradient fp32 store vs f64"] = rel(g32, g_d)
+    errs["gradient fp16 store vs f64"] = rel(g16, g_d)
+    say("  rel L2:", {k: f"{v:.3e}" for k, v in errs.items()})
+    if name == "c2":
+        assert errs["records vs f32s"] <= 1e-6
+        assert errs["gradient vs f32s"] <= 1e-6
+        assert errs["adjoint source vs f32s"] <= 1e-6
+    assert errs["records vs f64"] <= 5e-5
+    assert errs["gradient fp32 store vs f64"] <= 2e-4
+    assert errs["gradient fp16 store vs f64"] <= 2e-3
+
+
+def _taper_window(L, n):
+    f32 = np.float32
+    L = min(L, n // 3)
+    w = np.ones(n, np.float64)
+    if L <= 0:
+        return w
+    i = np.arange(L + 1).astype(np.float32)
+    coef = f32(0.5) * (f32(1) + np.cos(f32(np.pi) * i / f32(L)))
+    w[:L + 1] = coef[::-1]
+    w[n - L - 1:] = coef
+    return w
+
+
+def _lowpass_np(x, fpass, fcut, d):
+    f32 = np.float32
+    n = x.shape[1]
+    df = f32(f32(1.0) / (f32(n) * f32(d)))
+    f = np.arange(n).astype(np.float32) * df
+    win = np.where(f < fpass, f32(1),
+                   np.where(f <= fcut,
+                            f32(0.5) * (f32(1) + np.cos(f32(np.pi) * (f - f32(fpass)) / f32(fcut - fpass))),
+                            f32(0))).astype(np.float64)
+    X = np.fft.rfft(x, axis=1) * win[: n // 2 + 1]
+    return np.fft.irfft(X, n=n, axis=1)
+
+
+def test_c2_lowpass_band_adjoint_source(L):
+    """400 x 1200, nt 5000, one shot, the 3 Hz band of the multiscale sequence
+    (filter 0: low-pass 3-6 Hz in time, kpass-2 kpass across the traces,
+    optimization.c:368-391): the adjoint source of a line-search trial
+    (problem 2) against compute_adjoint_source2d's chain restated with numpy
+    on the library's own modelled records."""
+    w = synth.workload("c2", nshot=1, freq=3.0, filter=0)
+    say(f"case: c2 low-pass 3 Hz {w.nz}x{w.nx} nt={w.nt}")
+    obs = observed(L, w, True)
+    # a trial model: the starting model
+    vp, rho = Array2D(w.vp_start), Array2D(w.rho_start)
+    raw = obs.header_copy()
+    flib.propagate(L, w, vp, rho, obs, 0, raw)
+    adj = obs.header_copy()
+    flib.propagate(L, w, vp, rho, obs, 2, adj)
+    r0 = obs.recs[0]
+    ntr, nt = r0.ntr, r0.ns
+    igx = np.ctypeslib.as_array(r0.igx, (ntr,)).astype(np.int64)
+    igz = np.ctypeslib.as_array(r0.igz, (ntr,)).astype(np.int64)
+    f32 = np.float32
+    # vp_avg over vp[igx][igz] of the full model (igx window-local; the window is the model here)
+    vavg = f32(w.vp_start[igx, igz].astype(np.float64).sum() / ntr)
+    freq = f32(3.0)
+    kpass = f32(freq / vavg)
+    dt, dtr = f32(r0.dt), f32(r0.dtr)
+    Lt = int(np.floor(float(f32(f32(2) / freq) / dt) + 0.5)) + 1
+    Lx = int(np.floor(float(f32(f32(2) / kpass) / dtr) + 0.5)) + 1
+    wt, wx = _taper_window(Lt, nt), _taper_window(Lx, ntr)
+    r = (raw.trace(0) - obs.trace(0)).astype(np.float32).astype(np.float64)
+    r = _lowpass_np(r * wt[None, :], freq, 2 * freq, dt)
+    r = r * wx[:, None]
+    r = _lowpass_np(r.T, kpass, 2 * kpass, dtr).T
+    want = r * wt[None, :] * wx[:, None]
+    e = rel(adj.trace(0), want)
+    say(f"  taper lengths {Lt} / {Lx}, rel L2 against numpy {e:.3e}")
+    assert np.abs(raw.trace(0) - obs.trace(0)).max() > 0
+    assert e <= 1e-5
+
+
+def test_c2_lanes_64_shots_bit_exact(L, oracles):
+    """400 x 1200, nt 300: 64 shots in one shots-in-lanes launch (autotuned
+    shapes, in-stencil receivers, staged accumulation) give records, adjoint
+    sources and gradient bit-identical to one shot per arena; and two of those
+    shots through the lanes layout are bit-identical to the strict FP32
+    oracle (all 64 through the oracle would take minutes of host time)."""
+    w = synth.workload("c2", nshot=64, nt=300)
+    say(f"case: c2 lanes {w.nz}x{w.nx} nt={w.nt} shots=64")
+    obs = observed(L, w, True)
+    out = {}
+    try:
+        for layout in (1, 0):
+            L.fwiedu_set_layout(layout)
+            out[layout] = gradient(L, w, obs, True)
+            assert flib.stat("layout") == layout
+            if layout:
+                kinds = [flib.stat(f"tile{m}_kind") for m in range(3)]
+        w2 = synth.workload("c2", nshot=2, nt=300)
+        obs_o = observed(oracles["f32s"], w2, False)
+        L.fwiedu_set_layout(1)
+        obs_g = observed(L, w2, True)
+        pre_g, g_g = gradient(L, w2, obs_o, True)
+        assert flib.stat("layout") == 1
+    finally:
+        L.fwiedu_set_layout(-1)
+    say("  stencil kinds", kinds)
+    assert np.array_equal(recs(out[1][0]), recs(out[0][0]))
+    assert np.array_equal(out[1][1], out[0][1])
+    pre_o, g_o = gradient(oracles["f32s"], w2, obs_o, False)
+    assert np.array_equal(recs(obs_g), recs(obs_o))
+    assert np.array_equal(recs(pre_g), recs(pre_o))
+    assert np.array_equal(g_g, g_o)
+
+
+# The serving stream's geometry range (128-1024 cells per side at 10 m, nt
+# 1000-4000, 1-64 shots): its largest corner as a gradient job, with jobs of
+# the stream's own generator around it.
+C3_MIX = [(1, 1024, 1024, 4000, 64), (0, 128, 1024, 1000, 1), (0, 700, 300, 2500, 40),
+          (1, 333, 128, 1800, 7), (0, 1024, 1024, 1500, 3), (1, 200, 900, 1200, 12)]
+
+
+def test_c3_serving_mix_with_largest_geometry(L):
+    """Six requests of the serving stream's generator on 3 workers, among them
+    a 1024 x 1024, nt 4000, 64-shot gradient: every record, adjoint source and
+    gradient is bit-identical to the same request run alone."""
+    say(f"case: serving mix with the largest c3 geometry, {len(C3_MIX)} jobs")
+    jobs = []
+    for i, (p, nz, nx, nt, ns) in enumerate(C3_MIX):
+        w = serving.request_workload(i, 303000 + i, nz, nx, nt, ns)
+        jobs.append(serving.Job(w, p, serving.observed_for(L, w) if p == 1 else None))
+    srv = serving.Server(slots=3, mem_fraction=0.8, graphs=True)
+    try:
+        for j in jobs:
+            srv.submit(j)
+        for j in jobs:
+            srv.wait(j)
+        peak = srv.stat("peak_concurrency")
+        say(f"  peak concurrency {peak:.0f}, p50 {srv.stat('p50_s'):.2f} s, p99 {srv.stat('p99_s'):.2f} s")
+    finally:
+        srv.close()
+    assert peak >= 2
+    for j in jobs:
+        w = j.w
+        pre = j.obs.header_copy()
+        g = Array2D(np.zeros((w.nx, w.nz), np.float32)) if j.problem == 1 else None
+        flib.propagate(L, w, j.vp, j.rho, j.obs, j.problem, pre, g)
+        assert np.array_equal(j.traces(), recs(pre)), w.name
+        if j.problem == 1:
+            assert np.array_equal(j.grad.a, g.a), w.name

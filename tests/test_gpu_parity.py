@@ -1,13 +1,17 @@
 """GPU parity: the HIP library (through its C ABI) against the CPU oracle on the
 same seeded inputs.
 
-Tolerances (relative L2 unless noted):
+Tolerances (relative L2 unless noted), fixed:
   vs the strict-order FP32 oracle (f32s, same operation order as the kernels):
       records, adjoint sources, gradients, misfit <= 1e-6 (bit-exact observed)
   vs the reference-flags FP32 oracle (f32, -O3 -ffast-math): records 1e-5,
-      gradients 1e-4, misfit 1e-5 -- or 1.5x the spread between the two CPU
-      builds of the same oracle when that spread is larger (the -ffast-math
-      build's own rounding, e.g. the c0 misfit, where the residual is small)
+      gradients 1e-4, misfit MISFIT_VS_FASTMATH = 5e-5.  The misfit bound is
+      wider than the 1e-5 of the records because the reference's own flags
+      move it that far: the -ffast-math build and the strict build of the same
+      oracle source differ in the misfit by 1.3e-5 (30 x 48 grid) and
+      1.05e-5 (101 x 201) on one host and 3.2e-5 (101 x 201) on another (the
+      residual is a small difference of two records; glibc picks its expf /
+      powf code per CPU), while the GPU equals the strict build bit for bit
   vs the FP64 oracle: stated, and bounded by 1e-4 (records) / 1e-3 (gradients)
 """
 import ctypes
@@ -21,6 +25,9 @@ from fwiedu import synth
 from fwiedu.records import Array2D, fptr
 
 pytestmark = pytest.mark.gpu
+
+
+MISFIT_VS_FASTMATH = 5e-5
 
 
 def rel(a, b):
@@ -131,8 +138,7 @@ def test_modeling_parity(L, oracles, case):
     errs = {k: rel(g, recs(observed(oracles[k], w, False))) for k in ("f32s", "f32", "f64")}
     say("  rel L2 records:", errs)
     assert errs["f32s"] <= 1e-6
-    spread = rel(recs(observed(oracles["f32"], w, False)), recs(observed(oracles["f32s"], w, False)))
-    assert errs["f32"] <= max(1e-5, 1.5 * spread)
+    assert errs["f32"] <= 1e-5
     assert errs["f64"] <= 1e-4
 
 
@@ -154,8 +160,8 @@ def test_gradient_parity(L, oracles, case):
     spread = (rel(ref["f32"][0], ref["f32s"][0]), abs(ref["f32"][1] - ref["f32s"][1]) / ref["f32s"][1])
     say("  (grad, adjoint source, misfit) rel errors:", out, "cpu-build spread:", spread)
     assert max(out["f32s"]) <= 1e-6
-    assert out["f32"][0] <= max(1e-4, 1.5 * spread[0])
-    assert out["f32"][2] <= max(1e-5, 1.5 * spread[1])
+    assert out["f32"][0] <= 1e-4
+    assert out["f32"][2] <= MISFIT_VS_FASTMATH
     assert out["f64"][0] <= 1e-3
 
 
@@ -700,9 +706,9 @@ def test_fma_arithmetic(L, oracles):
     """fwiedu_set_fma(1): the update with fused multiply-adds.  Every launch
     shape and both layouts still give the same bits as each other (the result
     depends only on the arithmetic mode), and the results stay within the
-    reference tolerances of the FP32 oracle (records 1e-5, gradient 1e-4,
-    misfit 1e-5 relative, or 1.5x the spread between the two FP32 oracle
-    builds when larger) and close to the FP64 oracle."""
+    tolerances against the reference-flags FP32 oracle (records 1e-5,
+    gradient 1e-4, misfit MISFIT_VS_FASTMATH relative) and close to the FP64
+    oracle."""
     w = synth.workload("tiny", nshot=5, freesurface=1, shot_depth_cells=1, nx=70)
     say("case: fma arithmetic tiny fs=1 5 shots")
     obs = observed(oracles["f32s"], w, False)
@@ -742,9 +748,9 @@ def test_fma_arithmetic(L, oracles):
                    abs(f_g - ref["f32"][2]) / ref["f32"][2])
             e64 = (rel(rec_g, ref["f64"][0]), rel(g_g, ref["f64"][1]))
             say(f"  {case}: vs f32 (records, grad, misfit) {e32}, vs f64 {e64}, spread {spread}")
-            assert e32[0] <= max(1e-5, 1.5 * spread[0])
-            assert e32[1] <= max(1e-4, 1.5 * spread[1])
-            assert e32[2] <= max(1e-5, 1.5 * spread[2])
+            assert e32[0] <= 1e-5
+            assert e32[1] <= 1e-4
+            assert e32[2] <= MISFIT_VS_FASTMATH
             assert e64[0] <= 1e-4 and e64[1] <= 1e-3
     finally:
         L.fwiedu_set_fma(0)
