@@ -2095,6 +2095,17 @@ float model_perturbation_device(float fraction, const float *sl, const float *g,
     Ctx &c = ctx();
     float *fs = c.misc[2].as<float>(2 * 512 + 64 + 1001 + 64);
     int *hist = reinterpret_cast<int *>(fs + 2 * 512 + 64);
+    double *scr = c.scratch.as<double>(reduce_scratch_doubles() + 64);
+    return model_perturbation_with(fraction, sl, g, out, len, fs, hist, scr);
+}
+
+// The same with the caller's scratch: mm 2 * 512 + 2 floats (min/max block
+// partials, then the result), hist 1001 ints (cleared here), red
+// reduce_scratch_doubles() + 2 doubles (FP64 block partials, then the sums).
+float model_perturbation_with(float fraction, const float *sl, const float *g, float *out, int len,
+                              float *fs, int *hist, double *scr)
+{
+    Ctx &c = ctx();
     launch_minmax(g, len, fs, fs + 2 * 512, c.st);
     float mm[2];
     d2h(mm, fs + 2 * 512, 2 * sizeof(float));
@@ -2105,7 +2116,6 @@ float model_perturbation_device(float fraction, const float *sl, const float *g,
     launch_histogram(g, len, Min, interval, hist, c.st);
     int h[1001];
     d2h(h, hist, sizeof(h));
-    double *scr = c.scratch.as<double>(reduce_scratch_doubles() + 64);
     double *res = scr + reduce_scratch_doubles();
     launch_sum(sl, len, scr, res, c.st);
     double avg[2];

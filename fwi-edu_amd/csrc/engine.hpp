@@ -176,6 +176,8 @@ float objective_device(const float *d, long n);
 float step_length_device(const float *a, const float *b, long n);
 float model_perturbation_device(float fraction, const float *sl, const float *g, float *out,
                                 int len);
+float model_perturbation_with(float fraction, const float *sl, const float *g, float *out, int len,
+                              float *mm, int *hist, double *red);
 void smooth2d_device(const float *in, float *out, int nx, int nz, int N, int zdir);
 void call_footprint(const Call &P, bool ck16, size_t *fixed, size_t *per_shot);
 void check_common(int nz, int nx, float dx, float dz, float dt, int nt, int pml, int fs);

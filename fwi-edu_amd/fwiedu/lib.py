@@ -120,6 +120,8 @@ def load():
     _sig(L, "fwiedu_stat", c_double, [c_char_p])
     _sig(L, "fwiedu_reset_stats", None, [])
     _sig(L, "fwiedu_lanes_max_shots", c_long, [c_int, c_int, c_int])
+    _sig(L, "fwiedu_model_perturbation_noalloc", c_float,
+         [c_float, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_int])
     _sig(L, "fwiedu_release", None, [])
     _sig(L, "fwiedu_device_ok", c_int, [])
     _sig(L, "fwiedu_version", c_char_p, [])
@@ -140,7 +142,7 @@ EXPORTED = [
     "fwiedu_session_create", "fwiedu_session_set_model", "fwiedu_session_gradient",
     "fwiedu_session_destroy", "fwiedu_session_iterate", "fwiedu_set_batch", "fwiedu_set_tile", "fwiedu_set_march", "fwiedu_set_tile_order", "fwiedu_set_xcd_remap", "fwiedu_set_lanes", "fwiedu_set_layout", "fwiedu_set_graphs", "fwiedu_set_fma", "fwiedu_set_memory_fraction", "fwiedu_set_kernel_timing",
     "fwiedu_set_checkpoint_bits", "fwiedu_stat", "fwiedu_reset_stats", "fwiedu_release",
-    "fwiedu_lanes_max_shots",
+    "fwiedu_lanes_max_shots", "fwiedu_model_perturbation_noalloc",
     "fwiedu_device_ok", "fwiedu_version",
     "fwiedu_server_create", "fwiedu_server_submit", "fwiedu_server_done", "fwiedu_server_wait",
     "fwiedu_server_stat", "fwiedu_server_destroy",

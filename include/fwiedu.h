@@ -218,6 +218,18 @@ void fwiedu_set_checkpoint_bits(int bits);
  * Unknown names return -1. */
 double fwiedu_stat(const char *name);
 void fwiedu_reset_stats(void);
+/* No counterpart in the reference: compute_model_perturbation (above) on
+ * arrays already on the device, for callers that cannot allocate device memory
+ * (code that forbids hipMalloc, serving loops that keep allocation out of their
+ * steady state).  slowness, gradient and out are device pointers of len floats
+ * (out may be slowness, not gradient); returns the same scalar and writes the
+ * same out as compute_model_perturbation on the same values.  Its scratch is a
+ * fixed 32 KiB device array per GPU, held until the process ends, whose size
+ * does not depend on len (any len from 1 to INT_MAX); every call on a GPU
+ * shares it, so calls that overlap on one GPU race.  Allocates no device
+ * memory. */
+float fwiedu_model_perturbation_noalloc(float fraction, const float *slowness,
+                                        const float *gradient, float *out, int len);
 /* Shots per launch the shots-in-lanes layout can hold on an nz x nx model with
  * pmlThick absorbing cells: the largest multiple of 64 whose field (padded
  * grid x shots x 4 bytes) stays below 2 GiB, 0 if not even 64 fit (the layout
