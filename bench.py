@@ -213,6 +213,10 @@ def serving_leg(args):
                "shots_per_s": sum(j.shots() for j in jobs) / wall,
                "jobs_per_s": len(jobs) / wall,
                "p50_latency_s": srv.stat("p50_s"), "p99_latency_s": srv.stat("p99_s"),
+               "latency_ranks": f"nearest rank over these {args.serving} requests: p99 is "
+                                f"latency number {int(np.ceil(0.99 * args.serving))} of "
+                                f"{args.serving} in increasing order (the full 2000-request "
+                                f"stream: --serving 2000)",
                "max_latency_s": srv.stat("max_s"),
                "mean_queue_s": float(np.mean([j.queue for j in jobs])),
                "peak_concurrent_jobs": int(srv.stat("peak_concurrency")),
@@ -220,6 +224,10 @@ def serving_leg(args):
                          "nt 1000-4000; 1-64 shots; 30 % gradient jobs (observed traces zero)"}
     finally:
         srv.close()
+    if args.serving_log:
+        with open(args.serving_log, "w") as f:
+            for j in jobs:
+                f.write(json.dumps(j.row()) + "\n")
     log(f"serving: p50 {out['p50_latency_s']:.3f} s, p99 {out['p99_latency_s']:.3f} s, "
         f"{out['shots_per_s']:.1f} shots/s")
     return out
@@ -250,22 +258,55 @@ def large_leg(L, args):
 
 
 def cpu_model():
+    """The host CPU as /proc/cpuinfo names it, with its family / model /
+    stepping numbers and the logical CPU count."""
+    info = {}
+    n = 0
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k == "processor":
+                    n += 1
+                elif k in ("model name", "vendor_id", "cpu family", "model", "stepping",
+                           "cpu MHz", "cache size") and k not in info:
+                    info[k] = v.strip()
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    return (f"{info.get('model name', 'unknown')} ({info.get('vendor_id', '?')} family "
+            f"{info.get('cpu family', '?')} model {info.get('model', '?')} stepping "
+            f"{info.get('stepping', '?')}, {n} logical CPUs, {info.get('cache size', '?')} cache)")
+
+
+def native_oracle():
+    """The FP32 oracle built on this host with the reference's flags
+    (-O3 -march=native -ffast-math -fopenmp, Makefile.gcc:28); None (and the
+    prebuilt -march=x86-64-v3 build is used) if the build fails."""
+    import subprocess
+    import tempfile
+    d = tempfile.mkdtemp(prefix="fwiedu_oracle_")
+    try:
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native",
+                        f"NATIVE_DIR={d}"], check=True, capture_output=True, timeout=120)
+        return os.path.join(d, "liboracle_f32n.so")
+    except (subprocess.SubprocessError, OSError):
+        return None
 
 
 def cpu_baseline(w, nshots, shots_1t):
     """The FP32 oracle (reference algorithm and compiler flags, OpenMP over all
     host threads) on `nshots` gradient shots of the same workload, then on
     `shots_1t` of them with one OpenMP thread."""
+    # thread placement as the baseline plan states; libgomp reads these when
+    # the oracle library first loads it
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     from oracle import Oracle
-    o = Oracle("f32")
+    native = native_oracle()
+    o = Oracle("f32", path=native) if native else Oracle("f32")
+    build = ("-O3 -march=native -ffast-math, built on this host" if native else
+             "-O3 -march=x86-64-v3 -ffast-math (prebuilt; the native build failed)")
     sub = synth.workload(w.name, nshot=nshots)
     obs = sub.shots()
     vpt, rht = Array2D(sub.vp_true), Array2D(sub.rho_true)
@@ -285,7 +326,9 @@ def cpu_baseline(w, nshots, shots_1t):
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     out = {"value": nshots / dt, "unit": "gradient shots/s", "cores": cores, "kind": "port",
            "sample": f"{nshots} of the {w.nshot} gradient shots of the same workload "
-                     f"({dt:.1f} s), oracle/liboracle_f32.so (-O3 -ffast-math, OpenMP)",
+                     f"({dt:.1f} s), the FP32 oracle ({build}, OpenMP, "
+                     f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}, "
+                     f"OMP_PLACES={os.environ.get('OMP_PLACES')})",
            "cpu": cpu_model()}
     if shots_1t > 0:
         gomp = ctypes.CDLL("libgomp.so.1")
@@ -328,6 +371,8 @@ def main():
                     help="requests of the serving leg (0 = skip; 2000 = the full stream)")
     ap.add_argument("--serving-rate", type=float, default=3.0, help="Poisson arrivals per second")
     ap.add_argument("--serving-slots", type=int, default=4)
+    ap.add_argument("--serving-log", default="",
+                    help="write one JSON line per completed request (sizes, latency, queueing)")
     ap.add_argument("--ckpt-bits", type=int, default=32, choices=(32, 16),
                     help="checkpoint storage: FP32 or FP16 with a per-snapshot scale")
     ap.add_argument("--freq", type=float, default=0.0,
@@ -397,12 +442,14 @@ def main():
     if args.plain_shots > 0:
         pw = synth.workload(args.workload, nshot=args.plain_shots)
         pobs = observed(L, pw)
-        ptimes, pst, _ = timed_gradients(L, pw, pobs, 1, 0, 1, args.ktime_every)
+        # one warm-up gradient first: launch-shape tuning of the one-shot
+        # geometry stays out of the timed one
+        ptimes, pst, _ = timed_gradients(L, pw, pobs, 1, 1, 1, args.ktime_every)
         ps = pw.nshot / float(np.mean(ptimes))
         plain = {"shots_per_launch": 1, "gradient_shots_per_s": ps,
                  "gcell_updates_per_s": ps * cu / 1e9,
                  "fd_avg_launch_us": pst["fd_ms_avg"] * 1e3,
-                 "sample": f"{args.plain_shots} shots, 1 step"}
+                 "sample": f"{args.plain_shots} shots, 1 warm-up and 1 timed gradient"}
         log(f"plain: {ps:.2f} shots/s")
 
     serve = serving_leg(args) if args.serving > 0 else None

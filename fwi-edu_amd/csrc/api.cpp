@@ -531,6 +531,7 @@ extern "C" double fwiedu_stat(const char *name)
     if (n == "ckpt_bits") return (double)s.ckpt_bits;
     if (n == "graphs_built") return (double)s.graphs_built;
     if (n == "graph_launches") return (double)s.graph_launches;
+    if (n == "shapes_reused") return (double)s.shapes_reused;
     if (n == "h2d_bytes") return s.h2d_bytes;
     if (n == "d2h_bytes") return s.d2h_bytes;
     if (n == "device_bytes") return (double)held_bytes(ctx());

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <thread>
 #include <tuple>
 
@@ -92,6 +93,7 @@ void release_buffers(Ctx &c)
 {
     for (auto &e : c.graph_cache) (void)hipGraphExecDestroy(e.second.exec);
     c.graph_cache.clear();
+    c.graph_seen.clear();
     release_fft(c);
     for (DevBuf *d : all_buffers(c)) d->release();
 }
@@ -325,6 +327,60 @@ static FdTile fastest_shape(int mode, const FdArgs &a, const std::vector<FdTile>
     return cand[ms[best].second];
 }
 
+// Shapes tuned by any context of the process, by geometry class, for callers
+// that take a shape tuned for a similar geometry instead of timing their own
+// (serving workers, whose requests rarely repeat a geometry exactly: tuning a
+// new geometry times a few hundred launches per stencil mode).  Any shape gives
+// the same bits, so reuse changes the speed only.  Similar: same stencil mode,
+// accumulation variant, layout, 64-lane groups (lanes) or shot count (arenas,
+// whose shots per block depend on it), free surface and arithmetic, and
+// column count and stencil height within a quarter of the tuned one's.
+struct TunedClass {
+    int mode, accv, lanes, group, fs, fma, ncol, nrow;
+    FdTile tile;
+};
+static std::mutex g_similar_mu;
+static std::vector<TunedClass> g_similar;
+
+static TunedClass tuned_class(int mode, int accv, const FdArgs &a)
+{
+    TunedClass t;
+    t.mode = mode;
+    t.accv = accv;
+    t.lanes = a.cell_stride > 1 ? 1 : 0;
+    t.group = t.lanes ? (a.nshot + 63) / 64 : a.nshot;
+    t.fs = a.fs;
+    t.fma = a.fma;
+    t.ncol = a.ixe - a.ixs + 1;
+    t.nrow = a.ize - a.izs + 1;
+    return t;
+}
+
+static bool near(int a, int b) { return 4L * std::abs(a - b) <= (long)b; }
+
+static bool find_similar(const TunedClass &q, const FdArgs &base, FdTile &out)
+{
+    std::lock_guard<std::mutex> lk(g_similar_mu);
+    for (auto it = g_similar.rbegin(); it != g_similar.rend(); ++it) {
+        const TunedClass &t = *it;
+        if (t.mode != q.mode || t.accv != q.accv || t.lanes != q.lanes || t.group != q.group ||
+            t.fs != q.fs || t.fma != q.fma || !near(q.ncol, t.ncol) || !near(q.nrow, t.nrow))
+            continue;
+        if (t.tile.kind == 1 && march_lds_bytes(base, t.tile) > 65536) continue;
+        out = t.tile;
+        return true;
+    }
+    return false;
+}
+
+static void remember_similar(TunedClass q, const FdTile &tile)
+{
+    std::lock_guard<std::mutex> lk(g_similar_mu);
+    q.tile = tile;
+    if (g_similar.size() >= 256) g_similar.erase(g_similar.begin());
+    g_similar.push_back(q);
+}
+
 FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur, const float *prev)
 {
     Ctx &c = ctx();
@@ -344,6 +400,13 @@ FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur,
                                      base.cell_stride);
     auto it = c.tuned.find(key);
     if (it != c.tuned.end()) return it->second;
+    const TunedClass cls = tuned_class(mode, 0, base);
+    FdTile sim;
+    if (c.reuse_similar && find_similar(cls, base, sim)) {
+        c.stats.shapes_reused++;
+        c.tuned[key] = sim;
+        return sim;
+    }
     FdArgs a = base;
     a.out = out;
     a.cur = cur;
@@ -430,6 +493,7 @@ FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur,
     float *const f[3] = {out, const_cast<float *>(cur), const_cast<float *>(prev)};
     const FdTile best = fastest_shape(mode, a, cand, f);
     c.tuned[key] = best;
+    remember_similar(cls, best);
     return best;
 }
 
@@ -454,6 +518,14 @@ FdTile autotune_acc_tile(int mode, int accv, const FdTile &mode_tile, const FdAr
                                      base.cell_stride);
     auto it = c.tuned_acc.find(key);
     if (it != c.tuned_acc.end()) return it->second;
+    TunedClass cls = tuned_class(mode, accv, base);
+    cls.group = cls.group * 8 + mode_tile.kind;  // its candidates derive from the mode's shape
+    FdTile sim;
+    if (c.reuse_similar && find_similar(cls, base, sim)) {
+        c.stats.shapes_reused++;
+        c.tuned_acc[key] = sim;
+        return sim;
+    }
     FdArgs a = base;
     a.out = out;
     a.cur = cur;
@@ -480,6 +552,7 @@ FdTile autotune_acc_tile(int mode, int accv, const FdTile &mode_tile, const FdAr
     float *const f[3] = {out, const_cast<float *>(cur), const_cast<float *>(prev)};
     const FdTile best = fastest_shape(mode, a, cand, f);
     c.tuned_acc[key] = best;
+    remember_similar(cls, best);
     return best;
 }
 
@@ -953,8 +1026,12 @@ static void rec_d2h(Record2D *r, const float *src)
 // read them) and the adjoint sources come down during the adjoint pass (which
 // only reads them).  The copies start once the work enqueued on the compute
 // stream before `after` is done; join() returns when they have completed.
-// Not used while the time loops are replayed from HIP graphs: a loop captured
-// on the compute stream while another thread copied has ended with a HIP error.
+// The copy stream's wait for `after` is enqueued here, on the calling thread,
+// before the batch's time loop may begin a graph capture on the compute
+// stream: a wait issued by the helper thread could land while that stream is
+// capturing, and a stream that waits on an event of a capturing stream joins
+// the capture, where the helper's copies and stream synchronisation are not
+// allowed (the capture then ends with an error).
 struct RecordCopies {
     struct Piece {
         void *dst;
@@ -974,10 +1051,9 @@ struct RecordCopies {
                 (double)p.bytes;
         const int dev = c.device;
         hipStream_t cs = c.cst;
-        hipEvent_t ev = after;
-        th = std::thread([dev, cs, ev, kind, pieces] {
+        HIPCHECK(hipStreamWaitEvent(cs, after, 0));
+        th = std::thread([dev, cs, kind, pieces] {
             HIPCHECK(hipSetDevice(dev));
-            HIPCHECK(hipStreamWaitEvent(cs, ev, 0));
             for (const Piece &p : pieces)
                 HIPCHECK(hipMemcpyAsync(p.dst, p.src, p.bytes, kind, cs));
             HIPCHECK(hipStreamSynchronize(cs));
@@ -1437,6 +1513,16 @@ static void run_loop(const std::vector<long> &key, F body)
         return;
     }
     auto it = c.graph_cache.find(key);
+    if (it == c.graph_cache.end() && c.graph_seen.insert(key).second) {
+        // first sight of this loop: run it as it is (a graph pays its capture
+        // and instantiation only when the same loop comes again)
+        if (c.graph_seen.size() > 256) {
+            c.graph_seen.clear();
+            c.graph_seen.insert(key);
+        }
+        body();
+        return;
+    }
     if (it == c.graph_cache.end()) {
         if (c.graph_cache.size() >= 16) {
             for (auto &e : c.graph_cache) (void)hipGraphExecDestroy(e.second.exec);
@@ -1706,10 +1792,6 @@ void propagate_core(const Call &P)
                 obsd = c.obs.as<float>(b.rec_total + 64);
                 if (P.obs_dev) {
                     copy_pieces(obsd, b.roff.data(), P.obs_dev, boff.data(), blen.data(), nb);
-                } else if (c.graphs) {
-                    // a time loop captured into a graph must not run beside
-                    // another thread's copies: these go first, on the stream
-                    for (int k = 0; k < nb; ++k) rec_h2d(obsd + b.roff[k], &P.obs[grp[b0 + k]]);
                 } else {
                     // uploaded while the forward pass runs; joined before the
                     // adjoint sources read them
@@ -1749,9 +1831,7 @@ void propagate_core(const Call &P)
                     launch_sum_sq(b.rec + b.roff[k], (long)recs[k]->ntr * recs[k]->ns, scr,
                                   mis + 2 * grp[b0 + k], c.st);
             }
-            if (P.records_to_host && c.graphs) {
-                for (int k = 0; k < nb; ++k) rec_d2h(recs[k], b.rec + b.roff[k]);
-            } else if (P.records_to_host) {
+            if (P.records_to_host) {
                 // downloaded while the adjoint pass runs (it only reads
                 // b.rec); `down` is joined when this batch's scope ends
                 std::vector<RecordCopies::Piece> pc(nb);
@@ -2184,6 +2264,21 @@ extern "C" void iso_acoustic2d_propagation_engine(Record2D *record2d, Acoustic2d
     if (mode != 0 && !chkpt) fail("<error>: engine mode %d needs a checkpoint", mode);
     if ((mode == 2 || (mode == 1 && precondition == 1)) && (!grad || !grad[0]))
         fail("<error>: engine mode %d needs an output array", mode);
+    if (mode == 2) {
+        if (chkpt->nt != nt || !chkpt->p || !chkpt->status)
+            fail("<error>: checkpoint not allocated for nt=%d", nt);
+        // the reverse pass restarts each segment from the pair the forward
+        // pass stored (get_acoustic2dCheckPointStatus, checkpoint.c:236-265,
+        // finds it through status): every pair must be there.  Fields the
+        // status marks beyond the pairs are recomputed, as the reference's
+        // first reverse pass after mode 1 does; their values are the same.
+        const Schedule sc0(nt);
+        for (int it = 0; it < nt; ++it)
+            if (sc0.ckslot[it] != -1 && chkpt->status[it] != 1)
+                fail("<error>: iso_acoustic2d_propagation_engine mode 2: the checkpoint holds no "
+                     "field at it=%d (status %d); run mode 1 with this checkpoint first",
+                     it, chkpt->status[it]);
+    }
     ensure_ready();
     Ctx &c = ctx();
     const long nmod = (long)nx * nz;
@@ -2228,7 +2323,6 @@ extern "C" void iso_acoustic2d_propagation_engine(Record2D *record2d, Acoustic2d
             }
         }
     } else {
-        if (chkpt->nt != nt || !chkpt->p) fail("<error>: checkpoint not allocated for nt=%d", nt);
         for (int it = 0; it < nt; ++it) {
             if (sc.ckslot[it] < 0) continue;
             HIPCHECK(hipMemcpy2DAsync(b.F(BatchPlan::CK0 + sc.ckslot[it]), g.ld * sizeof(float),

@@ -7,6 +7,7 @@
 #include <stdlib.h>
 
 #include <map>
+#include <set>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -59,6 +60,7 @@ struct Stats {
     FdTile tiles[3];              // launch shape per stencil mode of the last batch
     FdTile tiles_acc[3][3];       // ... and of its launches with the accumulation epilogue
     long graphs_built = 0, graph_launches = 0;
+    long shapes_reused = 0;  // launch shapes taken from a similar tuned geometry
 };
 
 struct GraphEntry {
@@ -84,6 +86,8 @@ struct Ctx {
     size_t mem_cap = 0;           // serving worker: device bytes its batches may use (0: free memory)
     bool capturing = false;
     std::map<std::vector<long>, GraphEntry> graph_cache;
+    std::set<std::vector<long>> graph_seen;  // loops run once; captured when they come again
+    int reuse_similar = 0;        // take launch shapes tuned for similar geometries (serving)
     std::map<std::tuple<int, int, int, int, int, int, int, int, long>, FdTile> tuned;
     std::map<std::tuple<int, int, int, int, int, int, int, int, long>, FdTile> tuned_acc;
     Stats stats;

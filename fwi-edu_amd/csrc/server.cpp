@@ -130,6 +130,8 @@ static void worker_main(Server *s, int w)
     bind_ctx(wk.ctx);
     wk.ctx->graphs = s->graphs;
     wk.ctx->ckpt_bits = s->ckpt_bits;
+    // requests rarely repeat a geometry: take shapes tuned for similar ones
+    wk.ctx->reuse_similar = 1;
     for (;;) {
         ServerJob *sj = nullptr;
         size_t reserve = 0;

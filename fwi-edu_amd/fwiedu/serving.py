@@ -149,6 +149,11 @@ class Done:
     def __init__(self, job):
         self.problem, self.latency, self.queue = job.problem, job.latency, job.queue
         self.misfit, self.nshot = job.misfit, job.w.nshot
+        self.nz, self.nx, self.nt = job.w.nz, job.w.nx, job.w.nt
+
+    def row(self):
+        return {"problem": self.problem, "nz": self.nz, "nx": self.nx, "nt": self.nt,
+                "shots": self.nshot, "latency_s": self.latency, "queue_s": self.queue}
 
     def shots(self):
         return self.nshot

@@ -127,8 +127,8 @@ float calculate_memory_cost_checkpoint(int nt, int NX, int NZ);              /* 
 
 /* ---- optimization.h (GPU reductions / elementwise) ---------------------- */
 
-/* optimization.h:17-18.  filter 2 (none) is supported; 0/1 (FFT band filters)
- * end with an error. */
+/* optimization.h:17-18.  filter 0 (low-pass) and 1 (band-pass) run the
+ * reference's FFT chain through hipFFT; 2 is no filter. */
 void compute_adjoint_source2d(Record2D *record2d_pre, Record2D *record2d_obs, float **vp,
                               float freq, int filter, Record2D *record2d_adjsource);
 void apply_gradient_precondition(float *grad, const float *precond, int n, float alpha); /* :40 */
@@ -181,11 +181,12 @@ void fwiedu_set_lanes(int wpb, int zrun, int pd, int lds);
  * compile of the reference (it is built with -ffast-math) rounds; fewer
  * instructions, results within FP32 rounding of the strict ones. */
 void fwiedu_set_fma(int on);
-/* 1: run each batch's forward and adjoint time loops as HIP graphs, captured
- * on first use and replayed whenever the same batch (same device buffers,
- * sizes, schedule and launch shape) comes again: the next gradient of a
- * session, the next FWI iteration.  0 (default): launch kernel by kernel.
- * Sampled kernel timing skips captured launches. */
+/* 1: run each batch's forward and adjoint time loops as HIP graphs: a loop
+ * runs kernel by kernel the first time, is captured when the same batch (same
+ * device buffers, sizes, schedule and launch shape) comes a second time, and
+ * is replayed from then on: the next gradients of a session, the next FWI
+ * iterations.  0 (default): launch kernel by kernel.  Sampled kernel timing
+ * skips captured launches. */
 void fwiedu_set_graphs(int on);
 /* Field layout of batched launches when the stencil shape is autotuned: -1
  * (default) shots in lanes from 48 shots per launch up, per-shot arenas below;
@@ -214,7 +215,9 @@ void fwiedu_set_checkpoint_bits(int bits);
  * sampled launch); the launch shape of mode m in the last batch, "tile<m>_kind",
  * "tile<m>_wpb", "tile<m>_zrun", "tile<m>_pd", "tile<m>_remap", "tile<m>_zfast"
  * ..., and of its launches that stage their accumulation term (v = 1) or add
- * the staged terms to the FP64 accumulator (v = 2): "tile<m><v>_zrun", ...
+ * the staged terms to the FP64 accumulator (v = 2): "tile<m><v>_zrun", ...;
+ * "graphs_built", "graph_launches", "shapes_reused" (launch shapes a serving
+ * worker took from a similar geometry instead of timing its own).
  * Unknown names return -1. */
 double fwiedu_stat(const char *name);
 void fwiedu_reset_stats(void);

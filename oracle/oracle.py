@@ -23,8 +23,8 @@ def _sig(lib, name, res, args):
 
 
 class Oracle:
-    def __init__(self, precision="f32"):
-        path = os.path.join(HERE, f"liboracle_{precision}.so")
+    def __init__(self, precision="f32", path=None):
+        path = path or os.path.join(HERE, f"liboracle_{precision}.so")
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
         self.lib = ctypes.CDLL(path)

@@ -158,3 +158,36 @@ def test_shot_positions_out_of_the_model_exit_like_err(fsx, sdepth, msg):
     assert r.returncode == 1, (r.stdout, r.stderr)
     assert msg in r.stderr
     assert "returned" not in r.stdout
+
+
+ENGINE_MODE2_UNFILLED = """
+import ctypes
+import numpy as np
+from fwiedu import lib as flib, synth
+from fwiedu.records import Array2D
+L = flib.load()
+w = synth.workload('tiny', nshot=1)
+obs = w.shots()
+r = obs.recs[0]
+nxx = r.ix_max - r.ix_min + 1
+vp = Array2D(w.vp_start[r.ix_min:r.ix_max + 1])
+rho = Array2D(w.rho_start[r.ix_min:r.ix_max + 1])
+ck = flib.Acoustic2dCheckPoint()
+L.acoustic2dCheckPoint_init(ctypes.byref(ck))
+L.alloc_acoustic2dCheckPoint(ctypes.byref(ck), w.nz + 2 * w.pml, nxx + 2 * w.pml, w.nt)
+g = Array2D(np.zeros((nxx, w.nz), np.float32))
+L.iso_acoustic2d_propagation_engine(ctypes.byref(r), ctypes.byref(ck), vp.pp, rho.pp, nxx, w.nz,
+                                    w.dx, w.dz, w.dt, w.nt, w.pml, w.freesurface, 1, 2, 1, 0, g.pp)
+print('returned')
+"""
+
+
+def test_engine_mode2_needs_the_checkpoint_pairs():
+    """The per-shot engine's reverse pass restarts from the checkpoint pairs
+    whose status mode 1 sets (checkpoint.c:133-156, 236-265): a freshly
+    allocated checkpoint (status all 0) ends the call with an error, before
+    any device call."""
+    r = _run_child(ENGINE_MODE2_UNFILLED)
+    assert r.returncode == 1, (r.stdout, r.stderr)
+    assert "the checkpoint holds no field at it=0" in r.stderr
+    assert "returned" not in r.stdout

@@ -212,17 +212,21 @@ def test_batched_equals_plain_and_deterministic(L, oracles):
         assert np.array_equal(r[0], res[0][0]) and np.array_equal(r[1], res[0][1])
 
 
-def test_host_arrays_equal_resident_session(L):
+@pytest.mark.parametrize("graphs", [0, 1])
+def test_host_arrays_equal_resident_session(L, graphs):
     """A gradient on the caller's host arrays (observed records uploaded and
     adjoint sources downloaded by a helper thread while the stencils run) is
     bit-identical to the same gradient of a resident session (records already
     on the device, no helper copies), in one batch and in three batches that
-    reuse the record buffers (2 + 2 + 1 shots)."""
+    reuse the record buffers (2 + 2 + 1 shots); also with the time loops
+    captured into HIP graphs and replayed while the helper copies run."""
     w = synth.workload("tiny", nshot=5, freesurface=1, shot_depth_cells=1, nx=128, nt=400)
-    say("case: host arrays vs session, tiny 128 columns, 5 shots, batches of 2 and 5")
+    say(f"case: host arrays vs session, tiny 128 columns, 5 shots, batches of 2 and 5, "
+        f"graphs {graphs}")
     obs = observed(L, w, True)
     vp, rho = Array2D(w.vp_start), Array2D(w.rho_start)
     out = {}
+    L.fwiedu_set_graphs(graphs)
     try:
         for b in (2, 0):
             L.fwiedu_set_batch(b)
@@ -237,8 +241,13 @@ def test_host_arrays_equal_resident_session(L):
             finally:
                 L.fwiedu_session_destroy(sess)
             out[f"session{b}"] = gs.a.copy()
+            if graphs:
+                # a replay of the captured loops beside the helper's copies
+                pre2, g2 = gradient(L, w, obs, True)
+                assert np.array_equal(g2, g) and np.array_equal(recs(pre2), recs(pre)), b
     finally:
         L.fwiedu_set_batch(0)
+        L.fwiedu_set_graphs(0)
     assert np.any(out["host0"][1] != 0)
     for b in (2, 0):
         assert np.array_equal(out[f"host{b}"][1], out[f"session{b}"]), b
@@ -542,7 +551,9 @@ def test_graph_replays_match_eager(L):
             pre_e, g_e = gradient(L, w, obs, True)
             L.fwiedu_set_graphs(1)
             L.fwiedu_reset_stats()
-            for rep in range(3):
+            # the first run of a loop is eager, the second captures and
+            # replays, the later ones replay
+            for rep in range(4):
                 pre_r, g_r = gradient(L, w, obs, True)
                 assert np.array_equal(g_r, g_e), (batch, layout, bits, rep)
                 assert np.array_equal(recs(pre_r), recs(pre_e)), (batch, layout, bits, rep)
