@@ -81,32 +81,46 @@ def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
 
 
 # HBM bytes per launch of each stencil mode of the c1 gradient at 120 shots in
-# lanes (forward and adjoint: averaged over the gradient's launches, one in five
-# with its accumulation epilogue), per stencil shape (kernel kind 3: lanes with
-# LDS row exchange; columns per block, rows per wave, XCD order; the rows of
-# loads in flight change when a row's loads are issued, not which bytes are
-# loaded): FETCH_SIZE x 2 + WRITE_SIZE from rocprofv3 counter runs scaled by a
-# 512 MiB copy; the runs and the calibration are in
-# profiles/fd_lanes_c1_traffic.md.
+# lanes (forward and adjoint: averaged over the gradient's launches, 3200 plain,
+# 600 staging and 201 adding launches of 4001), per stencil shape (kernel kind
+# 3: lanes with LDS row exchange; kind 4: the same with the register ring;
+# columns per block, rows per wave, XCD order, block order; the rows of loads
+# in flight change when a row's loads are issued, not which bytes are loaded):
+# FETCH_SIZE x 2 + WRITE_SIZE from rocprofv3 counter runs scaled by a 512 MiB
+# copy; the runs and the calibration are in profiles/fd_lanes_c1_traffic.md
+# (kind 3) and profiles/fd_ring_c1_traffic.md (kind 4).
 C1_LANES_TRAFFIC = {
-    "forward": {(3, 8, 16, 1): 380.1e6, (3, 8, 16, 0): 430.0e6,
-                (3, 8, 20, 1): 371.2e6, (3, 8, 20, 0): 421.5e6,
-                (3, 8, 24, 1): 363.7e6, (3, 8, 24, 0): 417.3e6},
-    "reconstruction": {(3, 8, 16, 1): 342.1e6, (3, 8, 16, 0): 393.6e6,
-                       (3, 8, 20, 1): 333.9e6, (3, 8, 20, 0): 385.2e6,
-                       (3, 8, 24, 1): 325.7e6, (3, 8, 24, 0): 381.1e6},
-    "adjoint": {(3, 8, 16, 1): 397.0e6, (3, 8, 16, 0): 448.5e6,
-                (3, 8, 20, 1): 387.8e6, (3, 8, 20, 0): 440.0e6,
-                (3, 8, 24, 1): 382.7e6, (3, 8, 24, 0): 435.8e6},
+    "forward": {(3, 8, 16, 1, 0): 380.1e6, (3, 8, 16, 0, 0): 430.0e6,
+                (3, 8, 20, 1, 0): 371.2e6, (3, 8, 20, 0, 0): 421.5e6,
+                (3, 8, 24, 1, 0): 363.7e6, (3, 8, 24, 0, 0): 417.3e6,
+                (4, 8, 30, 1, 0): 371.8e6, (4, 8, 30, 0, 0): 432.8e6,
+                (4, 8, 20, 1, 0): 391.9e6, (4, 8, 20, 0, 0): 450.9e6,
+                (4, 8, 30, 1, 2): 363.3e6, (4, 8, 30, 0, 2): 432.9e6,
+                (4, 8, 20, 1, 2): 367.3e6, (4, 8, 20, 0, 2): 451.0e6},
+    "reconstruction": {(3, 8, 16, 1, 0): 342.1e6, (3, 8, 16, 0, 0): 393.6e6,
+                       (3, 8, 20, 1, 0): 333.9e6, (3, 8, 20, 0, 0): 385.2e6,
+                       (3, 8, 24, 1, 0): 325.7e6, (3, 8, 24, 0, 0): 381.1e6,
+                       (4, 8, 30, 1, 0): 338.6e6, (4, 8, 30, 0, 0): 403.2e6,
+                       (4, 8, 20, 1, 0): 361.0e6, (4, 8, 20, 0, 0): 423.7e6,
+                       (4, 8, 30, 1, 2): 328.5e6, (4, 8, 30, 0, 2): 403.1e6,
+                       (4, 8, 20, 1, 2): 331.9e6, (4, 8, 20, 0, 2): 423.6e6},
+    "adjoint": {(3, 8, 16, 1, 0): 397.0e6, (3, 8, 16, 0, 0): 448.5e6,
+                (3, 8, 20, 1, 0): 387.8e6, (3, 8, 20, 0, 0): 440.0e6,
+                (3, 8, 24, 1, 0): 382.7e6, (3, 8, 24, 0, 0): 435.8e6,
+                (4, 8, 30, 1, 0): 390.9e6, (4, 8, 30, 0, 0): 451.5e6,
+                (4, 8, 20, 1, 0): 410.6e6, (4, 8, 20, 0, 0): 469.6e6,
+                (4, 8, 30, 1, 2): 381.2e6, (4, 8, 30, 0, 2): 451.7e6,
+                (4, 8, 20, 1, 2): 383.2e6, (4, 8, 20, 0, 2): 469.7e6},
 }
 
 
 def measured_traffic(w, st, mode):
     m = MODE_INDEX[mode]
     kind = int(st[f"tile{m}_kind"])
-    if w.name != "c1" or w.nshot != 120 or kind != 3 or int(st[f"tile{m}_zfast"]) != 0:
+    if w.name != "c1" or w.nshot != 120 or kind not in (3, 4):
         return None
-    key = (kind, int(st[f"tile{m}_wpb"]), int(st[f"tile{m}_zrun"]), int(st[f"tile{m}_remap"]))
+    key = (kind, int(st[f"tile{m}_wpb"]), int(st[f"tile{m}_zrun"]), int(st[f"tile{m}_remap"]),
+           int(st[f"tile{m}_zfast"]))
     return C1_LANES_TRAFFIC.get(mode, {}).get(key)
 
 
@@ -383,8 +397,9 @@ def main():
                     help="time steps of the 400x1200, 256-shot stencil leg beyond the Infinity "
                          "Cache (0 = skip)")
     ap.add_argument("--lanes", default="",
-                    help="force the shots-in-lanes stencil shape WPB,ZRUN,PD,XCD[,LDS] "
-                         "(default: autotuned)")
+                    help="force the shots-in-lanes stencil shape WPB,ZRUN,PD,XCD[,LDS[,ORDER]] "
+                         "(LDS 0 none, 1 LDS row exchange, 2 register ring; ORDER the block "
+                         "order 0, 1 or 2; default: autotuned)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported (the shots "
@@ -400,6 +415,7 @@ def main():
         v = [int(x) for x in args.lanes.split(",")]
         wpb, zrun, pd, xcd = v[:4]
         L.fwiedu_set_xcd_remap(xcd)
+        L.fwiedu_set_tile_order(v[5] if len(v) > 5 else 0)
         L.fwiedu_set_lanes(wpb, zrun, pd, v[4] if len(v) > 4 else 0)
     over = {"nshot": args.nshot} if args.nshot else {}
     if args.nt:
@@ -493,7 +509,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": modes[dom]["achieved"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": modes[dom]["frac"],
                      "traffic": measured_traffic(w, st, dom),
-                     "traffic_source": "profiles/fd_lanes_c1_traffic.md (counter runs of the same "
+                     "traffic_source": "profiles/fd_lanes_c1_traffic.md, "
+                                       "profiles/fd_ring_c1_traffic.md (counter runs of the same "
                                        "stencil shape; null when the autotuner picked a shape not "
                                        "measured there)",
                      "kernel": KERNELS[int(st[f"tile{MODE_INDEX[dom]}_kind"])].split(" ")[0] +

@@ -448,9 +448,10 @@ FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur,
                         t.pd = pd;
                         cand.push_back(t);
                     }
-        // the register-ring LDS stencil: runs of whole unrolled groups (10
-        // rows with 2 or 3 rows of loads in flight, 12 with 4 or 5)
-        for (int pd : {2, 3, 4, 5})
+        // the register-ring LDS stencil: runs of two or three unrolled groups
+        // of 10 rows, 2 or 3 rows of loads in flight (4 or 5 rows in flight,
+        // 12-row groups, never won at 221 x 601 or 400 x 1200)
+        for (int pd : {2, 3})
             for (int k : {2, 3})
                 for (int rm : {0, 1})
                     for (int order : {0, 2}) {
