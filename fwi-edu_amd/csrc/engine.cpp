@@ -627,6 +627,8 @@ struct Tables {
     // a shot per cell), as time-major stage slots rcol[ix] * Sp + lane; the
     // rest stays in the *_left tables of the separate kernels.
     int nci = 0;                  // columns with such a receiver row
+    int rec_zlo = 1 << 30;        // least and greatest receiver row of rrow
+    int rec_zhi = -1;
     const int *rrow = nullptr;    // [Nx]
     const int *rcol = nullptr;    // [Nx]
     const long *rec_slot = nullptr;      // [nci*Sp] record trace offset or -1
@@ -850,6 +852,11 @@ static Tables build_tables(const std::vector<ShotIn> &shots, const Geom &g, int 
     h2d(fL, hl.data(), hl.size() * sizeof(long));
     h2d(fI, hi.data(), hi.size() * sizeof(int));
     t.nci = nci;
+    for (int x = 0; x < (int)rrow.size(); ++x)
+        if (rrow[x] >= 0) {
+            t.rec_zlo = std::min(t.rec_zlo, rrow[x]);
+            t.rec_zhi = std::max(t.rec_zhi, rrow[x]);
+        }
     t.rec_slot = fL;
     t.inj_slot = fL + nslot;
     t.injfull_slot = fL + 2 * nslot;
@@ -884,6 +891,8 @@ static FdArgs base_args(const Geom &g, const float *coef, int S, long shot_strid
     a.cell_stride = cell_stride;
     a.acc_cell = 1;
     a.nshot = S;
+    a.src_zlo = a.rec_zlo = 1 << 30;
+    a.src_zhi = a.rec_zhi = -1;
     a.Nx = g.Nx;
     a.ld = g.ld;
     a.ixs = g.ixs;
@@ -1097,6 +1106,7 @@ struct BatchState {
     unsigned *ckmx = nullptr;     // [ck_mx_words(Sp)] scratch
     int ring0 = 0;                // first imaging-ring field
     int *cell_dev = nullptr;
+    int src_zlo = 1 << 30, src_zhi = -1;  // least and greatest row of the source cells
     float *wav_dev = nullptr;
     Tables tb;
     std::vector<long> roff;
@@ -1134,6 +1144,8 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
     b.adjoint = adjoint;
     b.sc = &sc;
     b.lanes = lanes;
+    b.src_zlo = 1 << 30;
+    b.src_zhi = -1;
     b.ck16 = ck16 && adjoint && sc.nck > 0;
     b.ring0 = b.ck16 ? BatchPlan::RING16 : BatchPlan::CK0 + sc.nck;
     const int S = b.S;
@@ -1188,6 +1200,8 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
                  "samples (has ntr=%d, ns=%d)",
                  r->shotID, (nt - 1) / r->sampleRate + 1, r->ntr, r->ns);
         cells[s] = isx * g.ld + isz;
+        b.src_zlo = std::min(b.src_zlo, isz);
+        b.src_zhi = std::max(b.src_zhi, isz);
         memcpy(&wav[(size_t)s * nt], r->src, nt * sizeof(float));
     }
     char *sb = c.src.as<char>(wav.size() * sizeof(float) + cells.size() * sizeof(int) + 256);
@@ -1290,6 +1304,8 @@ static void forward_loop(BatchState &b, bool save_ckpt, bool precond)
     const Schedule &sc = *b.sc;
     FdArgs a = base_args(b.g, b.coef, b.S, b.SS, b.CS);
     a.src_cell = b.cell_dev;
+    a.src_zlo = b.src_zlo;
+    a.src_zhi = b.src_zhi;
     a.src_wav = b.wav_dev;
     a.nt = b.nt;
     a.acc = b.acc;
@@ -1300,6 +1316,8 @@ static void forward_loop(BatchState &b, bool save_ckpt, bool precond)
     if (fused) {
         a.rrow = b.tb.rrow;
         a.rcol = b.tb.rcol;
+        a.rec_zlo = b.tb.rec_zlo;
+        a.rec_zhi = b.tb.rec_zhi;
         a.stage = b.stage;
         a.nslot = b.nslot;
         a.rec_on = 1;
@@ -1378,6 +1396,8 @@ static void adjoint_loop(BatchState &b)
     float *AW[3] = {b.F(BatchPlan::A0), b.F(BatchPlan::A1), b.F(BatchPlan::A2)};
     FdArgs fr = base_args(g, b.coef, b.S, b.SS, b.CS);
     fr.src_cell = b.cell_dev;
+    fr.src_zlo = b.src_zlo;
+    fr.src_zhi = b.src_zhi;
     fr.src_wav = b.wav_dev;
     fr.nt = nt;
     FdArgs ad = base_args(g, b.coef, b.S, b.SS, b.CS);
@@ -1394,6 +1414,8 @@ static void adjoint_loop(BatchState &b)
                             b.adjfull ? b.adjfull : b.rec, b.stage, b.nslot, nt, c.st);
         ad.rrow = b.tb.rrow;
         ad.rcol = b.tb.rcol;
+        ad.rec_zlo = b.tb.rec_zlo;
+        ad.rec_zhi = b.tb.rec_zhi;
         ad.stage = b.stage;
         ad.nslot = b.nslot;
     }

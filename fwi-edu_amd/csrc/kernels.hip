@@ -1076,9 +1076,12 @@ static void launch_lanes_lds_t(const FdArgs &a0, FdTile tl, hipStream_t st)
 //  * the coefficients of row r + 1 (one 32-byte scalar load of the
 //    cell-interleaved copy) come from a group base plus an immediate offset,
 //    loaded during row r;
-//  * blocks whose rows and columns hold no free-surface row, source or
-//    receiver take a loop with none of those tests; the others run the
-//    epilogue of lanes_update, as fd_lanes_lds_kernel does.
+//  * blocks whose rows hold no free-surface row and miss the row ranges of
+//    the sources and receivers (kernel arguments: a scalar test, made before
+//    any load) take a loop with none of those tests; the others load their
+//    source cells and receiver rows and run the epilogue of lanes_update, as
+//    fd_lanes_lds_kernel does.  Each branch issues its own first loads, so
+//    that no value in flight is moved between registers where they join.
 // Rows past a run's end (the last group of a run) still load, publish and
 // pass the barrier, computing nothing.
 template <int D>
@@ -1148,42 +1151,34 @@ __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
         return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(f, vo, boff, 0));
     };
 
-    // does the block touch a free-surface row, a source or a receiver?
-    const int scell = (MODE != FD_ADJ && a.src_cell) ? a.src_cell[s] : -1;
-    int rr = -1;
+    // does the block meet a free-surface row, a source row or a receiver row?
+    const bool recv = (MODE == FD_FWD && a.rec_on) || (MODE == FD_ADJ && a.inj_on);
+    const bool special = (a.fs && iz0 <= a.pml + 3) ||
+                         (MODE != FD_ADJ && a.src_cell && iz0 <= a.src_zhi && iz1 >= a.src_zlo) ||
+                         (recv && iz0 <= a.rec_zhi && iz1 >= a.rec_zlo);
+    int scell = -1, rr = -1;
     long slot = 0;
     float injv = 0.f;
-    if ((MODE == FD_FWD && a.rec_on) || (MODE == FD_ADJ && a.inj_on)) {
-        if (active) {
-            const int r = a.rrow[ix];
-            if (r >= iz0 && r <= iz1) {
-                rr = r;
-                slot = (long)a.rcol[ix] * CS + s;
-                if (MODE == FD_ADJ) injv = a.stage[(long)(a.it - 1) * a.nslot + slot];
-            }
-        }
-    }
-    const bool src_here = active && scell >= colb + iz0 && scell <= colb + iz1;
-    const int wave_special = (rr >= 0) || __any(src_here) || (a.fs && iz0 <= a.pml + 3);
-    const bool special = __syncthreads_or(wave_special) != 0;
 
     // rings: V holds p1 row iz0 - 3 + j in slot j mod U, P and H (p0 and the
     // halo column) row iz0 + j in slot j mod U
     float V[U], P[U], H[U];
-#pragma unroll
-    for (int j = 0; j < 6; ++j) V[j] = ldc(P1c, (iz0 - 3 + j) * rs4);
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        V[(6 + d) % U] = ldc(P1z, (iz0 + d) * rs4);
-        P[d % U] = ldc(P0c, (iz0 + d) * rs4);
-        H[d % U] = ldc(Hc, (iz0 + d) * rs4);
-    }
     const cfloat8v *c8 = (const cfloat8v *)(a.coef + CO_N * fsz);
     cfloat *cf = (cfloat *)a.coef;
     const frsrc P2 = field_rsrc(a.out, fbytes);
     int rl = (iz0 + D) * rs4;  // byte offset of the row whose loads are issued next
     float8v kc[2];
-    kc[0] = c8[colb + iz0];
+    auto preload = [&]() {
+#pragma unroll
+        for (int j = 0; j < 6; ++j) V[j] = ldc(P1c, (iz0 - 3 + j) * rs4);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            V[(6 + d) % U] = ldc(P1z, (iz0 + d) * rs4);
+            P[d % U] = ldc(P0c, (iz0 + d) * rs4);
+            H[d % U] = ldc(Hc, (iz0 + d) * rs4);
+        }
+        kc[0] = c8[colb + iz0];
+    };
 
     // one group of U rows; CHECK: rows past the run's end compute nothing;
     // ACT: the wave's column is inside the stencil (a compile-time choice on
@@ -1241,17 +1236,32 @@ __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
     using T = std::true_type;
     using F = std::false_type;
     if (special) {
+        // the source cells of the block's lanes and the column's receiver
+        // row, loaded ahead of the rings
+        scell = (MODE != FD_ADJ && a.src_cell) ? a.src_cell[s] : -1;
+        if (recv && active) {
+            const int r = a.rrow[ix];
+            if (r >= iz0 && r <= iz1) {
+                rr = r;
+                slot = (long)a.rcol[ix] * CS + s;
+                if (MODE == FD_ADJ) injv = a.stage[(long)(a.it - 1) * a.nslot + slot];
+            }
+        }
+        preload();
 #pragma unroll 1
         for (int g = 0; g < nfull; ++g) group(g, F(), T(), T());
         if (nfull * U < nrow) group(nfull, T(), T(), T());
-    } else if (active) {
-#pragma unroll 1
-        for (int g = 0; g < nfull; ++g) group(g, F(), F(), T());
-        if (nfull * U < nrow) group(nfull, T(), F(), T());
     } else {
+        preload();
+        if (active) {
 #pragma unroll 1
-        for (int g = 0; g < nfull; ++g) group(g, F(), F(), F());
-        if (nfull * U < nrow) group(nfull, T(), F(), F());
+            for (int g = 0; g < nfull; ++g) group(g, F(), F(), T());
+            if (nfull * U < nrow) group(nfull, T(), F(), T());
+        } else {
+#pragma unroll 1
+            for (int g = 0; g < nfull; ++g) group(g, F(), F(), F());
+            if (nfull * U < nrow) group(nfull, T(), F(), F());
+        }
     }
 }
 

@@ -104,6 +104,11 @@ struct FdArgs {
     long nslot;
     int rec_on;             // FD_FWD: stage[it][slot] = p1 at the receiver row
     int inj_on;             // FD_ADJ: p2 at the receiver row -= stage[it-1][slot] * K dt^2/(dx dz)
+    // Row ranges [lo, hi] of the source cells (src_cell) and of the receiver
+    // rows (rrow); lo > hi: none.  The register-ring stencil tests sources and
+    // receivers only in the blocks whose rows meet them; whoever sets
+    // src_cell or rrow sets these.
+    int src_zlo, src_zhi, rec_zlo, rec_zhi;
 };
 
 void launch_fd(int mode, const FdArgs &a, FdTile tile, hipStream_t st);

@@ -525,6 +525,50 @@ def test_lanes_stencil_receivers_match_separate_kernels(L, sr):
         assert np.array_equal(g_l, g_a), name
 
 
+def deep_shots(w):
+    """Sources whose depths differ from shot to shot (3 to 14 cells) and one
+    receiver row 20 cells deep on every column, so that the free-surface rows,
+    the sources and the receivers fall in different row runs of the stencils."""
+    from fwiedu.records import ShotSet
+    dx, dz = np.float32(w.dx), np.float32(w.dz)
+    sx = [np.float32(c) * dx for c in w.shot_columns()]
+    sz = [np.float32(3 + (5 * s) % 12) * dz for s in range(w.nshot)]
+    rc = np.arange(w.nx).astype(np.float32)
+    gx = [(rc * dx).astype(np.float32) for _ in range(w.nshot)]
+    gz = [np.full(rc.size, np.float32(20) * dz, np.float32) for _ in range(w.nshot)]
+    return ShotSet(sx, sz, gx, gz, w.src, w.nt, w.dt, w.dx, w.dz, w.nx, w.fpeak, w.sample_rate)
+
+
+@pytest.mark.parametrize("fs", [0, 1])
+def test_ring_stencil_sources_and_receivers_below_the_surface_runs(L, oracles, fs):
+    """The register-ring stencil runs its source and receiver epilogue only in
+    the row runs that the batch's source and receiver rows meet.  With sources
+    at six depths and receivers in another run, records and gradients equal
+    the strict-order oracle bit for bit, for runs of 8 to 24 rows."""
+    w = synth.workload("tiny", nshot=6, freesurface=fs, nx=70)
+    say(f"case: ring stencil, sources 3-14 cells deep, receivers 20 deep, fs={fs}")
+    o = oracles["f32s"]
+    obs_o = deep_shots(w)
+    vp, rho = Array2D(w.vp_true), Array2D(w.rho_true)
+    o.propagation(vp.pp, rho.pp, obs_o.recs, w.nshot, w.nz, w.nx, w.dx, w.dz, w.dt, w.nt, w.pml,
+                  w.freesurface, w.freq, float(w.filter), w.precondition, 0, obs_o.recs, None)
+    pre_o, g_o = gradient(o, w, obs_o, False)
+    assert np.abs(g_o).max() > 0
+    try:
+        for zrun, pd, rm in ((8, 2, 0), (10, 3, 1), (24, 2, 1), (12, 5, 0)):
+            L.fwiedu_set_xcd_remap(rm)
+            L.fwiedu_set_lanes(8, zrun, pd, 2)
+            obs_l = deep_shots(w)
+            flib.propagate(L, w, Array2D(w.vp_true), Array2D(w.rho_true), obs_l, 0)
+            assert np.array_equal(recs(obs_l), recs(obs_o)), (zrun, pd, rm)
+            pre_l, g_l = gradient(L, w, obs_l, True)
+            assert np.array_equal(recs(pre_l), recs(pre_o)), (zrun, pd, rm)
+            assert np.array_equal(g_l, g_o), (zrun, pd, rm)
+    finally:
+        L.fwiedu_set_tile(0, 0)
+        L.fwiedu_set_xcd_remap(0)
+
+
 def test_graph_replays_match_eager(L):
     """Time loops replayed from captured HIP graphs give the eager result bit
     for bit on every replay, with eager work and synchronisations between the
