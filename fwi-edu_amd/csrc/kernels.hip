@@ -546,6 +546,10 @@ static void launch_fd_mode(const FdArgs &a, FdTile tl, hipStream_t st)
 // the 7 rows of its z-stencil in registers (one new row per step).
 
 typedef const __attribute__((address_space(4))) float cfloat;
+// read-only tables indexed by a wave-uniform value (the receiver row and
+// column of a wave's column): through the scalar cache, so that waiting for
+// them never waits for the vector loads in flight
+typedef const __attribute__((address_space(4))) int cint;
 
 // Field access in the lanes layout through buffer instructions: the row
 // offset (cell * CS * 4 bytes) is wave-uniform and goes in the scalar offset,
@@ -887,10 +891,10 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
     float injv = 0.f;
     if ((MODE == FD_FWD && a.rec_on) || (MODE == FD_ADJ && a.inj_on)) {
         if (active) {
-            const int r = a.rrow[ix];
+            const int r = ((cint *)a.rrow)[ix];
             if (r >= iz0 && r <= iz1) {
                 rr = r;
-                slot = (long)a.rcol[ix] * CS + s;
+                slot = (long)((cint *)a.rcol)[ix] * CS + s;
                 if (MODE == FD_ADJ) injv = a.stage[(long)(a.it - 1) * a.nslot + slot];
             }
         }
@@ -1258,16 +1262,31 @@ __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
     using T = std::true_type;
     using F = std::false_type;
     if (special) {
-        // the source cells of the block's lanes and the column's receiver
-        // row, loaded ahead of the rings
-        scell = (MODE != FD_ADJ && a.src_cell) ? a.src_cell[s] : -1;
+        // the column's receiver row and stage column (scalar loads), then the
+        // adjoint source to inject there (adjoint) or the source cells of the
+        // block's lanes, ahead of the rings; the per-lane loads go through
+        // resources of zero range where there is nothing to read (value 0: no
+        // cell of the stencil has index 0), so that no load sits in a branch
+        // whose join would wait for it
+        int r = -1, rc = 0;
         if (recv && active) {
-            const int r = a.rrow[ix];
-            if (r >= iz0 && r <= iz1) {
-                rr = r;
-                slot = (long)a.rcol[ix] * CS + s;
-                if (MODE == FD_ADJ) injv = a.stage[(long)(a.it - 1) * a.nslot + slot];
-            }
+            r = ((cint *)a.rrow)[ix];
+            rc = ((cint *)a.rcol)[ix];
+        }
+        const bool hasr = r >= iz0 && r <= iz1;
+        rr = hasr ? r : -1;
+        slot = (long)rc * CS + s;
+        if (MODE == FD_ADJ) {
+            const frsrc St = hasr ? col_rsrc(a.stage, (long)(a.it - 1) * a.nslot, a.nslot * 4)
+                                  : col_rsrc(a.out, 0, 0);
+            injv = __builtin_bit_cast(float,
+                                      __builtin_amdgcn_raw_buffer_load_b32(St, vo, rc * rs4, 0));
+        } else {
+            const frsrc Sc = a.src_cell ? __builtin_amdgcn_make_buffer_rsrc(
+                                              const_cast<int *>(a.src_cell), (short)0,
+                                              a.lz_ngrp * 256, 0x00020000)
+                                        : col_rsrc(a.out, 0, 0);
+            scell = (int)__builtin_amdgcn_raw_buffer_load_b32(Sc, vo, 0, 0);
         }
         preload();
 #pragma unroll 1
