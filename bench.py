@@ -90,27 +90,34 @@ def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
 # copy; the runs and the calibration are in profiles/fd_lanes_c1_traffic.md
 # (kind 3) and profiles/fd_ring_c1_traffic.md (kind 4).
 C1_LANES_TRAFFIC = {
-    "forward": {(3, 8, 16, 1, 0): 380.1e6, (3, 8, 16, 0, 0): 430.0e6,
-                (3, 8, 20, 1, 0): 371.2e6, (3, 8, 20, 0, 0): 421.5e6,
-                (3, 8, 24, 1, 0): 363.7e6, (3, 8, 24, 0, 0): 417.3e6,
-                (4, 8, 30, 1, 0): 371.8e6, (4, 8, 30, 0, 0): 432.8e6,
-                (4, 8, 20, 1, 0): 391.9e6, (4, 8, 20, 0, 0): 450.9e6,
-                (4, 8, 30, 1, 2): 363.3e6, (4, 8, 30, 0, 2): 432.9e6,
-                (4, 8, 20, 1, 2): 367.3e6, (4, 8, 20, 0, 2): 451.0e6},
-    "reconstruction": {(3, 8, 16, 1, 0): 342.1e6, (3, 8, 16, 0, 0): 393.6e6,
-                       (3, 8, 20, 1, 0): 333.9e6, (3, 8, 20, 0, 0): 385.2e6,
-                       (3, 8, 24, 1, 0): 325.7e6, (3, 8, 24, 0, 0): 381.1e6,
-                       (4, 8, 30, 1, 0): 338.6e6, (4, 8, 30, 0, 0): 403.2e6,
-                       (4, 8, 20, 1, 0): 361.0e6, (4, 8, 20, 0, 0): 423.7e6,
-                       (4, 8, 30, 1, 2): 328.5e6, (4, 8, 30, 0, 2): 403.1e6,
-                       (4, 8, 20, 1, 2): 331.9e6, (4, 8, 20, 0, 2): 423.6e6},
-    "adjoint": {(3, 8, 16, 1, 0): 397.0e6, (3, 8, 16, 0, 0): 448.5e6,
-                (3, 8, 20, 1, 0): 387.8e6, (3, 8, 20, 0, 0): 440.0e6,
-                (3, 8, 24, 1, 0): 382.7e6, (3, 8, 24, 0, 0): 435.8e6,
-                (4, 8, 30, 1, 0): 390.9e6, (4, 8, 30, 0, 0): 451.5e6,
-                (4, 8, 20, 1, 0): 410.6e6, (4, 8, 20, 0, 0): 469.6e6,
-                (4, 8, 30, 1, 2): 381.2e6, (4, 8, 30, 0, 2): 451.7e6,
-                (4, 8, 20, 1, 2): 383.2e6, (4, 8, 20, 0, 2): 469.7e6},
+    "forward": {(3, 8, 16, 1, 0): 353.8e6, (3, 8, 16, 1, 2): 349.9e6, (3, 8, 16, 0, 0): 443.6e6,
+                (3, 8, 16, 0, 2): 443.7e6, (3, 8, 20, 1, 0): 360.0e6, (3, 8, 20, 1, 2): 348.2e6,
+                (3, 8, 20, 0, 0): 431.6e6, (3, 8, 20, 0, 2): 431.6e6, (3, 8, 24, 1, 0): 357.4e6,
+                (3, 8, 24, 1, 2): 347.8e6, (3, 8, 24, 0, 0): 426.0e6, (3, 8, 24, 0, 2): 425.9e6,
+                (3, 8, 32, 1, 0): 351.0e6, (3, 8, 32, 1, 2): 347.6e6, (3, 8, 32, 0, 0): 420.1e6,
+                (3, 8, 32, 0, 2): 420.2e6, (4, 8, 20, 1, 0): 363.0e6, (4, 8, 20, 1, 2): 347.8e6,
+                (4, 8, 20, 0, 0): 445.0e6, (4, 8, 20, 0, 2): 444.8e6, (4, 8, 30, 1, 0): 359.0e6,
+                (4, 8, 30, 1, 2): 346.9e6, (4, 8, 30, 0, 0): 429.5e6, (4, 8, 30, 0, 2): 429.5e6},
+    "reconstruction": {(3, 8, 16, 1, 0): 318.1e6, (3, 8, 16, 1, 2): 311.4e6,
+                       (3, 8, 16, 0, 0): 404.1e6, (3, 8, 16, 0, 2): 404.2e6,
+                       (3, 8, 20, 1, 0): 323.9e6, (3, 8, 20, 1, 2): 310.7e6,
+                       (3, 8, 20, 0, 0): 392.9e6, (3, 8, 20, 0, 2): 393.0e6,
+                       (3, 8, 24, 1, 0): 320.6e6, (3, 8, 24, 1, 2): 310.8e6,
+                       (3, 8, 24, 0, 0): 387.6e6, (3, 8, 24, 0, 2): 387.7e6,
+                       (3, 8, 32, 1, 0): 313.4e6, (3, 8, 32, 1, 2): 311.3e6,
+                       (3, 8, 32, 0, 0): 382.1e6, (3, 8, 32, 0, 2): 382.1e6,
+                       (4, 8, 20, 1, 0): 328.7e6, (4, 8, 20, 1, 2): 312.4e6,
+                       (4, 8, 20, 0, 0): 412.1e6, (4, 8, 20, 0, 2): 412.0e6,
+                       (4, 8, 30, 1, 0): 324.7e6, (4, 8, 30, 1, 2): 311.6e6,
+                       (4, 8, 30, 0, 0): 395.4e6, (4, 8, 30, 0, 2): 395.4e6},
+    "adjoint": {(3, 8, 16, 1, 0): 363.6e6, (3, 8, 16, 1, 2): 370.6e6, (3, 8, 16, 0, 0): 463.8e6,
+                (3, 8, 16, 0, 2): 463.9e6, (3, 8, 20, 1, 0): 374.1e6, (3, 8, 20, 1, 2): 366.5e6,
+                (3, 8, 20, 0, 0): 451.5e6, (3, 8, 20, 0, 2): 451.7e6, (3, 8, 24, 1, 0): 375.7e6,
+                (3, 8, 24, 1, 2): 365.6e6, (3, 8, 24, 0, 0): 446.0e6, (3, 8, 24, 0, 2): 445.8e6,
+                (3, 8, 32, 1, 0): 371.9e6, (3, 8, 32, 1, 2): 364.2e6, (3, 8, 32, 0, 0): 439.5e6,
+                (3, 8, 32, 0, 2): 439.7e6, (4, 8, 20, 1, 0): 383.3e6, (4, 8, 20, 1, 2): 368.0e6,
+                (4, 8, 20, 0, 0): 464.9e6, (4, 8, 20, 0, 2): 464.7e6, (4, 8, 30, 1, 0): 378.6e6,
+                (4, 8, 30, 1, 2): 366.7e6, (4, 8, 30, 0, 0): 449.1e6, (4, 8, 30, 0, 2): 449.0e6},
 }
 
 
