@@ -117,6 +117,47 @@ void d2h(void *dst, const void *src, size_t bytes)
     ctx().stats.d2h_bytes += (double)bytes;
 }
 
+static int env_interval(const char *name, int dflt)
+{
+    const char *v = getenv(name);
+    return v ? atoi(v) : dflt;
+}
+
+// propagation.c:846-852 reads shotInterval on every call; the engine reads
+// both intervals at verbose > 1 (:423-441).  A zero or negative interval would
+// divide by zero there; here it is an error once a level needs it.
+Verbose verbose_from_env(int level, int iter)
+{
+    Verbose v;
+    v.level = level;
+    v.iter = iter;
+    v.shot_interval = env_interval("shotInterval", 20);
+    if (level > 1) v.snap_interval = env_interval("snapInterval", 1000);
+    if (level > 0 && v.shot_interval <= 0)
+        fail("<error>: shotInterval must be a positive integer (got %d)", v.shot_interval);
+    if (level > 1 && v.snap_interval <= 0)
+        fail("<error>: snapInterval must be a positive integer (got %d)", v.snap_interval);
+    return v;
+}
+
+void write_bin_file(const char *name, const float *data, size_t n)
+{
+    FILE *fp = fopen(name, "wb");
+    if (!fp) fail("write_bin: failed to open file: %s", name);
+    const size_t k = fwrite(data, sizeof(float), n, fp);
+    fclose(fp);
+    if (k != n) fail("write_bin: expected to write %zu elements, but only wrote %zu", n, k);
+}
+
+// n floats of device memory into a verbose file (the stream's work first).
+static void write_bin_device(const char *name, const float *dev, size_t n)
+{
+    std::vector<float> h(n);
+    HIPCHECK(hipMemcpyAsync(h.data(), dev, n * sizeof(float), hipMemcpyDeviceToHost, ctx().st));
+    HIPCHECK(hipStreamSynchronize(ctx().st));
+    write_bin_file(name, h.data(), n);
+}
+
 Geom make_geom(int nx, int nz, int pml, int fs, float dx, float dz, float dt)
 {
     Geom g;
@@ -1129,8 +1170,53 @@ struct BatchState {
         return tile_acc[mode][!a.acc_on ? 0 : (a.acc_stage ? 1 : 2)];
     }
     FdTile tile;               // the reconstruction sweep's (reported)
+    // verbose >= 2: wavefield files of the batch's shots whose index in the
+    // call (snap_shot[k], -1 for none) is a multiple of shotInterval
+    const Verbose *vb = nullptr;
+    std::vector<int> snap_shot;
     float *F(int k) const { return arena + (long)k * KS; }
 };
+
+// The padded field f (per-shot arenas) of every selected shot of the batch, as
+// the reference's engine writes p1, p1_fwd or p0..p2 at verbose > 1
+// (propagation.c:597-601, 717-723): [Nx][Nz] floats.  The arena stencils
+// write the full free-surface mirror of the cells they compute; what the
+// reference's acoustic_iso2d_freesurface (:76-101) also writes is added here
+// when the field went through it (fs_applied): -0.0 in rows 0..pml of the six
+// edge columns no stencil writes (T9), and in the mirror rows of rows the
+// stencil never reaches (pml + i > Nz - 4, models shallower than pml + 4).
+static void snapshot(const BatchState &b, const char *what, int it, const float *f,
+                     bool fs_applied)
+{
+    Ctx &c = ctx();
+    const Geom &g = b.g;
+    std::vector<float> h((size_t)g.Nx * g.Nz);
+    char name[512];
+    for (int k = 0; k < b.S; ++k) {
+        const int ishot = b.snap_shot[k];
+        if (ishot < 0) continue;
+        HIPCHECK(hipMemcpy2DAsync(h.data(), g.Nz * sizeof(float), f + (long)k * b.SS,
+                                  g.ld * sizeof(float), g.Nz * sizeof(float), g.Nx,
+                                  hipMemcpyDeviceToHost, c.st));
+        HIPCHECK(hipStreamSynchronize(c.st));
+        if (fs_applied && g.fs) {
+            for (int ix = 0; ix < g.Nx; ++ix) {
+                float *col = h.data() + (size_t)ix * g.Nz;
+                const bool edge = ix < g.ixs || ix > g.ixe;
+                for (int i = 0; i <= g.pml; ++i)
+                    if (edge || g.pml + i > g.ize) col[g.pml - i] = -0.0f;
+            }
+        }
+        snprintf(name, sizeof(name), "%s_iter%03d_ishot%04d_it%05d_%d_%d_%.2fm_%.2fm.bin", what,
+                 b.vb->iter, ishot, it, g.Nz, g.Nx, g.dz, g.dx);
+        write_bin_file(name, h.data(), h.size());
+    }
+}
+
+static bool snap_step(const BatchState &b, int it)
+{
+    return b.vb && b.vb->snaps() && it % b.vb->snap_interval == 0;
+}
 
 static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
                           const std::vector<Record2D *> &recs, int nt, bool adjoint, Schedule &sc,
@@ -1349,6 +1435,8 @@ static void forward_loop(BatchState &b, bool save_ckpt, bool precond)
         set_deferral(a, defer, staged, it + MST > b.nt - 1);
         fd_launch(FD_FWD, a, b.cells_per_launch, b.tile_for(FD_FWD, a));
         if (ck && b.ck16) ck_store(b, sc.ckslot[it + 1], out);
+        // propagation.c:597-601: p1 = u(it); u(0) is the zero start field
+        if (snap_step(b, it)) snapshot(b, "wavefield_forward", it, cur, it > 0);
         prev = cur;
         cur = out;
     }
@@ -1425,6 +1513,11 @@ static void adjoint_loop(BatchState &b)
     ad.accstage_stride = b.accstage_stride;
     int staged = 0;
     const float *cur = AW[0], *prev = AW[1];
+    // what the reference's p1_fwd holds (verbose snapshots): zeros at first,
+    // then u(t) of the latest step t that ended a recomputation or imaged
+    // (get_acoustic2dWavefield copies u(t) into it, propagation.c:709)
+    const float *fwd_seen = b.F(BatchPlan::ZERO);
+    int fwd_t = -1;
     for (int it = nt - 1; it >= 0; --it) {
         if (!stored[it]) {
             // get_acoustic2dCheckPointStatus: the segment restarts at the last
@@ -1451,6 +1544,8 @@ static void adjoint_loop(BatchState &b)
                 stored[t] = 1;
                 where[t] = fout;
             }
+            fwd_seen = where[it];
+            fwd_t = it;
         }
         const bool left = fused && it < nt - 1;
         if (b.adjfull)
@@ -1474,6 +1569,17 @@ static void adjoint_loop(BatchState &b)
             ad.u = b.F(BatchPlan::RU);
         }
         fd_launch(FD_ADJ, ad, b.cells_per_launch, b.tile_for(FD_ADJ, ad));
+        if (ad.acc_on) {
+            fwd_seen = ad.u;
+            fwd_t = it;
+        }
+        if (snap_step(b, it)) {
+            // propagation.c:717-723: p1_fwd, and the adjoint p1 of level it
+            // (injected; the start field of it = nt - 1 never went through
+            // the free surface)
+            snapshot(b, "wavefield_forward_reconstruction", it, fwd_seen, fwd_t > 0);
+            snapshot(b, "wavefield_backward", it, cur, it < nt - 1);
+        }
         prev = cur;
         cur = out;
     }
@@ -1582,14 +1688,24 @@ void drop_graphs()
     c.graph_cache.clear();
 }
 
+// Loops that write verbose wavefield files run kernel by kernel (the files
+// are written between launches).
 static void forward_pass(BatchState &b, bool save_ckpt, bool precond)
 {
+    if (b.vb && b.vb->snaps()) {
+        forward_loop(b, save_ckpt, precond);
+        return;
+    }
     run_loop(batch_key(0, b, save_ckpt, precond), [&] { forward_loop(b, save_ckpt, precond); });
 }
 
 static void adjoint_pass(BatchState &b)
 {
     adjoint_clear(b);
+    if (b.vb && b.vb->snaps()) {
+        adjoint_loop(b);
+        return;
+    }
     run_loop(batch_key(1, b, false, false), [&] { adjoint_loop(b); });
 }
 
@@ -1623,7 +1739,9 @@ long lanes_max_shots(long fsz)
     return (0x7FFFFFFFL / (fsz * 4)) / 64 * 64;
 }
 
-static int choose_batch(int n, size_t per_shot, long fsz, bool &lanes)
+// arenas: the batch must run in per-shot arenas (verbose wavefield files need
+// the arena stencils' full free-surface mirror)
+static int choose_batch(int n, size_t per_shot, long fsz, bool &lanes, bool arenas)
 {
     Ctx &c = ctx();
     long Smax;
@@ -1646,7 +1764,12 @@ static int choose_batch(int n, size_t per_shot, long fsz, bool &lanes)
         Smax = std::max<long>(1, (long)(budget / (double)per_shot));
     }
     long S = std::min<long>(n, Smax);
-    if (c.tile_req.tz > 0)
+    if (arenas) {
+        if (c.tile_req.tz > 0 && c.tile_req.kind >= 2)
+            fail("<error>: verbose 2 writes wavefields from the per-shot-arena stencils; a forced "
+                 "shots-in-lanes stencil shape cannot run with it");
+        lanes = false;
+    } else if (c.tile_req.tz > 0)
         lanes = c.tile_req.kind >= 2;
     else if (c.layout_req >= 0)
         lanes = c.layout_req == 1;
@@ -1786,8 +1909,9 @@ void propagate_core(const Call &P)
         for (int s : grp) recmax = std::max<long>(recmax, (long)P.pre[s].ntr * P.pre[s].ns);
         bool lanes = false;
         const bool ck16 = c.ckpt_bits == 16;
-        const int S = choose_batch(
-            (int)grp.size(), per_shot_bytes(g, sc, problem == 1, ck16, recmax, nt), g.fsz, lanes);
+        const int S = choose_batch((int)grp.size(),
+                                   per_shot_bytes(g, sc, problem == 1, ck16, recmax, nt), g.fsz,
+                                   lanes, P.vb.snaps());
         c.stats.batch = S;
         c.stats.layout = lanes ? 1 : 0;
         for (size_t b0 = 0; b0 < grp.size(); b0 += S) {
@@ -1802,6 +1926,31 @@ void propagate_core(const Call &P)
             }
             BatchState b;
             prepare_batch(b, g, coef, recs, nt, problem == 1, sc, lanes, ck16);
+            if (P.vb.snaps()) {
+                b.vb = &P.vb;
+                b.snap_shot.assign(nb, -1);
+                for (int k = 0; k < nb; ++k)
+                    if (P.vb.shot(grp[b0 + k])) b.snap_shot[k] = grp[b0 + k];
+            }
+            // verbose >= 1: the gradient's per-shot files (propagation.c:917-943)
+            auto shot_files = [&](const char *what, bool records, const float *src, long stride) {
+                if (problem != 1 || P.vb.level < 1) return;
+                char name[512];
+                for (int k = 0; k < nb; ++k) {
+                    const int ishot = grp[b0 + k];
+                    if (!P.vb.shot(ishot)) continue;
+                    const Record2D &r = *recs[k];
+                    if (records) {
+                        snprintf(name, sizeof(name), "%s_iter%03d_shot%05d_%d_%d_%.2fms_%.2fm.bin",
+                                 what, P.vb.iter, ishot + 1, r.ns, r.ntr, r.dt * 1e3, r.dtr);
+                        write_bin_device(name, src + b.roff[k], (size_t)r.ntr * r.ns);
+                    } else {
+                        snprintf(name, sizeof(name), "%s_iter%03d_shot%05d_%d_%d_%.2fm_%.2fm.bin",
+                                 what, P.vb.iter, ishot + 1, nz, nxx, P.dz, P.dx);
+                        write_bin_device(name, src + (long)k * stride, (size_t)nxx * nz);
+                    }
+                }
+            };
             // the batch's records in the caller's device buffers (sessions)
             std::vector<long> boff(nb), blen(nb);
             for (int k = 0; k < nb; ++k) {
@@ -1845,9 +1994,11 @@ void propagate_core(const Call &P)
                     for (int k = 0; k < nb; ++k) rec_d2h(recs[k], b.rec + b.roff[k]);
                 continue;
             }
+            shot_files("predicted", true, b.rec, 0);
             up.join();
             adjoint_sources(recs.data(), nb, P.vp, P.freq, P.filter, b.rec, obsd, b.rec,
                             b.roff.data());
+            shot_files("adjsource", true, b.rec, 0);
             if (mis) {
                 double *scr = c.scratch.as<double>(reduce_scratch_doubles() + (size_t)Nshot * 512 + Nshot + 64);
                 for (int k = 0; k < nb; ++k)
@@ -1876,6 +2027,7 @@ void propagate_core(const Call &P)
             }
             adjoint_pass(b);
             launch_finalize(b.acc, b.ASS, b.ACS, gb, nacc, nb, vpw, rhow, (int)nacc, 2.0, 0, c.st);
+            shot_files("gradient_raw", false, gb, nacc);
             if (P.precondition == 1) {
                 double *scr = c.scratch.as<double>((size_t)nb * 512 + nb + reduce_scratch_doubles());
                 launch_shot_precondition(gb, pb, nacc, nb, (int)nacc, scr, c.st);
@@ -1905,9 +2057,21 @@ void propagate_core(const Call &P)
         float *gf = P.grad_dev ? P.grad_dev : c.misc[3].as<float>((size_t)2 * nmod);
         float *pf = c.misc[3].as<float>((size_t)2 * nmod) + nmod;
         launch_to_float(sums, gf, nmod, c.st);
+        char name[512];
+        if (P.vb.level > 0) {
+            // propagation.c:996-1005: before the global preconditioning
+            snprintf(name, sizeof(name), "gradient_raw_iter%03d_%d_%d_%.2fm_%.2fm.bin", P.vb.iter,
+                     nz, nx, P.dz, P.dx);
+            write_bin_device(name, gf, (size_t)nmod);
+        }
         if (P.precondition == 1) {
             launch_accumulate(pout, win_max, Nshot, wi, wi + Nshot, nz, nx, sums + nmod, c.st);
             launch_to_float(sums + nmod, pf, nmod, c.st);
+            if (P.vb.level > 0) {
+                snprintf(name, sizeof(name), "precond_iter%03d_%d_%d_%.2fm_%.2fm.bin", P.vb.iter,
+                         nz, nx, P.dz, P.dx);
+                write_bin_device(name, pf, (size_t)nmod);
+            }
             double *scr = c.scratch.as<double>(reduce_scratch_doubles() + (size_t)Nshot * 512 + Nshot + 64);
             launch_global_precondition(gf, pf, nmod, 0.005f, scr, c.st);
         }
@@ -1935,8 +2099,6 @@ extern "C" void iso_acoustic2d_propagation(float **vp, float **rho, Record2D *re
                                            float filter, int precondition, int problem, int iter,
                                            int verbose, Record2D *record2d_pre, float **grad)
 {
-    (void)iter;
-    (void)verbose;
     check_common(nz, nx, dx, dz, dt, nt, pmlThick, freesurface);
     if (Nshot <= 0) fail("<error>: Nshot must be positive");
     if (problem < 0 || problem > 2) fail("<error>: problem must be 0, 1 or 2 (got %d)", problem);
@@ -1961,6 +2123,7 @@ extern "C" void iso_acoustic2d_propagation(float **vp, float **rho, Record2D *re
     P.precondition = precondition;
     P.problem = problem;
     P.grad = grad;
+    P.vb = verbose_from_env(verbose, iter);
     propagate_core(P);
 }
 
@@ -2099,6 +2262,7 @@ extern "C" void fwiedu_session_iterate(void *sp, FwieduIteration *it, float **vp
     P.filter = (int)it->filter;
     P.pre_dev = pre0;
     P.models_on_device = true;
+    P.vb = verbose_from_env(it->verbose, it->iter);
     double f = 0.0;
     P.misfit = &f;
     propagate_core(P);
@@ -2280,9 +2444,11 @@ extern "C" void iso_acoustic2d_propagation_engine(Record2D *record2d, Acoustic2d
                                                   int pmlThick, int freesurface, int precondition,
                                                   int mode, int iter, int verbose, float **grad)
 {
-    (void)iter;
-    (void)verbose;
     check_common(nz, nx, dx, dz, dt, nt, pmlThick, freesurface);
+    // the reference's engine names its verbose wavefield files after a shot
+    // index it never assigns (propagation.c:399, 597); a call of this engine
+    // is one shot, named shot 0 here
+    const Verbose vb = verbose_from_env(verbose > 1 ? verbose : 0, iter);
     if (mode < 0 || mode > 2) fail("<error>: engine mode must be 0, 1 or 2 (got %d)", mode);
     if (mode != 0 && !chkpt) fail("<error>: engine mode %d needs a checkpoint", mode);
     if ((mode == 2 || (mode == 1 && precondition == 1)) && (!grad || !grad[0]))
@@ -2320,6 +2486,10 @@ extern "C" void iso_acoustic2d_propagation_engine(Record2D *record2d, Acoustic2d
              "stencil needs a batch");
     // the host checkpoint of the reference is FP32
     prepare_batch(b, g, coef, recs, nt, mode != 0, sc, false, false);
+    if (vb.snaps()) {
+        b.vb = &vb;
+        b.snap_shot.assign(1, 0);
+    }
     const long nacc = b.nacc;
     if (mode == 0 || mode == 1) {
         forward_pass(b, mode == 1, mode == 1 && precondition == 1);

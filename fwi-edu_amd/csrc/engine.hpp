@@ -154,6 +154,25 @@ FdTile autotune_acc_tile(int mode, int accv, const FdTile &mode_tile, const FdAr
 void collect_timing();
 void drop_graphs();
 
+// Debug files of the reference's verbose levels, written into the working
+// directory with its names: at level >= 1 iso_acoustic2d_propagation writes,
+// for every shotInterval-th shot of a gradient, the predicted records, the
+// adjoint source and the raw shot gradient, and after the shot loop the raw
+// gradient and the pseudo-Hessian (propagation.c:917-943, 996-1005); at level
+// >= 2 the engine writes the wavefields of those shots every snapInterval steps
+// (:597-601, 717-723).  The intervals come from the environment variables
+// shotInterval (default 20) and snapInterval (default 1000), as there
+// (:423-441, 846-852).
+struct Verbose {
+    int level = 0, iter = 0;
+    int shot_interval = 20, snap_interval = 1000;
+    bool shot(int ishot) const { return level > 0 && ishot % shot_interval == 0; }
+    bool snaps() const { return level > 1; }
+};
+Verbose verbose_from_env(int level, int iter);
+// write_bin (dataio.c:277-304): n floats to a new file, exit(1) on failure.
+void write_bin_file(const char *name, const float *data, size_t n);
+
 // The shot loop of iso_acoustic2d_propagation (propagation.c:854-1035) with
 // the choice of where the observed records come from and whether the
 // predicted / adjoint-source records go back to the host.
@@ -173,6 +192,7 @@ struct Call {
     double *misfit = nullptr;         // 0.5 * sum of squared adjoint sources (problem 1/2)
     float *pre_dev = nullptr;         // device copy of every adjoint source, at obs_off
     bool models_on_device = false;    // c.model[] already hold vp, rho
+    Verbose vb;                       // debug files (level 0: none)
 };
 
 void propagate_core(const Call &P);

@@ -9,7 +9,9 @@
  * [waterdepth=0] [vp_water=1500] [rho_water=1000] [vpmin=0.001]
  * [vpmax=10000] [rhomin=0.001] [rhomax=10000] [verbose=0].
  * Outputs: grad_/vp_/rho_iterNNN_*.bin per iteration, objective_function_N.bin,
- * vp_/rho_inversion_*.bin and the predicted gather shots_p_predicted_*.su.
+ * vp_/rho_inversion_*.bin and the predicted gather shots_p_predicted_*.su;
+ * with verbose > 0 also vp_init_*, rho_init_*, wavelet_*.bin and the
+ * library's per-shot files of each gradient (see fwiedu.h).
  */
 #include <math.h>
 #include <time.h>
@@ -78,6 +80,7 @@ int main(int argc, char **argv)
         obs[s].src = src;
         obs[s].fpeak = fpeak;
     }
+    if (verbose > 0) write_inputs("_init", vp, rho, src, nz, nx, dz, dx, nt, dt);
     create_grid_observe2d(obs, Nshot, dx, dz, nx);
 
     fprintf(stdout, "%s\nInversion starts: %d shots, %d x %d, nt = %d, %d iterations\n",
@@ -106,6 +109,8 @@ int main(int argc, char **argv)
             it.rhomax = rhomax;
             it.gardner = gardner ? 1 : 0;
             it.fraction = 0.01f;
+            it.iter = iter + 1;
+            it.verbose = verbose;
             fwiedu_session_iterate(sess, &it, vp, rho, grad);
             fun1[iter] = it.fun1;
             snprintf(name, sizeof(name), "grad_iter%03d_%05d_%05d_%.2fm_%.2fm.bin", iter + 1, nz, nx,

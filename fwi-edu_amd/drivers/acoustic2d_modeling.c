@@ -7,10 +7,12 @@
  *
  * Parameters (key=value): nx nz nt dx dz dt srcfile vpfile [rhofile]
  * [Gardner=1] [freesurface=0] [pmlThick=30] [waterdepth=0] [vp_water=1500]
- * [rho_water=1000] [sampleRate=1] [prefix=shots_p], and the geometry:
+ * [rho_water=1000] [sampleRate=1] [prefix=shots_p] [verbose=0], and the geometry:
  * jsonfile=, or, without it, the spread of the reference's parameters (in
  * cells; :191-246) [Nshot=99999] [fsx=1] [ds=1] [sdepth=2] [offsmin=0]
  * [offsmax=99999] [dtr=1] [gdepth=2].
+ * verbose > 0 also writes the models and the wavelet (vp_*, rho_*, wavelet_*
+ * .bin); verbose > 1 the library's wavefield files (see fwiedu.h).
  */
 #include <math.h>
 #include <time.h>
@@ -38,6 +40,7 @@ int main(int argc, char **argv)
     const char *rhofile = arg_string("rhofile", 0);
     const char *jsonfile = arg_string("jsonfile", 0);
     const char *prefix = arg_string("prefix", 0);
+    const int verbose = arg_int("verbose", 0, 0);
     if (!gardner && !rhofile) die("didn't specify data file of density model:", "rhofile=");
 
     float *x = (float *)malloc(sizeof(float) * nx), *z = (float *)malloc(sizeof(float) * nz);
@@ -53,6 +56,7 @@ int main(int argc, char **argv)
         read_floats(rhofile, rho[0], (size_t)nx * nz);
     set_water(vp, vpw, wd, nx);
     set_water(rho, rhow, wd, nx);
+    if (verbose > 0) write_inputs("", vp, rho, src, nz, nx, dz, dx, nt, dt);
 
     int Nshot;
     Record2D *obs;
@@ -87,7 +91,7 @@ int main(int argc, char **argv)
     fflush(stdout);
     const double t0 = now();
     iso_acoustic2d_propagation(vp, rho, obs, Nshot, nz, nx, dx, dz, dt, nt, pml, freesurface, 0, 0,
-                               0, 0, 0, 0, pre, NULL);
+                               0, 0, 0, verbose, pre, NULL);
     write_acoustic2d_traces((char *)(prefix ? prefix : "shots_p"), pre, Nshot);
     fprintf(stdout, "Modeling takes %.3f s\n", now() - t0);
     return 0;

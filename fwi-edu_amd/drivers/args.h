@@ -118,6 +118,21 @@ static inline void write_floats(const char *path, const float *src, size_t n)
     fclose(f);
 }
 
+/* The model and wavelet files both drivers write at verbose > 0
+ * (acoustic2d_modeling.c:327-336, acoustic2d_fwi.c:370-381); tag is "" or
+ * "_init". */
+static inline void write_inputs(const char *tag, float **vp, float **rho, const float *src,
+                                int nz, int nx, float dz, float dx, int nt, float dt)
+{
+    char name[512];
+    snprintf(name, sizeof(name), "vp%s_%d_%d_%.2fm_%.2fm.bin", tag, nz, nx, dz, dx);
+    write_floats(name, vp[0], (size_t)nz * nx);
+    snprintf(name, sizeof(name), "rho%s_%d_%d_%.2fm_%.2fm.bin", tag, nz, nx, dz, dx);
+    write_floats(name, rho[0], (size_t)nz * nx);
+    snprintf(name, sizeof(name), "wavelet_%d_%.2fms.bin", nt, dt * 1000);
+    write_floats(name, src, (size_t)nt);
+}
+
 /* set_waterlayer_value (common.c:898-924): rows iz < wd of every column */
 static inline void set_water(float **m, float value, int wd, int nx)
 {

@@ -36,7 +36,8 @@ class FwieduIteration(ctypes.Structure):
                 ("waterdepth", c_int), ("vp_water", c_float), ("rho_water", c_float),
                 ("vpmin", c_float), ("vpmax", c_float), ("rhomin", c_float), ("rhomax", c_float),
                 ("gardner", c_int), ("fraction", c_float),
-                ("fun1", c_float), ("fun2", c_float), ("steplen", c_float), ("scalar", c_float)]
+                ("fun1", c_float), ("fun2", c_float), ("steplen", c_float), ("scalar", c_float),
+                ("iter", c_int), ("verbose", c_int)]
 
 
 def _sig(lib, name, res, args):
@@ -157,11 +158,12 @@ def stat(name):
     return load().fwiedu_stat(name.encode())
 
 
-def propagate(lib, w, vp, rho, obs, problem, pre=None, grad=None, precondition=None):
+def propagate(lib, w, vp, rho, obs, problem, pre=None, grad=None, precondition=None, iter=1,
+              verbose=0):
     """iso_acoustic2d_propagation with a synth.Workload's parameters.
     vp/rho are records.Array2D; obs/pre are records.ShotSet."""
     pc = w.precondition if precondition is None else precondition
     lib.iso_acoustic2d_propagation(
         vp.pp, rho.pp, obs.recs, w.nshot, w.nz, w.nx, w.dx, w.dz, w.dt, w.nt, w.pml,
-        w.freesurface, w.freq, float(w.filter), pc, problem, 1, 0,
+        w.freesurface, w.freq, float(w.filter), pc, problem, iter, verbose,
         (pre or obs).recs, grad.pp if grad is not None else None)

@@ -77,7 +77,15 @@ typedef struct
  * problem 1: gradient; record2d_pre[].traces_p holds the adjoint source on return
  *            and grad[nx][nz] the (preconditioned when precondition==1) gradient.
  * problem 2: model + adjoint source (line-search trial) into record2d_pre[].traces_p.
- * Shots that share a simulation window and fpeak advance together on the GPU. */
+ * Shots that share a simulation window and fpeak advance together on the GPU.
+ * verbose >= 1 writes the reference's debug files of a gradient into the working
+ * directory (predicted_*, adjsource_*, gradient_raw_* of every shotInterval-th
+ * shot, gradient_raw_* and precond_* of the call: propagation.c:917-944,
+ * 996-1005); verbose >= 2 also the wavefields of those shots every snapInterval
+ * steps (wavefield_forward_*, wavefield_forward_reconstruction_*,
+ * wavefield_backward_*: :597-601, 717-723), named after the shot's index in the
+ * call, from the per-shot-arena stencils run kernel by kernel.  shotInterval
+ * (default 20) and snapInterval (default 1000) are read from the environment. */
 void iso_acoustic2d_propagation(float **vp, float **rho, Record2D *record2d_obs, int Nshot, int nz,
                                 int nx, float dx, float dz, float dt, int nt, int pmlThick,
                                 int freesurface, float freq, float filter, int precondition,
@@ -88,7 +96,9 @@ void iso_acoustic2d_propagation(float **vp, float **rho, Record2D *record2d_obs,
  * Note the (nx, nz) order.  mode 0 forward; mode 1 forward that fills the host
  * checkpoint chkpt (pairs at multiples of dCheckPoint, status set) and, when
  * precondition==1, the pseudo-Hessian into grad[nx][nz]; mode 2 adjoint that
- * reads the checkpoint pairs from chkpt and writes the gradient to grad. */
+ * reads the checkpoint pairs from chkpt and writes the gradient to grad.
+ * verbose >= 2 writes the wavefield files of iso_acoustic2d_propagation, named
+ * as shot 0. */
 void iso_acoustic2d_propagation_engine(Record2D *record2d, Acoustic2dCheckPoint *chkpt,
                                        float **vpm, float **rhom, int nx, int nz, float dx,
                                        float dz, float dt, int nt, int pmlThick, int freesurface,
@@ -269,6 +279,10 @@ typedef struct FwieduIteration {
     int gardner;
     float fraction;
     float fun1, fun2, steplen, scalar;
+    /* iter and verbose of the driver's two iso_acoustic2d_propagation calls
+     * (acoustic2d_fwi.c:411-416, 441-446): the debug files of the verbose
+     * levels, as that function writes them; 0 writes none */
+    int iter, verbose;
 } FwieduIteration;
 void fwiedu_session_iterate(void *session, FwieduIteration *it, float **vp, float **rho,
                             float **grad);

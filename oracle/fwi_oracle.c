@@ -69,6 +69,59 @@ static void *xcalloc(size_t n, size_t sz)
 EXPORT int oracle_is_f64(void) { return IS_F64 ? 1 : 0; }
 
 /* ------------------------------------------------------------------------- */
+/* Verbose debug files (propagation.c:419-441, 597-601, 717-723, 846-852,
+ * 917-943, 996-1005; write_bin, dataio.c:277-304).  The engine's file names
+ * use a shot index the reference never assigns (propagation.c:399): the
+ * oracle passes the shot loop's index (vb_shot), and 0 for a direct engine call. */
+static int vb_level = 0, vb_iter = 0, vb_shot = 0;
+static int vb_shot_interval = 20, vb_snap_interval = 1000;
+
+static int env_int(const char *name, int dflt)
+{
+    const char *v = getenv(name);
+    return v ? atoi(v) : dflt;
+}
+
+static void vb_write(const char *name, const real *x, size_t n)
+{
+    FILE *fp = fopen(name, "wb");
+    if (!fp)
+        die("write_bin: cannot open file");
+    for (size_t i = 0; i < n; i++)
+    {
+        float v = (float)x[i];
+        if (fwrite(&v, sizeof(float), 1, fp) != 1)
+            die("write_bin: short write");
+    }
+    fclose(fp);
+}
+
+static void vb_write_f(const char *name, const float *x, size_t n)
+{
+    FILE *fp = fopen(name, "wb");
+    if (!fp)
+        die("write_bin: cannot open file");
+    if (fwrite(x, sizeof(float), n, fp) != n)
+        die("write_bin: short write");
+    fclose(fp);
+}
+
+/* the engine's snapshot condition: verbose > 1, every shotInterval-th shot,
+ * every snapInterval-th step */
+static int vb_snap(int it)
+{
+    return vb_level > 1 && vb_shot % vb_shot_interval == 0 && it % vb_snap_interval == 0;
+}
+
+static void vb_field(const char *what, int it, const real *p, int Nz, int Nx, float dz, float dx)
+{
+    char name[512];
+    snprintf(name, sizeof(name), "%s_iter%03d_ishot%04d_it%05d_%d_%d_%.2fm_%.2fm.bin", what,
+             vb_iter, vb_shot, it, Nz, Nx, dz, dx);
+    vb_write(name, p, (size_t)Nz * Nx);
+}
+
+/* ------------------------------------------------------------------------- */
 /* Stencil: propagation.c:13-74.  Same operand order as the reference line 66-71. */
 static void fd_sweep(real *restrict p2, const real *restrict p1, const real *restrict p0,
                      const real *invrho, const real *invrhox, const real *invrhoz,
@@ -949,6 +1002,14 @@ static void engine(OracleRecord2D *rec, Ckpt *ck, const float *vpm, const float 
 {
     const int N = 3;
     Coefs c;
+    if (vb_level > 1)
+    {
+        /* lines 423-441 */
+        vb_shot_interval = env_int("shotInterval", 20);
+        vb_snap_interval = env_int("snapInterval", 1000);
+        if (vb_shot_interval <= 0 || vb_snap_interval <= 0)
+            die("shotInterval and snapInterval must be positive");
+    }
     make_coefs(&c, vpm, rhom, nx, nz, pml, dx, dz, dt, rec->fpeak);
     const int Nx = c.Nx, Nz = c.Nz;
     const size_t Np = (size_t)Nx * Nz;
@@ -1009,6 +1070,9 @@ static void engine(OracleRecord2D *rec, Ckpt *ck, const float *vpm, const float 
                         acc[(size_t)(ix - pml) * nz + iz - pml] += (double)(v * v);
                     }
             }
+            /* lines 597-601 */
+            if (vb_snap(it))
+                vb_field("wavefield_forward", it, p1, Nz, Nx, dz, dx);
             tmp = p0;
             p0 = p1;
             p1 = p2;
@@ -1036,6 +1100,10 @@ static void engine(OracleRecord2D *rec, Ckpt *ck, const float *vpm, const float 
         for (int i = 0; i < ntr; i++)
             adj[i] = adjblk + (size_t)i * nt;
         oracle_interpolate_traces(rec->traces_p, ntr, rec->ns, sr, adj);
+        /* the content of the reference's p1_fwd, which its verbose snapshot
+         * writes: zeros (line 498), the last recomputed field, or u(it) copied
+         * in for imaging (line 709) */
+        real *pf = (vb_level > 1) ? xcalloc(Np, sizeof(real)) : NULL;
 
         for (int it = nt - 1; it >= 0; --it)
         {
@@ -1067,6 +1135,8 @@ static void engine(OracleRecord2D *rec, Ckpt *ck, const float *vpm, const float 
                         f1 = f2;
                         f2 = tmp;
                     }
+                    if (pf)
+                        memcpy(pf, f1, Np * sizeof(real));
                 }
             }
             for (int itr = 0; itr < ntr; ++itr)
@@ -1080,6 +1150,8 @@ static void engine(OracleRecord2D *rec, Ckpt *ck, const float *vpm, const float 
             if (it % MST == 0)
             {
                 const real *u = ckpt_field(ck, it);
+                if (pf)
+                    memcpy(pf, u, Np * sizeof(real));
 #pragma omp parallel for schedule(static)
                 for (int ix = pml; ix < pml + nx; ++ix)
                     for (int iz = pml; iz < pml + nz; ++iz)
@@ -1088,6 +1160,12 @@ static void engine(OracleRecord2D *rec, Ckpt *ck, const float *vpm, const float 
                         acc[(size_t)(ix - pml) * nz + iz - pml] +=
                             (double)((p2[i] - (real)2.0f * p1[i] + p0[i]) * invdtdt * u[i]);
                     }
+            }
+            /* lines 717-723 */
+            if (vb_snap(it))
+            {
+                vb_field("wavefield_forward_reconstruction", it, pf, Nz, Nx, dz, dx);
+                vb_field("wavefield_backward", it, p1, Nz, Nx, dz, dx);
             }
             tmp = p0;
             p0 = p1;
@@ -1103,6 +1181,7 @@ static void engine(OracleRecord2D *rec, Ckpt *ck, const float *vpm, const float 
         free(f0);
         free(f1);
         free(f2);
+        free(pf);
         free(adj);
         free(adjblk);
     }
@@ -1120,11 +1199,17 @@ static void propagation(float **vp, float **rho, OracleRecord2D *obs, int Nshot,
                         float filter, int precondition, int problem, OracleRecord2D *pre,
                         float **grad, int fullstore)
 {
+    /* lines 846-852 */
+    vb_shot_interval = env_int("shotInterval", 20);
+    if (vb_level > 0 && vb_shot_interval <= 0)
+        die("shotInterval must be positive");
+    char name[512];
     if (problem == 0 || problem == 2)
     {
         for (int s = 0; s < Nshot; s++)
         {
             int ix0 = obs[s].ix_min, nxx = obs[s].ix_max - ix0 + 1;
+            vb_shot = s;
             engine(&pre[s], NULL, vp[0] + (size_t)ix0 * nz, rho[0] + (size_t)ix0 * nz, nxx, nz,
                    dx, dz, dt, nt, pml, fs, precondition, 0, NULL);
             if (problem == 2)
@@ -1146,9 +1231,32 @@ static void propagation(float **vp, float **rho, OracleRecord2D *obs, int Nshot,
         float *gs = xcalloc((size_t)nxx * nz, sizeof(float));
         float *ps = (precondition == 1) ? xcalloc((size_t)nxx * nz, sizeof(float)) : NULL;
         const float *vw = vp[0] + (size_t)ix0 * nz, *rw = rho[0] + (size_t)ix0 * nz;
+        const int out = vb_level > 0 && s % vb_shot_interval == 0;
+        const size_t nrec = (size_t)pre[s].ns * pre[s].ntr;
+        vb_shot = s;
         engine(&pre[s], &ck, vw, rw, nxx, nz, dx, dz, dt, nt, pml, fs, precondition, 1, ps);
+        /* lines 917-922 */
+        if (out)
+        {
+            snprintf(name, sizeof(name), "predicted_iter%03d_shot%05d_%d_%d_%.2fms_%.2fm.bin",
+                     vb_iter, s + 1, pre[s].ns, pre[s].ntr, pre[s].dt * 1e3, pre[s].dtr);
+            vb_write_f(name, pre[s].traces_p[0], nrec);
+        }
         oracle_adjoint_source(&pre[s], &obs[s], vp, freq, (int)filter, &pre[s]);
+        if (out)
+        {
+            snprintf(name, sizeof(name), "adjsource_iter%03d_shot%05d_%d_%d_%.2fms_%.2fm.bin",
+                     vb_iter, s + 1, pre[s].ns, pre[s].ntr, pre[s].dt * 1e3, pre[s].dtr);
+            vb_write_f(name, pre[s].traces_p[0], nrec);
+        }
         engine(&pre[s], &ck, vw, rw, nxx, nz, dx, dz, dt, nt, pml, fs, precondition, 2, gs);
+        /* lines 939-944 */
+        if (out)
+        {
+            snprintf(name, sizeof(name), "gradient_raw_iter%03d_shot%05d_%d_%d_%.2fm_%.2fm.bin",
+                     vb_iter, s + 1, nz, nxx, dz, dx);
+            vb_write_f(name, gs, (size_t)nz * nxx);
+        }
         if (precondition == 1)
             oracle_shot_precondition(gs, ps, nxx * nz);
         for (size_t i = 0; i < (size_t)nxx * nz; i++)
@@ -1162,11 +1270,24 @@ static void propagation(float **vp, float **rho, OracleRecord2D *obs, int Nshot,
     }
     for (size_t i = 0; i < n; i++)
         grad[0][i] = (float)gsum[i];
+    /* lines 996-1005 */
+    if (vb_level > 0)
+    {
+        snprintf(name, sizeof(name), "gradient_raw_iter%03d_%d_%d_%.2fm_%.2fm.bin", vb_iter, nz, nx,
+                 dz, dx);
+        vb_write_f(name, grad[0], n);
+    }
     if (precondition == 1)
     {
         float *pf = xcalloc(n, sizeof(float));
         for (size_t i = 0; i < n; i++)
             pf[i] = (float)psum[i];
+        if (vb_level > 0)
+        {
+            snprintf(name, sizeof(name), "precond_iter%03d_%d_%d_%.2fm_%.2fm.bin", vb_iter, nz, nx,
+                     dz, dx);
+            vb_write_f(name, pf, n);
+        }
         oracle_global_precondition(grad[0], pf, (int)n, 0.005f);
         free(pf);
     }
@@ -1181,6 +1302,23 @@ EXPORT void oracle_propagation(float **vp, float **rho, OracleRecord2D *obs, int
 {
     propagation(vp, rho, obs, Nshot, nz, nx, dx, dz, dt, nt, pmlThick, freesurface, freq, filter,
                 precondition, problem, pre, grad, 0);
+}
+
+/* oracle_propagation with the reference's iter and verbose arguments: the
+ * debug files of the verbose levels in the working directory. */
+EXPORT void oracle_propagation_verbose(float **vp, float **rho, OracleRecord2D *obs, int Nshot,
+                                       int nz, int nx, float dx, float dz, float dt, int nt,
+                                       int pmlThick, int freesurface, float freq, float filter,
+                                       int precondition, int problem, OracleRecord2D *pre,
+                                       float **grad, int iter, int verbose)
+{
+    vb_level = verbose;
+    vb_iter = iter;
+    propagation(vp, rho, obs, Nshot, nz, nx, dx, dz, dt, nt, pmlThick, freesurface, freq, filter,
+                precondition, problem, pre, grad, 0);
+    vb_level = 0;
+    vb_iter = 0;
+    vb_shot = 0;
 }
 
 EXPORT void oracle_gradient_fullstore(float **vp, float **rho, OracleRecord2D *obs, int Nshot,

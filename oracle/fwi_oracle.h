@@ -45,6 +45,14 @@ void oracle_propagation(float **vp, float **rho, OracleRecord2D *obs, int Nshot,
                         float freq, float filter, int precondition, int problem,
                         OracleRecord2D *pre, float **grad);
 
+/* oracle_propagation with iter and verbose: the reference's verbose debug
+ * files (predicted / adjsource / gradient_raw / precond at level >= 1,
+ * wavefield snapshots at level >= 2) in the working directory. */
+void oracle_propagation_verbose(float **vp, float **rho, OracleRecord2D *obs, int Nshot, int nz,
+                                int nx, float dx, float dz, float dt, int nt, int pmlThick,
+                                int freesurface, float freq, float filter, int precondition,
+                                int problem, OracleRecord2D *pre, float **grad, int iter,
+                                int verbose);
 /* Same as oracle_propagation(problem=1) but the adjoint pass reads forward
  * wavefields stored during the forward pass (every MST steps) instead of the
  * checkpoint/recompute schedule.  Used to pin the schedule logic. */

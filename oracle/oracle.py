@@ -34,6 +34,8 @@ class Oracle:
         prop_args = [c_float_pp, c_float_pp, R, c_int, c_int, c_int, c_float, c_float, c_float,
                      c_int, c_int, c_int, c_float, c_float, c_int, c_int, R, c_float_pp]
         self.propagation = _sig(L, "oracle_propagation", None, prop_args)
+        self.propagation_verbose = _sig(L, "oracle_propagation_verbose", None,
+                                        prop_args + [c_int, c_int])
         self.gradient_fullstore = _sig(L, "oracle_gradient_fullstore", None,
                                        prop_args[:15] + [R, c_float_pp])
         self.checkpoint_plan_ = _sig(L, "oracle_checkpoint_plan", None,

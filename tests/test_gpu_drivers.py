@@ -8,7 +8,11 @@ on the library's C ABI, run as separate processes on files:
     gdepth) writes the same file;
   * FWI: that gather, a smoothed start model and one iteration; the written
     model and gradient are bit-identical to fwiedu_session_iterate called
-    in-process on the same inputs, and the final predicted gather is written.
+    in-process on the same inputs, and the final predicted gather is written;
+  * verbose=1: both drivers write their input models and wavelet
+    (acoustic2d_modeling.c:327-336, acoustic2d_fwi.c:370-381), and the FWI
+    driver's gradient writes the library's per-shot debug files, the same
+    bytes as the in-process iteration with iter 1, verbose 1.
 """
 import ctypes
 import json
@@ -61,7 +65,7 @@ def traces(rec):
     return np.ctypeslib.as_array(rec.traces_p[0], (rec.ntr * rec.ns,)).copy()
 
 
-def test_modeling_and_fwi_drivers(L, tmp_path):
+def test_modeling_and_fwi_drivers(L, tmp_path, monkeypatch):
     w = synth.workload("tiny", nshot=3, freesurface=1, shot_depth_cells=1)
     say("case: C drivers, tiny 30x48, 3 shots")
     d = str(tmp_path)
@@ -80,9 +84,16 @@ def test_modeling_and_fwi_drivers(L, tmp_path):
     common = [f"nx={w.nx}", f"nz={w.nz}", f"nt={w.nt}", f"dx={w.dx}", f"dz={w.dz}", f"dt={w.dt}",
               f"pmlThick={w.pml}", "freesurface=1", "Gardner=0", "srcfile=src.bin"]
     run([os.path.join(BIN, "acoustic2d_modeling")] + common +
-        ["vpfile=vp.bin", "rhofile=rho.bin", "jsonfile=acq.json"], d)
+        ["vpfile=vp.bin", "rhofile=rho.bin", "jsonfile=acq.json", "verbose=1"], d)
     su = [f for f in os.listdir(d) if f.startswith("shots_p_") and f.endswith(".su")]
     assert len(su) == 1, os.listdir(d)
+    mtag = f"{w.nz}_{w.nx}_{w.dz:.2f}m_{w.dx:.2f}m.bin"
+    wav = f"wavelet_{w.nt}_{w.dt * 1000:.2f}ms.bin"
+    assert np.array_equal(np.fromfile(os.path.join(d, "vp_" + mtag), np.float32),
+                          w.vp_true.ravel())
+    assert np.array_equal(np.fromfile(os.path.join(d, "rho_" + mtag), np.float32),
+                          w.rho_true.ravel())
+    assert np.array_equal(np.fromfile(os.path.join(d, wav), np.float32), w.src)
 
     # in-process modeling on the same model and geometry
     x = (np.arange(w.nx) * np.float32(w.dx)).astype(np.float32)
@@ -113,7 +124,8 @@ def test_modeling_and_fwi_drivers(L, tmp_path):
     run([os.path.join(BIN, "acoustic2d_fwi")] + common +
         ["vpfile=vp0.bin", "rhofile=rho0.bin", f"recordfile={su[0]}", f"Nshot={n}",
          "iteration=1", "freq=0", "filter=2", "rx=3", "rz=3", "precondition=1",
-         f"waterdepth={w.waterdepth}", "vpmin=1400", "vpmax=5000", "rhomin=1000", "rhomax=3000"], d)
+         f"waterdepth={w.waterdepth}", "vpmin=1400", "vpmax=5000", "rhomin=1000", "rhomax=3000",
+         "verbose=1"], d)
     tag = f"{w.nz:05d}_{w.nx:05d}_{w.dz:.2f}m_{w.dx:.2f}m"
     vp_drv = np.fromfile(os.path.join(d, f"vp_iter001_{tag}.bin"), np.float32).reshape(w.nx, w.nz)
     g_drv = np.fromfile(os.path.join(d, f"grad_iter001_{tag}.bin"), np.float32).reshape(w.nx, w.nz)
@@ -133,8 +145,12 @@ def test_modeling_and_fwi_drivers(L, tmp_path):
                                    w.pml, 1, 0.0, 2.0, 1)
     p = flib.FwieduIteration(freq=0.0, filter=2.0, rz=3, rx=3, waterdepth=w.waterdepth,
                              vp_water=1500.0, rho_water=1000.0, vpmin=1400.0, vpmax=5000.0,
-                             rhomin=1000.0, rhomax=3000.0, gardner=0, fraction=0.01)
+                             rhomin=1000.0, rhomax=3000.0, gardner=0, fraction=0.01, iter=1,
+                             verbose=1)
     vg, rg, gg = (Array2D(np.zeros((w.nx, w.nz), np.float32)) for _ in range(3))
+    dv = tmp_path / "inproc"
+    dv.mkdir()
+    monkeypatch.chdir(dv)
     try:
         L.fwiedu_session_iterate(sess, ctypes.byref(p), vg.pp, rg.pp, gg.pp)
     finally:
@@ -142,3 +158,22 @@ def test_modeling_and_fwi_drivers(L, tmp_path):
     say(f"  fun1 {p.fun1:.6e} step {p.steplen:.6e}")
     assert np.array_equal(vp_drv, vg.a)
     assert np.array_equal(g_drv, gg.a)
+
+    # verbose=1: the start model as the driver holds it after the water rows,
+    # and the gradient's debug files (shot 1 of 3 at shotInterval 20)
+    assert np.array_equal(np.fromfile(os.path.join(d, "vp_init_" + mtag), np.float32),
+                          vp0.a.ravel())
+    assert np.array_equal(np.fromfile(os.path.join(d, "rho_init_" + mtag), np.float32),
+                          rho0.a.ravel())
+    assert np.array_equal(np.fromfile(os.path.join(d, wav), np.float32), w.src)
+    dbg = sorted(f for f in os.listdir(dv) if f.endswith(".bin"))
+    r = obs[0]
+    nxx = r.ix_max - r.ix_min + 1
+    assert dbg == sorted([
+        "adjsource_iter001_shot00001_%d_%d_%.2fms_%.2fm.bin" % (r.ns, r.ntr, r.dt * 1e3, r.dtr),
+        "predicted_iter001_shot00001_%d_%d_%.2fms_%.2fm.bin" % (r.ns, r.ntr, r.dt * 1e3, r.dtr),
+        "gradient_raw_iter001_shot00001_%d_%d_%.2fm_%.2fm.bin" % (w.nz, nxx, w.dz, w.dx),
+        "gradient_raw_iter001_" + mtag, "precond_iter001_" + mtag])
+    for f in dbg:
+        with open(os.path.join(d, f), "rb") as fa, open(os.path.join(dv, f), "rb") as fb:
+            assert fa.read() == fb.read(), f
