@@ -546,6 +546,9 @@ extern "C" void fwiedu_release(void)
     if (!c.ready) return;
     sync();
     release_buffers(c);
+    // what the buffers held goes back to the device, not only to the pool
+    sync();
+    trim_pool();
 }
 
 extern "C" int fwiedu_device_ok(void)

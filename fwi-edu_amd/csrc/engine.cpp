@@ -31,24 +31,90 @@ void fail(const char *fmt, ...)
     exit(1);
 }
 
+// Device buffers come from the device's stream-ordered memory pool.  hipFree
+// performs an implicit hipDeviceSynchronize: in a serving process a worker
+// whose buffers grow for a larger job would wait there for every other
+// worker's queued time loops (seconds), and so would its job.  A pooled
+// buffer is returned with hipFreeAsync in the order of the owner's stream,
+// and the pool keeps what is returned (release threshold: everything), so
+// no allocation or return waits for the device; pool_idle_bytes counts that
+// memory as free where the engine sizes its batches, fwiedu_release trims it.
+static std::once_flag g_pool_once;
+static bool g_pool = false;
+static hipMemPool_t g_mempool = nullptr;
+
+static void pool_setup()
+{
+    std::call_once(g_pool_once, [] {
+        int dev = 0, ok = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return;
+        if (hipDeviceGetAttribute(&ok, hipDeviceAttributeMemoryPoolsSupported, dev) != hipSuccess ||
+            !ok)
+            return;
+        if (hipDeviceGetDefaultMemPool(&g_mempool, dev) != hipSuccess) return;
+        uint64_t keep = UINT64_MAX;
+        HIPCHECK(hipMemPoolSetAttribute(g_mempool, hipMemPoolAttrReleaseThreshold, &keep));
+        g_pool = true;
+    });
+}
+
+size_t pool_idle_bytes()
+{
+    if (!g_pool) return 0;
+    uint64_t reserved = 0, used = 0;
+    HIPCHECK(hipMemPoolGetAttribute(g_mempool, hipMemPoolAttrReservedMemCurrent, &reserved));
+    HIPCHECK(hipMemPoolGetAttribute(g_mempool, hipMemPoolAttrUsedMemCurrent, &used));
+    return reserved > used ? (size_t)(reserved - used) : 0;
+}
+
+void trim_pool()
+{
+    if (g_pool) HIPCHECK(hipMemPoolTrimTo(g_mempool, 0));
+}
+
 void *DevBuf::get(size_t bytes)
 {
     if (bytes == 0) bytes = 256;
     if (bytes > cap) {
-        if (p) HIPCHECK(hipFree(p));
-        p = nullptr;
-        cap = 0;
-        HIPCHECK(hipMalloc(&p, bytes));
+        ensure_ready();
+        Ctx &c = ctx();
+        // inside a stream capture an allocation or a return would become a
+        // graph node: the captured time loops allocate nothing
+        if (c.capturing) fail("<error>: a device buffer grew inside a captured time loop");
+        release(c.st);
+        if (g_pool) {
+            hipError_t e = hipMallocAsync(&p, bytes, c.st);
+            if (e == hipErrorOutOfMemory) {
+                // the pool's idle memory may be split in pieces too small:
+                // give it back to the device and try once more
+                (void)hipGetLastError();
+                HIPCHECK(hipStreamSynchronize(c.st));
+                trim_pool();
+                e = hipMallocAsync(&p, bytes, c.st);
+            }
+            if (e != hipSuccess)
+                fail("<error>: hipMallocAsync(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+            pooled = true;
+        } else {
+            HIPCHECK(hipMalloc(&p, bytes));
+            pooled = false;
+        }
         cap = bytes;
     }
     return p;
 }
 
-void DevBuf::release()
+void DevBuf::release(hipStream_t st)
 {
-    if (p) (void)hipFree(p);
+    if (p) {
+        if (pooled)
+            (void)hipFreeAsync(p, st);
+        else
+            (void)hipFree(p);
+    }
     p = nullptr;
     cap = 0;
+    pooled = false;
 }
 
 // The engine state of the calling thread: the process-wide default context,
@@ -68,6 +134,7 @@ void ensure_ready()
     HIPCHECK(hipGetDevice(&c.device));
     HIPCHECK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
     HIPCHECK(hipStreamCreateWithFlags(&c.cst, hipStreamNonBlocking));
+    pool_setup();
     c.ready = true;
 }
 
@@ -95,7 +162,7 @@ void release_buffers(Ctx &c)
     c.graph_cache.clear();
     c.graph_seen.clear();
     release_fft(c);
-    for (DevBuf *d : all_buffers(c)) d->release();
+    for (DevBuf *d : all_buffers(c)) d->release(c.st);
 }
 
 void sync() { HIPCHECK(hipStreamSynchronize(ctx().st)); }
@@ -332,7 +399,7 @@ static float time_shape(int mode, FdArgs a, const FdTile &t, float *const f[3], 
 // wins, so that shapes within a percent of each other are not told apart by
 // one short sample.
 static FdTile fastest_shape(int mode, const FdArgs &a, const std::vector<FdTile> &cand,
-                            float *const f[3])
+                            float *const f[3], bool once = false)
 {
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
@@ -347,7 +414,7 @@ static FdTile fastest_shape(int mode, const FdArgs &a, const std::vector<FdTile>
     }
     std::sort(ms.begin(), ms.end());
     int top = 1;
-    while (top < 4 && top < (int)ms.size() && ms[top].first <= 1.03f * ms[0].first) ++top;
+    while (!once && top < 4 && top < (int)ms.size() && ms[top].first <= 1.03f * ms[0].first) ++top;
     if (top == 1) {
         HIPCHECK(hipEventDestroy(e0));
         HIPCHECK(hipEventDestroy(e1));
@@ -371,11 +438,14 @@ static FdTile fastest_shape(int mode, const FdArgs &a, const std::vector<FdTile>
 // Shapes tuned by any context of the process, by geometry class, for callers
 // that take a shape tuned for a similar geometry instead of timing their own
 // (serving workers, whose requests rarely repeat a geometry exactly: tuning a
-// new geometry times a few hundred launches per stencil mode).  Any shape gives
-// the same bits, so reuse changes the speed only.  Similar: same stencil mode,
-// accumulation variant, layout, 64-lane groups (lanes) or shot count (arenas,
-// whose shots per block depend on it), free surface and arithmetic, and
-// column count and stencil height within a quarter of the tuned one's.
+// new geometry times a few hundred launches per stencil mode, and beside the
+// other workers' kernels each timed group of launches waits behind them, so
+// that one new geometry cost its request seconds).  Any shape gives the same
+// bits, so reuse changes the speed only.  Similar: same stencil mode,
+// accumulation variant, layout, 64-lane groups (lanes) or shot-count bucket
+// (arenas: 1, 2-3, 4-7, 8 or more, the shots per block a shape may take), free
+// surface and arithmetic, and column count and stencil height within a
+// quarter of the tuned one's.
 struct TunedClass {
     int mode, accv, lanes, group, fs, fma, ncol, nrow;
     FdTile tile;
@@ -383,13 +453,17 @@ struct TunedClass {
 static std::mutex g_similar_mu;
 static std::vector<TunedClass> g_similar;
 
+// The largest power of two up to 8 that is at most n: shots per block that a
+// per-shot-arena shape tuned for n shots may use.
+static int spb_bucket(int n) { return n >= 8 ? 8 : n >= 4 ? 4 : n >= 2 ? 2 : 1; }
+
 static TunedClass tuned_class(int mode, int accv, const FdArgs &a)
 {
     TunedClass t;
     t.mode = mode;
     t.accv = accv;
     t.lanes = a.cell_stride > 1 ? 1 : 0;
-    t.group = t.lanes ? (a.nshot + 63) / 64 : a.nshot;
+    t.group = t.lanes ? (a.nshot + 63) / 64 : spb_bucket(a.nshot);
     t.fs = a.fs;
     t.fma = a.fma;
     t.ncol = a.ixe - a.ixs + 1;
@@ -457,6 +531,51 @@ FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur,
     a.acc_on = 0;
     a.u = nullptr;
     std::vector<FdTile> cand;
+    if (c.quick_tune) {
+        // serving: a short list, timed once (the shapes that won most often:
+        // the register ring with 20 or 30 rows and the plain lanes kernel for
+        // small grids; square tiles with one or the most shots per block)
+        for (int rm : {0, 1}) {
+            if (lanes) {
+                for (int k : {2, 3}) {
+                    FdTile t;
+                    t.tz = 64;
+                    t.kind = 4;
+                    t.wpb = 8;
+                    t.pd = 2;
+                    t.zrun = k * ring_unroll(2);
+                    t.remap = rm;
+                    cand.push_back(t);
+                }
+                FdTile t;
+                t.tz = 64;
+                t.kind = 2;
+                t.wpb = 4;
+                t.zrun = 8;
+                t.pd = 1;
+                t.remap = rm;
+                cand.push_back(t);
+            } else {
+                // (tile rows, shots per block)
+                const int b = spb_bucket(base.nshot);
+                const std::vector<std::pair<int, int>> shapes =
+                    b > 1 ? std::vector<std::pair<int, int>>{{64, b}, {32, b}, {64, 1}}
+                          : std::vector<std::pair<int, int>>{{64, 1}, {32, 1}};
+                for (const auto &sh : shapes) {
+                    FdTile t;
+                    t.tz = sh.first;
+                    t.spb = sh.second;
+                    t.remap = rm;
+                    cand.push_back(t);
+                }
+            }
+        }
+        float *const f[3] = {out, const_cast<float *>(cur), const_cast<float *>(prev)};
+        const FdTile best = fastest_shape(mode, a, cand, f, true);
+        c.tuned[key] = best;
+        remember_similar(cls, best);
+        return best;
+    }
     if (lanes) {
         // the shapes that won somewhere among grids of 128^2 to 3000 x 1000
         // cells and 48 to 256 shots: the LDS kernel with 8 columns per block
@@ -583,7 +702,8 @@ FdTile autotune_acc_tile(int mode, int accv, const FdTile &mode_tile, const FdAr
         for (int pd : {2, 4}) {
             // the register-ring stencil has no accumulation epilogue: its
             // accumulating launches run the LDS stencil of the same block
-            // shape and order
+            // shape and order (serving: two of these, timed once)
+            if (c.quick_tune && !((zrun == 16 && pd == 4) || (zrun == 24 && pd == 2))) continue;
             FdTile t = mode_tile;
             t.kind = 3;
             t.wpb = 8;
@@ -592,7 +712,7 @@ FdTile autotune_acc_tile(int mode, int accv, const FdTile &mode_tile, const FdAr
             cand.push_back(t);
         }
     float *const f[3] = {out, const_cast<float *>(cur), const_cast<float *>(prev)};
-    const FdTile best = fastest_shape(mode, a, cand, f);
+    const FdTile best = fastest_shape(mode, a, cand, f, c.quick_tune != 0);
     c.tuned_acc[key] = best;
     remember_similar(cls, best);
     return best;
@@ -1759,7 +1879,7 @@ static int choose_batch(int n, size_t per_shot, long fsz, bool &lanes, bool aren
             size_t held = c.arena.cap + c.acc.cap + c.rec.cap + c.obs.cap + c.src.cap +
                           c.misc[0].cap + c.misc[1].cap + c.ck16.cap + c.stage.cap +
                           c.accstage.cap;
-            budget = ((double)fr + (double)held) * c.memfrac;
+            budget = ((double)fr + (double)held + (double)pool_idle_bytes()) * c.memfrac;
         }
         Smax = std::max<long>(1, (long)(budget / (double)per_shot));
     }
@@ -2227,7 +2347,7 @@ extern "C" void fwiedu_session_destroy(void *sp)
     if (!S) return;
     sync();
     for (DevBuf *d : {&S->obs_dev, &S->grad_dev, &S->pre0, &S->pre1, &S->slow, &S->pert})
-        d->release();
+        d->release(ctx().st);
     delete S;
 }
 

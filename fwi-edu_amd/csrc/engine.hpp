@@ -28,14 +28,22 @@ namespace fwiedu {
                            __FILE__, __LINE__);                                          \
     } while (0)
 
-// A grow-only device allocation.
+// A grow-only device allocation from the device's stream-ordered memory pool
+// (allocated and returned in the order of the calling thread's engine
+// stream), or from hipMalloc where pools are missing.
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
+    bool pooled = false;
     void *get(size_t bytes);
     template <class T> T *as(size_t n) { return static_cast<T *>(get(n * sizeof(T))); }
-    void release();
+    void release(hipStream_t st);
 };
+// Bytes the memory pool holds but no buffer uses (free for the next
+// allocation, counted as used by hipMemGetInfo); trim_pool returns them to
+// the device.
+size_t pool_idle_bytes();
+void trim_pool();
 
 struct Stats {
     long fd_launches = 0;
@@ -88,6 +96,7 @@ struct Ctx {
     std::map<std::vector<long>, GraphEntry> graph_cache;
     std::set<std::vector<long>> graph_seen;  // loops run once; captured when they come again
     int reuse_similar = 0;        // take launch shapes tuned for similar geometries (serving)
+    int quick_tune = 0;           // time a short list of launch shapes, once (serving)
     std::map<std::tuple<int, int, int, int, int, int, int, int, long>, FdTile> tuned;
     std::map<std::tuple<int, int, int, int, int, int, int, int, long>, FdTile> tuned_acc;
     Stats stats;

@@ -130,8 +130,10 @@ static void worker_main(Server *s, int w)
     bind_ctx(wk.ctx);
     wk.ctx->graphs = s->graphs;
     wk.ctx->ckpt_bits = s->ckpt_bits;
-    // requests rarely repeat a geometry: take shapes tuned for similar ones
+    // requests rarely repeat a geometry: take shapes tuned for similar ones,
+    // and time a short list for a new one
     wk.ctx->reuse_similar = 1;
+    wk.ctx->quick_tune = 1;
     for (;;) {
         ServerJob *sj = nullptr;
         size_t reserve = 0;
@@ -194,7 +196,7 @@ extern "C" void *fwiedu_server_create(int slots, double mem_fraction, int graphs
     size_t fr = 0, tot = 0;
     HIPCHECK(hipMemGetInfo(&fr, &tot));
     Server *s = new Server();
-    s->budget = (size_t)((double)fr * mem_fraction);
+    s->budget = (size_t)((double)(fr + pool_idle_bytes()) * mem_fraction);
     s->graphs = graphs ? 1 : 0;
     s->ckpt_bits = ckpt_bits;
     s->workers.resize(slots);
