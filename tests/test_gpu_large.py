@@ -1,5 +1,6 @@
-"""GPU parity at the large workloads (400 x 1200 multiscale, 1000 x 3000 high
-frequency, the serving stream's largest geometry), within the time of a GPU
+"""GPU parity at the demo-shape and large workloads (221 x 601, 400 x 1200
+multiscale, 1000 x 3000 high frequency, the serving stream's largest
+geometry), within the time of a GPU
 test run: full grids and full nt, with few shots where the CPU oracle has to
 follow.
 
@@ -10,8 +11,10 @@ Tolerances (relative L2):
     stencil sweeps are bit-exact, FP64 sums run in another order);
   * against the FP64 oracle, FP32 arithmetic: records <= 5e-5, FP32-stored
     gradients <= 2e-4, FP16-stored gradients <= 2e-3 -- about four times the
-    figures measured at these sizes (records 7.0e-6 / 8.5e-6, gradients
-    1.44e-5 / 5.31e-5 with FP32 and 5.17e-4 / 5.54e-4 with FP16 storage);
+    figures measured at the two larger sizes (records 7.0e-6 / 8.5e-6,
+    gradients 1.44e-5 / 5.31e-5 with FP32 and 5.17e-4 / 5.54e-4 with FP16
+    storage; at 221 x 601 the precision report gives records 5.8e-6,
+    gradients 1.36e-5 and 3.75e-4 on two shots);
   * a low-pass band of the multiscale sequence (filter 0 at 3 Hz): the
     library's hipFFT adjoint source against the same chain restated with
     numpy's FFT in FP64 on the library's own records, <= 1e-5.
@@ -81,11 +84,12 @@ def gradient(o_or_lib, w, obs, is_lib, bits=32):
     return pre, g.a.copy()
 
 
-@pytest.mark.parametrize("name", ["c2", "c4"])
+@pytest.mark.parametrize("name", ["c1", "c2", "c4"])
 def test_full_size_one_shot_against_fp32_and_fp64(L, oracles, name):
-    """400 x 1200 (nt 5000) and 1000 x 3000 (nt 8000, checkpoint spacing 126,
-    251 slots): one shot, full grid and full nt.  At 400 x 1200 records,
-    adjoint source and gradient against the strict FP32 oracle; at both sizes
+    """221 x 601 (nt 4001), 400 x 1200 (nt 5000) and 1000 x 3000 (nt 8000,
+    checkpoint spacing 126, 251 slots): one shot, full grid and full nt.  At
+    221 x 601 and 400 x 1200 records, adjoint source and gradient against the
+    strict FP32 oracle; at all sizes
     records and the gradients with FP32 and FP16 checkpoint storage against
     the FP64 oracle (observed data from the FP64 oracle's modeling).  The
     strict FP32 oracle is left out at 1000 x 3000 for time: its 1.0e11
@@ -94,7 +98,7 @@ def test_full_size_one_shot_against_fp32_and_fp64(L, oracles, name):
     w = synth.workload(name, nshot=1)
     say(f"case: {name} one shot {w.nz}x{w.nx} nt={w.nt}")
     errs = {}
-    if name == "c2":
+    if name != "c4":
         obs_s = observed(oracles["f32s"], w, False)
         errs["records vs f32s"] = rel(recs(observed(L, w, True)), recs(obs_s))
         pre_g, g_g = gradient(L, w, obs_s, True, 32)
@@ -110,7 +114,7 @@ def test_full_size_one_shot_against_fp32_and_fp64(L, oracles, name):
     errs["gradient fp32 store vs f64"] = rel(g32, g_d)
     errs["gradient fp16 store vs f64"] = rel(g16, g_d)
     say("  rel L2:", {k: f"{v:.3e}" for k, v in errs.items()})
-    if name == "c2":
+    if name != "c4":
         assert errs["records vs f32s"] <= 1e-6
         assert errs["gradient vs f32s"] <= 1e-6
         assert errs["adjoint source vs f32s"] <= 1e-6
