@@ -381,6 +381,8 @@ def main():
                     help="also run a device-to-device copy of this size (counter calibration)")
     ap.add_argument("--batch", type=int, default=0, help="shots per launch (0 = all that fit)")
     ap.add_argument("--plain-shots", type=int, default=4)
+    ap.add_argument("--graph-steps", type=int, default=2,
+                    help="gradients timed with the time loops replayed from HIP graphs (0 = skip)")
     ap.add_argument("--cpu-shots", type=int, default=4)
     ap.add_argument("--cpu-1thread-shots", type=int, default=1,
                     help="gradient shots of the CPU baseline timed on one thread (0 = skip)")
@@ -463,6 +465,25 @@ def main():
                 "call": "iso_acoustic2d_propagation(problem=1) on host arrays, 1 call"}
         log(f"host-boundary gradient: {w.nshot / th:.2f} shots/s")
 
+    # the same gradients with each batch's time loops captured into HIP graphs
+    # and replayed (one hipGraphLaunch per loop; sampled kernel timing skips
+    # replayed launches, so the stencil figures above come from the eager runs)
+    graph = None
+    if args.graph_steps > 0:
+        L.fwiedu_set_graphs(1)
+        try:
+            # two warm-up gradients: the first runs a loop as it is, the
+            # second captures it; the timed ones replay it
+            gtimes, _, _ = timed_gradients(L, w, obs, args.graph_steps, 2, args.batch, 0)
+            launches = flib.stat("graph_launches")
+        finally:
+            L.fwiedu_set_graphs(0)
+        gs = w.nshot / float(np.mean(gtimes))
+        graph = {"gradient_shots_per_s": gs, "ms_per_step": float(np.mean(gtimes)) * 1e3,
+                 "graph_launches": int(launches),
+                 "sample": f"{args.graph_steps} timed gradients after 2 warm-up (eager, capture)"}
+        log(f"graph replay: {gs:.2f} shots/s")
+
     # plain path: one shot per launch
     plain = None
     if args.plain_shots > 0:
@@ -515,6 +536,7 @@ def main():
         "plain": plain,
         "batched_over_plain": (shots_per_s / plain["gradient_shots_per_s"]) if plain else None,
         "host_boundary": host,
+        "graph_replay": graph,
         "serving": serve,
         "roofline": {"bound": "hbm", "achieved": modes[dom]["achieved"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": modes[dom]["frac"],
