@@ -1638,8 +1638,16 @@ static void adjoint_loop(BatchState &b)
     // (get_acoustic2dWavefield copies u(t) into it, propagation.c:709)
     const float *fwd_seen = b.F(BatchPlan::ZERO);
     int fwd_t = -1;
+    // The imaging condition reads the forward field at it % MST == 0 only, so
+    // a segment is recomputed when the reverse loop first meets such a step
+    // that is not stored, up to that step: the reference recomputes up to the
+    // segment's last step (kd + d - 2), whose fields past its last imaging
+    // step nothing reads (the fields computed are the same).  Verbose
+    // wavefield files follow the reference's steps, whose recomputation ends
+    // are what its p1_fwd shows.
+    const bool trim = !(b.vb && b.vb->snaps());
     for (int it = nt - 1; it >= 0; --it) {
-        if (!stored[it]) {
+        if (!stored[it] && (!trim || it % MST == 0)) {
             // get_acoustic2dCheckPointStatus: the segment restarts at the last
             // multiple of d below it, from the stored pair (kd-1, kd)
             const int s0 = (it / sc.d) * sc.d;
