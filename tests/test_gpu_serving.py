@@ -76,3 +76,35 @@ def test_jobs_same_alone_and_in_the_mix(L, bits):
                 assert abs(j.misfit - mis) <= 1e-6 * abs(mis), (j.w.name, j.misfit, mis)
     finally:
         L.fwiedu_set_checkpoint_bits(32)
+
+
+def test_jobs_same_with_scarce_memory(L):
+    """A server whose device budget (90 MB) holds one job at a time and only
+    some shots of the 5-shot gradient: jobs wait for memory (an idle worker
+    gives its buffers back to a waiting job) and that gradient runs in
+    batches of fewer shots; every result is still the job's own alone."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+    frac = 90e6 / float(free.value)
+    say(f"case: serving mix of {len(MIX)} jobs, 3 slots, 90 MB budget (fraction {frac:.2e})")
+    jobs = []
+    for i, (p, nz, nx, nt, ns) in enumerate(MIX):
+        w = serving.request_workload(i, 5151 + i, nz, nx, nt, ns)
+        jobs.append(serving.Job(w, p, serving.observed_for(L, w) if p == 1 else None))
+    srv = serving.Server(slots=3, mem_fraction=frac, graphs=True)
+    try:
+        for j in jobs:
+            srv.submit(j)
+        for j in jobs:
+            srv.wait(j)
+        say(f"  peak concurrency {srv.stat('peak_concurrency'):.0f}, "
+            f"mean queueing {np.mean([j.queue for j in jobs]) * 1e3:.1f} ms")
+    finally:
+        srv.close()
+    for j in jobs:
+        rec, g, mis = alone(L, j)
+        assert np.array_equal(j.traces(), rec), j.w.name
+        if j.problem == 1:
+            assert np.array_equal(j.grad.a, g), j.w.name
