@@ -698,11 +698,22 @@ FdTile autotune_acc_tile(int mode, int accv, const FdTile &mode_tile, const FdAr
     a.acc_slot = 0;
     a.acc_n = 0;
     std::vector<FdTile> cand;
+    if (accv == 1 && mode_tile.kind == 4)
+        // the register-ring stencil stages the term itself (runs of two or
+        // three unrolled groups, 2 or 3 rows of loads in flight)
+        for (int pd : {2, 3})
+            for (int k : {2, 3}) {
+                if (c.quick_tune && (pd != 2 || k != 2)) continue;
+                FdTile t = mode_tile;
+                t.pd = pd;
+                t.zrun = k * ring_unroll(pd);
+                cand.push_back(t);
+            }
     for (int zrun : {16, 20, 24, 32})
         for (int pd : {2, 4}) {
-            // the register-ring stencil has no accumulation epilogue: its
-            // accumulating launches run the LDS stencil of the same block
-            // shape and order (serving: two of these, timed once)
+            // the LDS stencil of the ring's block shape and order: the
+            // launches that add the staged terms to the accumulator, and
+            // staging candidates (serving: two of these, timed once)
             if (c.quick_tune && !((zrun == 16 && pd == 4) || (zrun == 24 && pd == 2))) continue;
             FdTile t = mode_tile;
             t.kind = 3;

@@ -1103,7 +1103,13 @@ static void launch_lanes_lds_t(const FdArgs &a0, FdTile tl, hipStream_t st)
 //    any load) take a loop with none of those tests; the others load their
 //    source cells and receiver rows and run the epilogue of lanes_update, as
 //    fd_lanes_lds_kernel does.  Each branch issues its own first loads, so
-//    that no value in flight is moved between registers where they join.
+//    that no value in flight is moved between registers where they join;
+//  * STG (forward and adjoint launches that stage their accumulation term,
+//    every 5th step three times in four): the cell's pseudo-Hessian or
+//    imaging term goes to the launch's staging slot (non-temporal), the
+//    adjoint's forward field coming through one more ring of the wave's
+//    column, loaded with the rows; the launches that add the staged terms to
+//    the FP64 accumulator run fd_lanes_lds_kernel.
 // Rows past a run's end (the last group of a run) still load, publish and
 // pass the barrier, computing nothing.
 template <int D>
@@ -1118,12 +1124,13 @@ __device__ __forceinline__ frsrc col_rsrc(const float *base, long off_floats, lo
                                              (int)(unsigned)bytes_left, 0x00020000);
 }
 
-template <int MODE, int D>
+template <int MODE, int D, bool STG>
 __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
 {
     constexpr int WPB = 8;
     constexpr int NC = WPB + 8;  // 6 halo columns and 2 spare (see below)
     constexpr int U = RingShape<D>::U;
+    constexpr bool UF = STG && MODE == FD_ADJ;  // the forward field rides with the rows
     __shared__ float row[2][NC][64];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1187,8 +1194,15 @@ __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
     float injv = 0.f;
 
     // rings: V holds p1 row iz0 - 3 + j in slot j mod U, P and H (p0 and the
-    // halo column) row iz0 + j in slot j mod U
-    float V[U], P[U], H[U];
+    // halo column) row iz0 + j in slot j mod U, and with UF the forward field
+    // of the imaging condition (Fu) too
+    float V[U], P[U], H[U], Fu[UF ? U : 1];
+    const frsrc Uc = col_rsrc(UF ? a.u : a.prev, colb * CS, UF ? fbytes - cb : 0);
+    // staging: the column's place in the model ([nx][nz] cells x lanes), or
+    // none outside it
+    const bool mcol = STG && active && ix >= a.pml && ix < a.pml + a.nx;
+    const long abase = STG ? (long)(ix - a.pml) * a.nz * a.acc_cell + s : 0;
+    float *const stg = STG ? a.accstage + (long)a.acc_slot * a.accstage_stride : nullptr;
     const cfloat8v *c8 = (const cfloat8v *)(a.coef + CO_N * fsz);
     cfloat *cf = (cfloat *)a.coef;
     const frsrc P2 = field_rsrc(a.out, fbytes);
@@ -1202,6 +1216,7 @@ __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
             V[(6 + d) % U] = ldc(P1z, (iz0 + d) * rs4);
             P[d % U] = ldc_stream(P0c, (iz0 + d) * rs4);
             H[d % U] = ldc(Hc, (iz0 + d) * rs4);
+            if (UF) Fu[d % U] = ldc_stream(Uc, (iz0 + d) * rs4);
         }
         kc[0] = c8[colb + iz0];
     };
@@ -1224,6 +1239,7 @@ __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
             V[(u + 6 + D) % U] = ldc(P1z, rl);
             P[(u + D) % U] = ldc_stream(P0c, rl);
             H[(u + D) % U] = ldc(Hc, rl);
+            if (UF) Fu[(u + D) % U] = ldc_stream(Uc, rl);
             __syncthreads();
             const int iz = iz0 + rel;
             if (SPECIAL && MODE == FD_FWD && iz == rr) a.stage[(long)a.it * a.nslot + slot] = V[(u + 3) % U];
@@ -1242,10 +1258,10 @@ __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
                 const float q6 = V[(u + 6) % U];
                 if (SPECIAL) {
                     const float k[7] = {kv[0], kv[1], kv[2], kv[3], kv[4], kv[5], kv[6]};
-                    lanes_update<MODE, false, 1, true>(
+                    lanes_update<MODE, false, STG ? 3 : 1, true>(
                         a, cf, fsz, colb, ix, iz, s, scell, P2, P2, CS, q0, q1, q2, q3, q4, q5,
-                        q6, xm1, xp1, xm2, xp2, xm3, xp3, P[u % U], k, 0.0, 0.f,
-                        MODE == FD_ADJ && iz == rr, injv, nullptr);
+                        q6, xm1, xp1, xm2, xp2, xm3, xp3, P[u % U], k, 0.0,
+                        UF ? Fu[u % U] : 0.f, MODE == FD_ADJ && iz == rr, injv, nullptr);
                 } else {
                     const float v = fd_update<false>(q3, P[u % U], q2, q4, q1, q5, q0, q6, xm1,
                                                      xp1, xm2, xp2, xm3, xp3, kv[0], kv[1], kv[2],
@@ -1253,6 +1269,14 @@ __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
                                                      a.invdxdx, a.invdzdz);
                     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), P2c, vo, rl,
                                                           CP_STREAM);
+                    const int pz = iz - a.pml;
+                    if (STG && mcol && pz >= 0 && pz < a.nz) {
+                        // propagation.c:590 (forward) and :711 (adjoint), as
+                        // lanes_update forms them
+                        const float d2 = (v - 2.0f * q3 + P[u % U]) * a.invdtdt;
+                        const float term = MODE == FD_FWD ? d2 * d2 : d2 * Fu[UF ? u % U : 0];
+                        __builtin_nontemporal_store(term, stg + abase + (long)pz * a.acc_cell);
+                    }
                 }
             }
             rl += rs4;
@@ -1306,7 +1330,7 @@ __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
     }
 }
 
-template <int MODE>
+template <int MODE, bool STG>
 static void launch_ring_t(const FdArgs &a0, FdTile tl, hipStream_t st)
 {
     FdArgs a = a0;
@@ -1321,13 +1345,13 @@ static void launch_ring_t(const FdArgs &a0, FdTile tl, hipStream_t st)
     a.lz_order = tl.zfast;
     const dim3 grid((unsigned)(a.remap ? (nb + 7) / 8 * 8 : nb)), block(512);
     if (tl.pd >= 5)
-        fd_go(fd_ring_kernel<MODE, 5>, grid, block, 0, st, a);
+        fd_go(fd_ring_kernel<MODE, 5, STG>, grid, block, 0, st, a);
     else if (tl.pd == 4)
-        fd_go(fd_ring_kernel<MODE, 4>, grid, block, 0, st, a);
+        fd_go(fd_ring_kernel<MODE, 4, STG>, grid, block, 0, st, a);
     else if (tl.pd == 3)
-        fd_go(fd_ring_kernel<MODE, 3>, grid, block, 0, st, a);
+        fd_go(fd_ring_kernel<MODE, 3, STG>, grid, block, 0, st, a);
     else
-        fd_go(fd_ring_kernel<MODE, 2>, grid, block, 0, st, a);
+        fd_go(fd_ring_kernel<MODE, 2, STG>, grid, block, 0, st, a);
 }
 
 int ring_unroll(int pd)
@@ -1340,20 +1364,22 @@ void launch_fd(int mode, const FdArgs &a0, FdTile tl, hipStream_t st)
 {
     FdArgs a = a0;
     a.remap = tl.remap;
-    if (tl.kind == 4 && mode != FD_RECON && a.acc_on) {
-        // no accumulation epilogue in the ring stencil: the LDS stencil of
-        // the same block shape runs these launches
+    const bool stage = mode != FD_RECON && a.acc_on && a.acc_stage && a.accstage;
+    if (tl.kind == 4 && mode != FD_RECON && a.acc_on && !stage) {
+        // the ring stencil stages accumulation terms but does not add them
+        // to the accumulator: the LDS stencil of the same block shape runs
+        // these launches
         tl.kind = 3;
         tl.wpb = 8;
         tl.pd = min(tl.pd, 4);
     }
     if (tl.kind == 4) {
         if (mode == FD_FWD)
-            launch_ring_t<FD_FWD>(a, tl, st);
+            stage ? launch_ring_t<FD_FWD, true>(a, tl, st) : launch_ring_t<FD_FWD, false>(a, tl, st);
         else if (mode == FD_RECON)
-            launch_ring_t<FD_RECON>(a, tl, st);
+            launch_ring_t<FD_RECON, false>(a, tl, st);
         else
-            launch_ring_t<FD_ADJ>(a, tl, st);
+            stage ? launch_ring_t<FD_ADJ, true>(a, tl, st) : launch_ring_t<FD_ADJ, false>(a, tl, st);
         return;
     }
     if (tl.kind == 3) {
