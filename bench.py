@@ -14,6 +14,7 @@ import argparse
 import ctypes
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -218,6 +219,27 @@ def copy_calibration(mb):
     log(f"copy calibration: 3 x {mb} MiB device-to-device")
 
 
+def stream_leg():
+    """The GPU's HBM streaming rate for the stencil's traffic pattern (two
+    float32 streams read, one written, each element once), measured by
+    fwi-edu_amd/bin/hbm_stream in a child process at array sizes inside and
+    beyond the 256 MB Infinity Cache: the rate a plain streaming kernel reaches
+    here, beside which the stencil's fraction of the 8 TB/s peak reads."""
+    exe = os.path.join(ROOT, "fwi-edu_amd", "bin", "hbm_stream")
+    if not os.path.exists(exe):
+        log("hbm_stream probe not built (make -C fwi-edu_amd): stream rates omitted")
+        return None
+    r = subprocess.run([exe, "3"], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        log(f"hbm_stream probe failed ({r.returncode}): {r.stderr.strip()[-300:]}")
+        return None
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    beyond = [z for z in res["sizes"] if z["array_mib"] >= 700]
+    res["add_beyond_cache_tb_s"] = max(z["add_tb_s"] for z in beyond)
+    log(f"hbm stream: add {res['add_beyond_cache_tb_s']:.2f} TB/s beyond the Infinity Cache")
+    return res
+
+
 def serving_leg(args):
     """Continuous-batching serving loop: a seeded Poisson stream of modeling /
     gradient requests (70/30) on random-layer models, grids uniform in 128-1024
@@ -310,7 +332,6 @@ def native_oracle():
     """The FP32 oracle built on this host with the reference's flags
     (-O3 -march=native -ffast-math -fopenmp, Makefile.gcc:28); None (and the
     prebuilt -march=x86-64-v3 build is used) if the build fails."""
-    import subprocess
     import tempfile
     d = tempfile.mkdtemp(prefix="fwiedu_oracle_")
     try:
@@ -390,6 +411,8 @@ def main():
     ap.add_argument("--cpu-1thread-shots", type=int, default=1,
                     help="gradient shots of the CPU baseline timed on one thread (0 = skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-stream", action="store_true",
+                    help="skip the HBM streaming-rate probe")
     ap.add_argument("--no-host", action="store_true",
                     help="skip the host-array (PCIe-inclusive) iso_acoustic2d_propagation timing")
     ap.add_argument("--ktime-every", type=int, default=41,
@@ -507,6 +530,7 @@ def main():
 
     if args.copy_mb:
         copy_calibration(args.copy_mb)
+    stream = None if args.no_stream else stream_leg()
     cpu = None if args.no_cpu else cpu_baseline(w, args.cpu_shots, args.cpu_1thread_shots)
     if cpu:
         log(f"cpu baseline: {cpu['value']:.3f} shots/s on {cpu['cores']} threads")
@@ -553,9 +577,15 @@ def main():
                                f"({dom}: the stencil mode with the largest share of kernel time)",
                      "bytes_per_launch": modes[dom]["bytes_per_launch"],
                      "avg_launch_us": modes[dom]["avg_launch_us"],
-                     "launches_timed": modes[dom]["launches_timed"]},
+                     "launches_timed": modes[dom]["launches_timed"],
+                     # the same achieved rate over what a plain streaming
+                     # kernel of the stencil's pattern reaches beyond the
+                     # Infinity Cache on this GPU (hbm_stream below)
+                     "frac_of_stream": (modes[dom]["achieved"] /
+                                        (stream["add_beyond_cache_tb_s"] * 1e3)) if stream else None},
         "stencil_modes": modes,
         "beyond_infinity_cache": large,
+        "hbm_stream": stream,
         "cpu_baseline": cpu,
         "device_bytes": st["device_bytes"],
     }
