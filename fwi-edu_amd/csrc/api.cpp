@@ -443,6 +443,13 @@ extern "C" void fwiedu_set_lanes(int wpb, int zrun, int pd, int lds)
 }
 extern "C" void fwiedu_set_fma(int on) { ctx().fma = on ? 1 : 0; }
 
+extern "C" void fwiedu_set_inplace(int on)
+{
+    Ctx &c = ctx();
+    c.inplace = on ? 1 : 0;
+    if (c.ready) drop_graphs();  // captured loops hold the other buffer sequence
+}
+
 extern "C" void fwiedu_set_graphs(int on)
 {
     Ctx &c = ctx();
@@ -531,6 +538,7 @@ extern "C" double fwiedu_stat(const char *name)
     if (n == "ckpt_bits") return (double)s.ckpt_bits;
     if (n == "graphs_built") return (double)s.graphs_built;
     if (n == "graph_launches") return (double)s.graph_launches;
+    if (n == "fd_inplace") return (double)s.fd_inplace;
     if (n == "shapes_reused") return (double)s.shapes_reused;
     if (n == "h2d_bytes") return s.h2d_bytes;
     if (n == "d2h_bytes") return s.d2h_bytes;

@@ -325,6 +325,7 @@ void fd_launch(int mode, const FdArgs &a, long cells, FdTile tile)
         c.ev_used += 2;
     }
     c.stats.fd_launches++;
+    if (a.out == a.prev) c.stats.fd_inplace++;
     c.stats.fd_cells += cells;
     c.stats.shots_per_launch = a.nshot;
 }
@@ -1086,6 +1087,11 @@ static FdArgs base_args(const Geom &g, const float *coef, int S, long shot_strid
     return a;
 }
 
+static inline bool is_work(float *const w[3], const float *x)
+{
+    return x == w[0] || x == w[1] || x == w[2];
+}
+
 static inline float *pick_free(float *const w[3], const float *x, const float *y)
 {
     for (int i = 0; i < 3; ++i)
@@ -1515,6 +1521,24 @@ static void set_deferral(FdArgs &a, bool defer, int &staged, bool last)
 // propagation.c:539-608.  With save_ckpt the fields u(kd-1), u(kd) land in
 // their checkpoint slots; with precond the pseudo-Hessian accumulates in acc.
 // Kernel launches only, so a HIP graph can capture it.
+// Time steps of the shots-in-lanes stencils may write the new field over p0:
+// every lanes stencil reads p0 only at the cell it updates, before it stores
+// that cell.  Only a work field is overwritten (never the zero field, a
+// checkpoint or a stored imaging field), and not when wavefield snapshots
+// are written.  It pays where the two work fields fit the 256 MB Infinity
+// Cache, which then holds them from one step to the next (the in-place
+// launches load p0 and store the output at the default cache policy); larger
+// batches keep three rotating fields streamed past the caches (221 x 601 with
+// 120 shots: 2 x 111.6 MB in place, 2 % off a gradient; 400 x 1200 with 256
+// shots, 2 x 671 MB, in place and at the default policy: 148 against 156-162
+// gradient shots/s).
+static constexpr size_t INFINITY_CACHE_BYTES = (size_t)256 << 20;
+static bool in_place(const BatchState &b)
+{
+    return ctx().inplace && b.lanes && !(b.vb && b.vb->snaps()) &&
+           (size_t)2 * b.KS * sizeof(float) <= INFINITY_CACHE_BYTES;
+}
+
 static void forward_loop(BatchState &b, bool save_ckpt, bool precond)
 {
     Ctx &c = ctx();
@@ -1550,11 +1574,14 @@ static void forward_loop(BatchState &b, bool save_ckpt, bool precond)
     float *W[3] = {b.F(BatchPlan::A0), b.F(BatchPlan::A1), b.F(BatchPlan::A2)};
     const float *zero = b.F(BatchPlan::ZERO);
     const float *cur = zero, *prev = zero;
+    const bool inpl = in_place(b);
     for (int it = 0; it < b.nt; ++it) {
         float *out;
         const bool ck = save_ckpt && it + 1 < b.nt && sc.ckslot[it + 1] >= 0;
         if (ck && !b.ck16)
             out = b.F(BatchPlan::CK0 + sc.ckslot[it + 1]);
+        else if (inpl && is_work(W, prev))
+            out = const_cast<float *>(prev);
         else
             out = pick_free(W, cur, prev);
         launch_record(rt, cur, b.SS, b.CS, b.rec, it, c.st);
@@ -1657,6 +1684,7 @@ static void adjoint_loop(BatchState &b)
     // wavefield files follow the reference's steps, whose recomputation ends
     // are what its p1_fwd shows.
     const bool trim = !(b.vb && b.vb->snaps());
+    const bool inpl = in_place(b);
     for (int it = nt - 1; it >= 0; --it) {
         if (!stored[it] && (!trim || it % MST == 0)) {
             // get_acoustic2dCheckPointStatus: the segment restarts at the last
@@ -1672,7 +1700,9 @@ static void adjoint_loop(BatchState &b)
             }
             for (int itt = s0; itt < it; ++itt) {
                 const int t = itt + 1;
-                float *fout = (t % MST == 0) ? b.F(ring0 + (t - s0) / MST) : pick_free(FW, fcur, fprev);
+                float *fout = (t % MST == 0)               ? b.F(ring0 + (t - s0) / MST)
+                              : (inpl && is_work(FW, fprev)) ? const_cast<float *>(fprev)
+                                                             : pick_free(FW, fcur, fprev);
                 fr.out = fout;
                 fr.cur = fcur;
                 fr.prev = fprev;
@@ -1694,7 +1724,7 @@ static void adjoint_loop(BatchState &b)
             launch_inject(left ? b.tb.inj_left : b.tb.inj, const_cast<float *>(cur), b.SS, b.CS,
                           K, b.rec, it, g.dt, g.invdx, g.invdz, c.st);
         ad.inj_on = (fused && it >= 1) ? 1 : 0;
-        float *out = pick_free(AW, cur, prev);
+        float *out = (inpl && is_work(AW, prev)) ? const_cast<float *>(prev) : pick_free(AW, cur, prev);
         ad.out = out;
         ad.cur = cur;
         ad.prev = prev;
@@ -1748,7 +1778,7 @@ static void key_push_geom(std::vector<long> &k, const Geom &g)
 
 static std::vector<long> batch_key(int pass, const BatchState &b, bool f1, bool f2)
 {
-    std::vector<long> k = {pass, (long)f1, (long)f2, (long)ctx().fma, (long)b.nt, (long)b.S, (long)b.Sp,
+    std::vector<long> k = {pass, (long)f1, (long)f2, (long)ctx().fma, (long)in_place(b), (long)b.nt, (long)b.S, (long)b.Sp,
                            (long)b.lanes, (long)b.ck16, b.sc ? (long)b.sc->d : 0L,
                            b.tb.rcv.n, b.tb.inj.n, (long)b.tb.nci, b.nslot, b.tb.rcv_left.n,
                            b.tb.inj_left.n};

@@ -47,6 +47,7 @@ void trim_pool();
 
 struct Stats {
     long fd_launches = 0;
+    long fd_inplace = 0;      // of which wrote the new field over p0
     long fd_cells = 0;        // interior cell-updates issued (all shots)
     long fd_timed = 0;            // sampled reconstruction launches (no accumulation epilogue)
     double fd_ms_timed = 0;
@@ -91,6 +92,7 @@ struct Ctx {
     int layout_req = -1;          // -1 automatic, 0 per-shot arenas, 1 shots in lanes
     int graphs = 0;               // replay time loops from captured HIP graphs
     int fma = 0;                  // stencil arithmetic: 0 strict source order, 1 FMA-contracted
+    int inplace = 1;              // lanes time steps write the new field over p0 when it is a work field
     size_t mem_cap = 0;           // serving worker: device bytes its batches may use (0: free memory)
     bool capturing = false;
     std::map<std::vector<long>, GraphEntry> graph_cache;

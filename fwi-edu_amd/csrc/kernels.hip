@@ -1124,7 +1124,7 @@ __device__ __forceinline__ frsrc col_rsrc(const float *base, long off_floats, lo
                                              (int)(unsigned)bytes_left, 0x00020000);
 }
 
-template <int MODE, int D, bool STG>
+template <int MODE, int D, bool STG, int PCP>
 __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
 {
     constexpr int WPB = 8;
@@ -1183,6 +1183,14 @@ __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
         return __builtin_bit_cast(float,
                                   __builtin_amdgcn_raw_buffer_load_b32(f, vo, boff, CP_STREAM));
     };
+    // p0 and the output: PCP is the non-temporal policy when they are
+    // separate fields (each touched once by the launch, so the caches may let
+    // them go first) and the default policy when the launch updates p0 in
+    // place (two fields in all, which the Infinity Cache then keeps between
+    // launches)
+    auto ldc_p0 = [&](frsrc f, int boff) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(f, vo, boff, PCP));
+    };
 
     // does the block meet a free-surface row, a source row or a receiver row?
     const bool recv = (MODE == FD_FWD && a.rec_on) || (MODE == FD_ADJ && a.inj_on);
@@ -1214,7 +1222,7 @@ __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             V[(6 + d) % U] = ldc(P1z, (iz0 + d) * rs4);
-            P[d % U] = ldc_stream(P0c, (iz0 + d) * rs4);
+            P[d % U] = ldc_p0(P0c, (iz0 + d) * rs4);
             H[d % U] = ldc(Hc, (iz0 + d) * rs4);
             if (UF) Fu[d % U] = ldc_stream(Uc, (iz0 + d) * rs4);
         }
@@ -1237,7 +1245,7 @@ __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
             row[u & 1][3 + w][lane] = V[(u + 3) % U];
             row[u & 1][hj][lane] = H[u % U];
             V[(u + 6 + D) % U] = ldc(P1z, rl);
-            P[(u + D) % U] = ldc_stream(P0c, rl);
+            P[(u + D) % U] = ldc_p0(P0c, rl);
             H[(u + D) % U] = ldc(Hc, rl);
             if (UF) Fu[(u + D) % U] = ldc_stream(Uc, rl);
             __syncthreads();
@@ -1268,7 +1276,7 @@ __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
                                                      kv[3], kv[4], kv[5], a.invdx, a.invdz,
                                                      a.invdxdx, a.invdzdz);
                     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), P2c, vo, rl,
-                                                          CP_STREAM);
+                                                          PCP);
                     const int pz = iz - a.pml;
                     if (STG && mcol && pz >= 0 && pz < a.nz) {
                         // propagation.c:590 (forward) and :711 (adjoint), as
@@ -1330,7 +1338,7 @@ __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
     }
 }
 
-template <int MODE, bool STG>
+template <int MODE, bool STG, int PCP>
 static void launch_ring_t(const FdArgs &a0, FdTile tl, hipStream_t st)
 {
     FdArgs a = a0;
@@ -1345,13 +1353,13 @@ static void launch_ring_t(const FdArgs &a0, FdTile tl, hipStream_t st)
     a.lz_order = tl.zfast;
     const dim3 grid((unsigned)(a.remap ? (nb + 7) / 8 * 8 : nb)), block(512);
     if (tl.pd >= 5)
-        fd_go(fd_ring_kernel<MODE, 5, STG>, grid, block, 0, st, a);
+        fd_go(fd_ring_kernel<MODE, 5, STG, PCP>, grid, block, 0, st, a);
     else if (tl.pd == 4)
-        fd_go(fd_ring_kernel<MODE, 4, STG>, grid, block, 0, st, a);
+        fd_go(fd_ring_kernel<MODE, 4, STG, PCP>, grid, block, 0, st, a);
     else if (tl.pd == 3)
-        fd_go(fd_ring_kernel<MODE, 3, STG>, grid, block, 0, st, a);
+        fd_go(fd_ring_kernel<MODE, 3, STG, PCP>, grid, block, 0, st, a);
     else
-        fd_go(fd_ring_kernel<MODE, 2, STG>, grid, block, 0, st, a);
+        fd_go(fd_ring_kernel<MODE, 2, STG, PCP>, grid, block, 0, st, a);
 }
 
 int ring_unroll(int pd)
@@ -1374,12 +1382,27 @@ void launch_fd(int mode, const FdArgs &a0, FdTile tl, hipStream_t st)
         tl.pd = min(tl.pd, 4);
     }
     if (tl.kind == 4) {
-        if (mode == FD_FWD)
-            stage ? launch_ring_t<FD_FWD, true>(a, tl, st) : launch_ring_t<FD_FWD, false>(a, tl, st);
-        else if (mode == FD_RECON)
-            launch_ring_t<FD_RECON, false>(a, tl, st);
-        else
-            stage ? launch_ring_t<FD_ADJ, true>(a, tl, st) : launch_ring_t<FD_ADJ, false>(a, tl, st);
+        // a launch that writes over p0 (out == prev) keeps both fields at
+        // the default cache policy
+        if (a.out == a.prev) {
+            if (mode == FD_FWD)
+                stage ? launch_ring_t<FD_FWD, true, 0>(a, tl, st)
+                      : launch_ring_t<FD_FWD, false, 0>(a, tl, st);
+            else if (mode == FD_RECON)
+                launch_ring_t<FD_RECON, false, 0>(a, tl, st);
+            else
+                stage ? launch_ring_t<FD_ADJ, true, 0>(a, tl, st)
+                      : launch_ring_t<FD_ADJ, false, 0>(a, tl, st);
+        } else {
+            if (mode == FD_FWD)
+                stage ? launch_ring_t<FD_FWD, true, CP_STREAM>(a, tl, st)
+                      : launch_ring_t<FD_FWD, false, CP_STREAM>(a, tl, st);
+            else if (mode == FD_RECON)
+                launch_ring_t<FD_RECON, false, CP_STREAM>(a, tl, st);
+            else
+                stage ? launch_ring_t<FD_ADJ, true, CP_STREAM>(a, tl, st)
+                      : launch_ring_t<FD_ADJ, false, CP_STREAM>(a, tl, st);
+        }
         return;
     }
     if (tl.kind == 3) {

@@ -100,6 +100,7 @@ def load():
     _sig(L, "fwiedu_set_lanes", None, [c_int, c_int, c_int, c_int])
     _sig(L, "fwiedu_set_layout", None, [c_int])
     _sig(L, "fwiedu_set_graphs", None, [c_int])
+    _sig(L, "fwiedu_set_inplace", None, [c_int])
     _sig(L, "fwiedu_set_fma", None, [c_int])
     _sig(L, "fwiedu_set_memory_fraction", None, [c_double])
     _sig(L, "fwiedu_set_kernel_timing", None, [c_int])
@@ -141,7 +142,7 @@ EXPORTED = [
     "apply_gradient_precondition", "apply_gradient_shot_precondition",
     "calculate_objective_value", "compute_model_perturbation", "calculate_step_length",
     "fwiedu_session_create", "fwiedu_session_set_model", "fwiedu_session_gradient",
-    "fwiedu_session_destroy", "fwiedu_session_iterate", "fwiedu_set_batch", "fwiedu_set_tile", "fwiedu_set_march", "fwiedu_set_tile_order", "fwiedu_set_xcd_remap", "fwiedu_set_lanes", "fwiedu_set_layout", "fwiedu_set_graphs", "fwiedu_set_fma", "fwiedu_set_memory_fraction", "fwiedu_set_kernel_timing",
+    "fwiedu_session_destroy", "fwiedu_session_iterate", "fwiedu_set_batch", "fwiedu_set_tile", "fwiedu_set_march", "fwiedu_set_tile_order", "fwiedu_set_xcd_remap", "fwiedu_set_lanes", "fwiedu_set_layout", "fwiedu_set_graphs", "fwiedu_set_inplace", "fwiedu_set_fma", "fwiedu_set_memory_fraction", "fwiedu_set_kernel_timing",
     "fwiedu_set_checkpoint_bits", "fwiedu_stat", "fwiedu_reset_stats", "fwiedu_release",
     "fwiedu_lanes_max_shots", "fwiedu_model_perturbation_noalloc",
     "fwiedu_device_ok", "fwiedu_version",

@@ -191,6 +191,13 @@ void fwiedu_set_lanes(int wpb, int zrun, int pd, int lds);
  * compile of the reference (it is built with -ffast-math) rounds; fewer
  * instructions, results within FP32 rounding of the strict ones. */
 void fwiedu_set_fma(int on);
+/* 1 (default): with shots in lanes, when two work fields of the batch fit the
+ * GPU's 256 MB Infinity Cache (the 221 x 601 demo size with 120 shots), a time
+ * step whose previous field is a work buffer writes the new field over it
+ * (two work fields instead of three, which the cache then holds between
+ * steps); the arithmetic and the results are the same.  0: every step writes
+ * a third field. */
+void fwiedu_set_inplace(int on);
 /* 1: run each batch's forward and adjoint time loops as HIP graphs: a loop
  * runs kernel by kernel the first time, is captured when the same batch (same
  * device buffers, sizes, schedule and launch shape) comes a second time, and
@@ -216,7 +223,8 @@ void fwiedu_set_kernel_timing(int every);
  * restarted segments.  The per-shot engine with a host Acoustic2dCheckPoint
  * (the reference's FP32 structure) always stores FP32. */
 void fwiedu_set_checkpoint_bits(int bits);
-/* Named counters of the last calls: "fd_launches", "fd_timed", "fd_ms_avg",
+/* Named counters of the last calls: "fd_launches", "fd_inplace" (of them the
+ * launches that wrote the new field over p0), "fd_timed", "fd_ms_avg",
  * "fd_cells_per_launch" (sampled reconstruction launches), "fd_shots_per_launch",
  * "batch", "device_bytes" ...; per stencil mode m = 0 forward, 1 reconstruction,
  * 2 adjoint: "fd<m>_timed", "fd<m>_ms_avg", "fd<m>_cells_per_launch",

@@ -456,6 +456,57 @@ def test_lanes_layout_matches_arenas(L, bits):
         assert np.array_equal(recs(pre_l), recs(pre_a)), name
 
 
+@pytest.mark.parametrize("bits", [32, 16])
+def test_in_place_steps_match_three_fields(L, oracles, bits):
+    """With shots in lanes the time steps write the new field over p0 when it
+    is a work field (fwiedu_set_inplace(1), the default): records, gradient
+    and misfit are bit-identical to three rotating fields and to the strict
+    oracle, through checkpoints, the reconstruction's imaging slots, staged
+    accumulation, fused receivers, the free surface, FP32 or FP16 checkpoint
+    storage, the autotuned and a forced register-ring shape, and graph
+    replays."""
+    w = synth.workload("tiny", nshot=70, freesurface=1, shot_depth_cells=1, nx=90)
+    say(f"case: in place vs three fields, tiny fs=1 70 shots, fp{bits} checkpoints")
+    obs = observed(oracles["f32s"], w, False)
+    pre_o, g_o = gradient(oracles["f32s"], w, obs, False)
+    out = {}
+    L.fwiedu_set_checkpoint_bits(bits)
+    L.fwiedu_set_layout(1)
+    try:
+        for shape in ("tuned", "ring"):
+            if shape == "ring":
+                L.fwiedu_set_xcd_remap(1)
+                L.fwiedu_set_lanes(8, 20, 2, 2)
+            for inplace in (0, 1):
+                L.fwiedu_set_inplace(inplace)
+                L.fwiedu_reset_stats()
+                out[(shape, inplace)] = gradient(L, w, obs, True)
+                assert flib.stat("layout") == 1
+                n, k = flib.stat("fd_launches"), flib.stat("fd_inplace")
+                say(f"  {shape} inplace {inplace}: {k:.0f} of {n:.0f} launches in place")
+                # most forward, reconstruction and adjoint steps, or none
+                assert (k > n / 2) if inplace else (k == 0)
+            # graph replays of the in-place loops (captured on the second run)
+            L.fwiedu_set_graphs(1)
+            for rep in range(3):
+                out[(shape, "graph", rep)] = gradient(L, w, obs, True)
+            L.fwiedu_set_graphs(0)
+    finally:
+        L.fwiedu_set_graphs(0)
+        L.fwiedu_set_inplace(1)
+        L.fwiedu_set_layout(-1)
+        L.fwiedu_set_tile(0, 0)
+        L.fwiedu_set_xcd_remap(0)
+        L.fwiedu_set_checkpoint_bits(32)
+    pre_r, g_r = out[("tuned", 0)]
+    for key, (pre_k, g_k) in out.items():
+        assert np.array_equal(g_k, g_r), key
+        assert np.array_equal(recs(pre_k), recs(pre_r)), key
+    if bits == 32:
+        assert np.array_equal(g_r, g_o)
+        assert np.array_equal(recs(pre_r), recs(pre_o))
+
+
 def irregular_shots(w, sample_rate):
     """70 shots sharing one window (receivers in the first and last model
     column) with irregular receivers: some shots miss columns 10-19, some have
