@@ -374,13 +374,20 @@ void collect_timing()
 // untimed), the three work fields rotating as in a time loop: each launch
 // reads the field the previous one wrote.
 static float time_shape(int mode, FdArgs a, const FdTile &t, float *const f[3], int reps,
-                        hipEvent_t e0, hipEvent_t e1)
+                        hipEvent_t e0, hipEvent_t e1, bool inplace)
 {
     Ctx &c = ctx();
+    // three rotating fields, or (inplace) two, each step writing over p0
     auto go = [&](int r) {
-        a.cur = f[r % 3];
-        a.out = f[(r + 1) % 3];
-        a.prev = f[(r + 2) % 3];
+        if (inplace) {
+            a.cur = f[r % 2];
+            a.out = f[(r + 1) % 2];
+            a.prev = a.out;
+        } else {
+            a.cur = f[r % 3];
+            a.out = f[(r + 1) % 3];
+            a.prev = f[(r + 2) % 3];
+        }
         launch_fd(mode, a, t, c.st);
     };
     go(0);
@@ -398,9 +405,10 @@ static float time_shape(int mode, FdArgs a, const FdTile &t, float *const f[3], 
 // fastest (four at most) are then timed again, twice each in alternating
 // order over three times as many launches, and the best of those averages
 // wins, so that shapes within a percent of each other are not told apart by
-// one short sample.
+// one short sample.  inplace: timed as the batch runs its steps in place (the
+// fields f[0] and f[1] only).
 static FdTile fastest_shape(int mode, const FdArgs &a, const std::vector<FdTile> &cand,
-                            float *const f[3], bool once = false)
+                            float *const f[3], bool once, bool inplace)
 {
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
@@ -409,7 +417,7 @@ static FdTile fastest_shape(int mode, const FdArgs &a, const std::vector<FdTile>
     std::vector<std::pair<float, int>> ms;
     int reps = 9;
     for (size_t i = 0; i < cand.size(); ++i) {
-        const float t = time_shape(mode, a, cand[i], f, reps, e0, e1);
+        const float t = time_shape(mode, a, cand[i], f, reps, e0, e1, inplace);
         ms.push_back({t, (int)i});
         if (reps == 9 && t > 0.2f) reps = 3;
     }
@@ -426,7 +434,8 @@ static FdTile fastest_shape(int mode, const FdArgs &a, const std::vector<FdTile>
         for (int j = 0; j < top; ++j) {
             const int k = pass ? top - 1 - j : j;
             again[k] = std::min(again[k],
-                                time_shape(mode, a, cand[ms[k].second], f, 3 * reps, e0, e1));
+                                time_shape(mode, a, cand[ms[k].second], f, 3 * reps, e0, e1,
+                                           inplace));
         }
     int best = 0;
     for (int k = 1; k < top; ++k)
@@ -445,10 +454,10 @@ static FdTile fastest_shape(int mode, const FdArgs &a, const std::vector<FdTile>
 // bits, so reuse changes the speed only.  Similar: same stencil mode,
 // accumulation variant, layout, 64-lane groups (lanes) or shot-count bucket
 // (arenas: 1, 2-3, 4-7, 8 or more, the shots per block a shape may take), free
-// surface and arithmetic, and column count and stencil height within a
-// quarter of the tuned one's.
+// surface and arithmetic, steps in place or not, and column count and stencil
+// height within a quarter of the tuned one's.
 struct TunedClass {
-    int mode, accv, lanes, group, fs, fma, ncol, nrow;
+    int mode, accv, lanes, group, fs, fma, inplace, ncol, nrow;
     FdTile tile;
 };
 static std::mutex g_similar_mu;
@@ -458,11 +467,12 @@ static std::vector<TunedClass> g_similar;
 // per-shot-arena shape tuned for n shots may use.
 static int spb_bucket(int n) { return n >= 8 ? 8 : n >= 4 ? 4 : n >= 2 ? 2 : 1; }
 
-static TunedClass tuned_class(int mode, int accv, const FdArgs &a)
+static TunedClass tuned_class(int mode, int accv, const FdArgs &a, bool inplace)
 {
     TunedClass t;
     t.mode = mode;
     t.accv = accv;
+    t.inplace = inplace ? 1 : 0;
     t.lanes = a.cell_stride > 1 ? 1 : 0;
     t.group = t.lanes ? (a.nshot + 63) / 64 : spb_bucket(a.nshot);
     t.fs = a.fs;
@@ -480,7 +490,8 @@ static bool find_similar(const TunedClass &q, const FdArgs &base, FdTile &out)
     for (auto it = g_similar.rbegin(); it != g_similar.rend(); ++it) {
         const TunedClass &t = *it;
         if (t.mode != q.mode || t.accv != q.accv || t.lanes != q.lanes || t.group != q.group ||
-            t.fs != q.fs || t.fma != q.fma || !near(q.ncol, t.ncol) || !near(q.nrow, t.nrow))
+            t.fs != q.fs || t.fma != q.fma || t.inplace != q.inplace || !near(q.ncol, t.ncol) ||
+            !near(q.nrow, t.nrow))
             continue;
         if (t.tile.kind == 1 && march_lds_bytes(base, t.tile) > 65536) continue;
         out = t.tile;
@@ -497,7 +508,8 @@ static void remember_similar(TunedClass q, const FdTile &tile)
     g_similar.push_back(q);
 }
 
-FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur, const float *prev)
+FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur, const float *prev,
+                     bool inplace)
 {
     Ctx &c = ctx();
     const bool lanes = base.cell_stride > 1;
@@ -512,11 +524,11 @@ FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur,
         return c.tile_req;
     }
     const auto key = std::make_tuple(base.Nx, base.ld, base.ixs, base.ixe, base.izs, base.ize,
-                                     base.fs, (base.nshot * 2 + base.fma) * 4 + mode,
+                                     base.fs, ((base.nshot * 2 + base.fma) * 4 + mode) * 2 + inplace,
                                      base.cell_stride);
     auto it = c.tuned.find(key);
     if (it != c.tuned.end()) return it->second;
-    const TunedClass cls = tuned_class(mode, 0, base);
+    const TunedClass cls = tuned_class(mode, 0, base, inplace);
     FdTile sim;
     if (c.reuse_similar && find_similar(cls, base, sim)) {
         c.stats.shapes_reused++;
@@ -572,7 +584,7 @@ FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur,
             }
         }
         float *const f[3] = {out, const_cast<float *>(cur), const_cast<float *>(prev)};
-        const FdTile best = fastest_shape(mode, a, cand, f, true);
+        const FdTile best = fastest_shape(mode, a, cand, f, true, inplace);
         c.tuned[key] = best;
         remember_similar(cls, best);
         return best;
@@ -653,7 +665,7 @@ FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur,
                 cand.push_back(t);
             }
     float *const f[3] = {out, const_cast<float *>(cur), const_cast<float *>(prev)};
-    const FdTile best = fastest_shape(mode, a, cand, f);
+    const FdTile best = fastest_shape(mode, a, cand, f, false, inplace);
     c.tuned[key] = best;
     remember_similar(cls, best);
     return best;
@@ -669,18 +681,20 @@ FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur,
 // and the accumulator adds +0 terms to +0, so nothing a later launch reads
 // changes.
 FdTile autotune_acc_tile(int mode, int accv, const FdTile &mode_tile, const FdArgs &base,
-                         float *out, const float *cur, const float *prev)
+                         float *out, const float *cur, const float *prev, bool inplace)
 {
     Ctx &c = ctx();
     if (c.tile_req.tz > 0 || mode_tile.kind < 3 || mode == FD_RECON || !base.acc ||
         !base.accstage)
         return mode_tile;
     const auto key = std::make_tuple(base.Nx, base.ld, base.ixs, base.ixe, base.izs, base.ize,
-                                     base.fs, ((base.nshot * 2 + base.fma) * 4 + mode) * 4 + accv,
+                                     base.fs,
+                                     (((base.nshot * 2 + base.fma) * 4 + mode) * 4 + accv) * 2 +
+                                         inplace,
                                      base.cell_stride);
     auto it = c.tuned_acc.find(key);
     if (it != c.tuned_acc.end()) return it->second;
-    TunedClass cls = tuned_class(mode, accv, base);
+    TunedClass cls = tuned_class(mode, accv, base, inplace);
     cls.group = cls.group * 8 + mode_tile.kind;  // its candidates derive from the mode's shape
     FdTile sim;
     if (c.reuse_similar && find_similar(cls, base, sim)) {
@@ -723,8 +737,11 @@ FdTile autotune_acc_tile(int mode, int accv, const FdTile &mode_tile, const FdAr
             t.pd = pd;
             cand.push_back(t);
         }
+    // (timed in place, the forward field of the imaging condition is the
+    // third field, as the reconstruction's stored fields are apart from the
+    // adjoint's work fields)
     float *const f[3] = {out, const_cast<float *>(cur), const_cast<float *>(prev)};
-    const FdTile best = fastest_shape(mode, a, cand, f, c.quick_tune != 0);
+    const FdTile best = fastest_shape(mode, a, cand, f, c.quick_tune != 0, inplace);
     c.tuned_acc[key] = best;
     remember_similar(cls, best);
     return best;
@@ -1355,6 +1372,24 @@ static bool snap_step(const BatchState &b, int it)
     return b.vb && b.vb->snaps() && it % b.vb->snap_interval == 0;
 }
 
+// Time steps of the shots-in-lanes stencils may write the new field over p0:
+// every lanes stencil reads p0 only at the cell it updates, before it stores
+// that cell.  Only a work field is overwritten (never the zero field, a
+// checkpoint or a stored imaging field), and not when wavefield snapshots
+// are written.  It pays where the two work fields fit the 256 MB Infinity
+// Cache, which then holds them from one step to the next (the in-place
+// launches load p0 and store the output at the default cache policy); larger
+// batches keep three rotating fields streamed past the caches (221 x 601 with
+// 120 shots: 2 x 111.6 MB in place, 2 % off a gradient; 400 x 1200 with 256
+// shots, 2 x 671 MB, in place and at the default policy: 148 against 156-162
+// gradient shots/s).
+static constexpr size_t INFINITY_CACHE_BYTES = (size_t)256 << 20;
+static bool in_place(const BatchState &b)
+{
+    return ctx().inplace && b.lanes && !(b.vb && b.vb->snaps()) &&
+           (size_t)2 * b.KS * sizeof(float) <= INFINITY_CACHE_BYTES;
+}
+
 static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
                           const std::vector<Record2D *> &recs, int nt, bool adjoint, Schedule &sc,
                           bool lanes, bool ck16)
@@ -1461,7 +1496,7 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
             continue;
         }
         b.tile_m[m] = autotune_tile(m, base_args(g, coef, S, b.SS, b.CS), b.F(BatchPlan::A2),
-                                    b.F(BatchPlan::A0), b.F(BatchPlan::A1));
+                                    b.F(BatchPlan::A0), b.F(BatchPlan::A1), in_place(b));
         c.stats.tiles[m] = b.tile_m[m];
         FdArgs aa = base_args(g, coef, S, b.SS, b.CS);
         aa.acc = b.acc;
@@ -1472,7 +1507,8 @@ static void prepare_batch(BatchState &b, const Geom &g, const float *coef,
         b.tile_acc[m][0] = b.tile_m[m];
         for (int v = 1; v <= 2; ++v)
             b.tile_acc[m][v] = autotune_acc_tile(m, v, b.tile_m[m], aa, b.F(BatchPlan::A2),
-                                                 b.F(BatchPlan::A0), b.F(BatchPlan::A1));
+                                                 b.F(BatchPlan::A0), b.F(BatchPlan::A1),
+                                                 in_place(b));
         for (int v = 0; v < 3; ++v) c.stats.tiles_acc[m][v] = b.tile_acc[m][v];
     }
     b.tile = b.tile_m[adjoint ? FD_RECON : FD_FWD];
@@ -1521,24 +1557,6 @@ static void set_deferral(FdArgs &a, bool defer, int &staged, bool last)
 // propagation.c:539-608.  With save_ckpt the fields u(kd-1), u(kd) land in
 // their checkpoint slots; with precond the pseudo-Hessian accumulates in acc.
 // Kernel launches only, so a HIP graph can capture it.
-// Time steps of the shots-in-lanes stencils may write the new field over p0:
-// every lanes stencil reads p0 only at the cell it updates, before it stores
-// that cell.  Only a work field is overwritten (never the zero field, a
-// checkpoint or a stored imaging field), and not when wavefield snapshots
-// are written.  It pays where the two work fields fit the 256 MB Infinity
-// Cache, which then holds them from one step to the next (the in-place
-// launches load p0 and store the output at the default cache policy); larger
-// batches keep three rotating fields streamed past the caches (221 x 601 with
-// 120 shots: 2 x 111.6 MB in place, 2 % off a gradient; 400 x 1200 with 256
-// shots, 2 x 671 MB, in place and at the default policy: 148 against 156-162
-// gradient shots/s).
-static constexpr size_t INFINITY_CACHE_BYTES = (size_t)256 << 20;
-static bool in_place(const BatchState &b)
-{
-    return ctx().inplace && b.lanes && !(b.vb && b.vb->snaps()) &&
-           (size_t)2 * b.KS * sizeof(float) <= INFINITY_CACHE_BYTES;
-}
-
 static void forward_loop(BatchState &b, bool save_ckpt, bool precond)
 {
     Ctx &c = ctx();

@@ -159,9 +159,12 @@ void adjoint_source_device(const Record2D *r, float **vp_full, float freq, int f
 
 // Sampled timing of stencil launches.
 void fd_launch(int mode, const FdArgs &a, long cells, FdTile tile);
-FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur, const float *prev);
+// inplace: time the shapes as the batch runs its time steps, each writing the
+// new field over p0 (out == prev)
+FdTile autotune_tile(int mode, const FdArgs &base, float *out, const float *cur, const float *prev,
+                     bool inplace = false);
 FdTile autotune_acc_tile(int mode, int accv, const FdTile &mode_tile, const FdArgs &base,
-                         float *out, const float *cur, const float *prev);
+                         float *out, const float *cur, const float *prev, bool inplace = false);
 void collect_timing();
 void drop_graphs();
 
