@@ -1703,6 +1703,7 @@ static void adjoint_loop(BatchState &b)
     // are what its p1_fwd shows.
     const bool trim = !(b.vb && b.vb->snaps());
     const bool inpl = in_place(b);
+    const bool snap2 = inpl && b.tile_m[FD_RECON].kind == 4;
     for (int it = nt - 1; it >= 0; --it) {
         if (!stored[it] && (!trim || it % MST == 0)) {
             // get_acoustic2dCheckPointStatus: the segment restarts at the last
@@ -1718,9 +1719,23 @@ static void adjoint_loop(BatchState &b)
             }
             for (int itt = s0; itt < it; ++itt) {
                 const int t = itt + 1;
-                float *fout = (t % MST == 0)               ? b.F(ring0 + (t - s0) / MST)
-                              : (inpl && is_work(FW, fprev)) ? const_cast<float *>(fprev)
-                                                             : pick_free(FW, fcur, fprev);
+                // the field of an imaging step goes to its ring slot: as the
+                // launch's output, or (steps in place, register-ring stencil)
+                // as a second store of the model cells beside the in-place
+                // update of the work field, so that the next step stays in
+                // place as well
+                float *const ring = (t % MST == 0) ? b.F(ring0 + (t - s0) / MST) : nullptr;
+                const bool inw = inpl && is_work(FW, fprev);
+                float *fout;
+                fr.snap = nullptr;
+                if (ring && inw && snap2) {
+                    fout = const_cast<float *>(fprev);
+                    fr.snap = ring;
+                } else if (ring) {
+                    fout = ring;
+                } else {
+                    fout = inw ? const_cast<float *>(fprev) : pick_free(FW, fcur, fprev);
+                }
                 fr.out = fout;
                 fr.cur = fcur;
                 fr.prev = fprev;
@@ -1729,8 +1744,9 @@ static void adjoint_loop(BatchState &b)
                 fprev = fcur;
                 fcur = fout;
                 stored[t] = 1;
-                where[t] = fout;
+                where[t] = ring ? ring : fout;
             }
+            fr.snap = nullptr;
             fwd_seen = where[it];
             fwd_t = it;
         }

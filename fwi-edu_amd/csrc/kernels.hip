@@ -602,9 +602,11 @@ __device__ __forceinline__ void lput(frsrc f, long cell, long CS, unsigned voff,
 // accumulator becomes ((accv + t0) + t1 ...) + this step's term, the
 // reference's sequence of double additions; 3 always, writing this step's
 // term (a float: the reference adds its exact double image) to staging slot
-// a.acc_slot instead of the accumulator.
+// a.acc_slot instead of the accumulator.  Returns the value of the new field
+// at the cell (after source and free surface; before the adjoint injection
+// that belongs to the next step).
 template <int MODE, bool FMA, int EPI = 0, bool K8 = false>
-__device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long fsz, long colb,
+__device__ __forceinline__ float lanes_update(const FdArgs &a, cfloat *cf, long fsz, long colb,
                                              int ix, int iz, unsigned s, int scell, frsrc P2,
                                              frsrc U, long CS,
                                              float q0, float q1, float q2, float q3, float q4,
@@ -681,6 +683,7 @@ __device__ __forceinline__ void lanes_update(const FdArgs &a, cfloat *cf, long f
             }
         }
     }
+    return stored;
 }
 
 // Block -> (z-run, 64-shot group, column group) of the lanes stencils.
@@ -1207,8 +1210,13 @@ __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
     float V[U], P[U], H[U], Fu[UF ? U : 1];
     const frsrc Uc = col_rsrc(UF ? a.u : a.prev, colb * CS, UF ? fbytes - cb : 0);
     // staging: the column's place in the model ([nx][nz] cells x lanes), or
-    // none outside it
+    // none outside it.  In the reconstruction STG stores the new field's model
+    // cells a second time, into a.snap (a field of the same layout): the
+    // forward field that an imaging step of the adjoint reads, while the
+    // launch itself updates its work field in place
     const bool mcol = STG && active && ix >= a.pml && ix < a.pml + a.nx;
+    const frsrc Snc = col_rsrc(STG && MODE == FD_RECON ? a.snap : a.out, (colb - D) * CS,
+                               STG && MODE == FD_RECON ? fbytes - cb + (long)D * rs4 : 0);
     const long abase = STG ? (long)(ix - a.pml) * a.nz * a.acc_cell + s : 0;
     float *const stg = STG ? a.accstage + (long)a.acc_slot * a.accstage_stride : nullptr;
     const cfloat8v *c8 = (const cfloat8v *)(a.coef + CO_N * fsz);
@@ -1266,10 +1274,14 @@ __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
                 const float q6 = V[(u + 6) % U];
                 if (SPECIAL) {
                     const float k[7] = {kv[0], kv[1], kv[2], kv[3], kv[4], kv[5], kv[6]};
-                    lanes_update<MODE, false, STG ? 3 : 1, true>(
+                    const float sv = lanes_update<MODE, false, STG ? 3 : 1, true>(
                         a, cf, fsz, colb, ix, iz, s, scell, P2, P2, CS, q0, q1, q2, q3, q4, q5,
                         q6, xm1, xp1, xm2, xp2, xm3, xp3, P[u % U], k, 0.0,
                         UF ? Fu[u % U] : 0.f, MODE == FD_ADJ && iz == rr, injv, nullptr);
+                    const int pz = iz - a.pml;
+                    if (STG && MODE == FD_RECON && mcol && pz >= 0 && pz < a.nz)
+                        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, sv), Snc,
+                                                              vo, rl, CP_STREAM);
                 } else {
                     const float v = fd_update<false>(q3, P[u % U], q2, q4, q1, q5, q0, q6, xm1,
                                                      xp1, xm2, xp2, xm3, xp3, kv[0], kv[1], kv[2],
@@ -1278,7 +1290,10 @@ __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
                     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), P2c, vo, rl,
                                                           PCP);
                     const int pz = iz - a.pml;
-                    if (STG && mcol && pz >= 0 && pz < a.nz) {
+                    if (STG && MODE == FD_RECON && mcol && pz >= 0 && pz < a.nz) {
+                        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), Snc, vo,
+                                                              rl, CP_STREAM);
+                    } else if (STG && MODE != FD_RECON && mcol && pz >= 0 && pz < a.nz) {
                         // propagation.c:590 (forward) and :711 (adjoint), as
                         // lanes_update forms them
                         const float d2 = (v - 2.0f * q3 + P[u % U]) * a.invdtdt;
@@ -1372,6 +1387,11 @@ void launch_fd(int mode, const FdArgs &a0, FdTile tl, hipStream_t st)
 {
     FdArgs a = a0;
     a.remap = tl.remap;
+    if (a.snap && (mode != FD_RECON || tl.kind != 4)) {
+        fprintf(stderr, "<error>: internal: a second store of the new field is a reconstruction "
+                        "launch of the register-ring stencil only\n");
+        exit(1);
+    }
     const bool stage = mode != FD_RECON && a.acc_on && a.acc_stage && a.accstage;
     if (tl.kind == 4 && mode != FD_RECON && a.acc_on && !stage) {
         // the ring stencil stages accumulation terms but does not add them
@@ -1389,7 +1409,8 @@ void launch_fd(int mode, const FdArgs &a0, FdTile tl, hipStream_t st)
                 stage ? launch_ring_t<FD_FWD, true, 0>(a, tl, st)
                       : launch_ring_t<FD_FWD, false, 0>(a, tl, st);
             else if (mode == FD_RECON)
-                launch_ring_t<FD_RECON, false, 0>(a, tl, st);
+                a.snap ? launch_ring_t<FD_RECON, true, 0>(a, tl, st)
+                       : launch_ring_t<FD_RECON, false, 0>(a, tl, st);
             else
                 stage ? launch_ring_t<FD_ADJ, true, 0>(a, tl, st)
                       : launch_ring_t<FD_ADJ, false, 0>(a, tl, st);
@@ -1398,7 +1419,8 @@ void launch_fd(int mode, const FdArgs &a0, FdTile tl, hipStream_t st)
                 stage ? launch_ring_t<FD_FWD, true, CP_STREAM>(a, tl, st)
                       : launch_ring_t<FD_FWD, false, CP_STREAM>(a, tl, st);
             else if (mode == FD_RECON)
-                launch_ring_t<FD_RECON, false, CP_STREAM>(a, tl, st);
+                a.snap ? launch_ring_t<FD_RECON, true, CP_STREAM>(a, tl, st)
+                       : launch_ring_t<FD_RECON, false, CP_STREAM>(a, tl, st);
             else
                 stage ? launch_ring_t<FD_ADJ, true, CP_STREAM>(a, tl, st)
                       : launch_ring_t<FD_ADJ, false, CP_STREAM>(a, tl, st);

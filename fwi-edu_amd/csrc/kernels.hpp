@@ -109,6 +109,9 @@ struct FdArgs {
     // receivers only in the blocks whose rows meet them; whoever sets
     // src_cell or rrow sets these.
     int src_zlo, src_zhi, rec_zlo, rec_zhi;
+    // FD_RECON with the register-ring stencil: a field of the same layout
+    // that receives the new field's model cells as well (nullptr: none)
+    float *snap;
 };
 
 void launch_fd(int mode, const FdArgs &a, FdTile tile, hipStream_t st);
