@@ -66,7 +66,7 @@ def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
         times.append(time.perf_counter() - t0)
         log(f"step {i + 1}/{steps}: {times[-1] * 1e3:.1f} ms, misfit {misfit:.6g}")
     L.fwiedu_set_kernel_timing(0)
-    names = ["fd_launches", "fd_timed", "fd_ms_avg", "fd_cells_per_launch",
+    names = ["fd_launches", "fd_inplace", "fd_timed", "fd_ms_avg", "fd_cells_per_launch",
              "fd_shots_per_launch", "batch", "device_bytes", "tile_tz", "tile_spb", "tile_kind",
              "tile_zfast", "tile_remap", "tile_txm", "tile_pd", "tile_wpb", "tile_zrun", "layout"]
     names += [f"fd{m}_{k}" for m in range(3) for k in
@@ -83,45 +83,51 @@ def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
 
 # HBM bytes per launch of each stencil mode of the c1 gradient at 120 shots in
 # lanes (forward and adjoint: averaged over the gradient's launches, 3200 plain,
-# 600 staging and 201 adding launches of 4001), per stencil shape (kernel kind
-# 3: lanes with LDS row exchange; kind 4: the same with the register ring;
-# columns per block, rows per wave, XCD order, block order; the rows of loads
-# in flight change when a row's loads are issued, not which bytes are loaded):
-# FETCH_SIZE x 2 + WRITE_SIZE from rocprofv3 counter runs scaled by a 512 MiB
-# copy; the runs and the calibration are in profiles/fd_lanes_c1_traffic.md
-# (kind 3) and profiles/fd_ring_c1_traffic.md (kind 4).  Kind 4's forward and
-# adjoint figures other than 20 rows, XCD order, z-runs fastest were measured
-# while the staging launches ran the LDS kernel (that shape read 0.4-0.5 % less
-# then than with the staging in the ring stencil).
+# 600 staging and 201 adding launches of 4001; the reconstruction's over its
+# plain launches and those that store an imaging step's field a second time),
+# per stencil shape (kernel kind 3: lanes with LDS row exchange; kind 4: the
+# same with the register ring; columns per block, rows per wave, XCD order,
+# block order) and time steps in place (1) or on three rotating fields (0);
+# the rows of loads in flight change when a row's loads are issued, not which
+# bytes are loaded: FETCH_SIZE x 2 + WRITE_SIZE from rocprofv3 counter runs
+# scaled by a 512 MiB copy; the runs and the calibration are in
+# profiles/fd_lanes_c1_traffic.md (kind 3) and profiles/fd_ring_c1_traffic.md
+# (kind 4).  Kind 4's rotating-field forward and adjoint figures other than 20
+# rows, XCD order, z-runs fastest were measured while the staging launches ran
+# the LDS kernel (that shape read 0.4-0.5 % less then than with the staging
+# in the ring stencil).
 C1_LANES_TRAFFIC = {
-    "forward": {(3, 8, 16, 1, 0): 353.8e6, (3, 8, 16, 1, 2): 349.9e6, (3, 8, 16, 0, 0): 443.6e6,
-                (3, 8, 16, 0, 2): 443.7e6, (3, 8, 20, 1, 0): 360.0e6, (3, 8, 20, 1, 2): 348.2e6,
-                (3, 8, 20, 0, 0): 431.6e6, (3, 8, 20, 0, 2): 431.6e6, (3, 8, 24, 1, 0): 357.4e6,
-                (3, 8, 24, 1, 2): 347.8e6, (3, 8, 24, 0, 0): 426.0e6, (3, 8, 24, 0, 2): 425.9e6,
-                (3, 8, 32, 1, 0): 351.0e6, (3, 8, 32, 1, 2): 347.6e6, (3, 8, 32, 0, 0): 420.1e6,
-                (3, 8, 32, 0, 2): 420.2e6, (4, 8, 20, 1, 0): 364.5e6, (4, 8, 20, 1, 2): 347.8e6,
-                (4, 8, 20, 0, 0): 445.0e6, (4, 8, 20, 0, 2): 444.8e6, (4, 8, 30, 1, 0): 359.0e6,
-                (4, 8, 30, 1, 2): 346.9e6, (4, 8, 30, 0, 0): 429.5e6, (4, 8, 30, 0, 2): 429.5e6},
-    "reconstruction": {(3, 8, 16, 1, 0): 318.1e6, (3, 8, 16, 1, 2): 311.4e6,
-                       (3, 8, 16, 0, 0): 404.1e6, (3, 8, 16, 0, 2): 404.2e6,
-                       (3, 8, 20, 1, 0): 323.9e6, (3, 8, 20, 1, 2): 310.7e6,
-                       (3, 8, 20, 0, 0): 392.9e6, (3, 8, 20, 0, 2): 393.0e6,
-                       (3, 8, 24, 1, 0): 320.6e6, (3, 8, 24, 1, 2): 310.8e6,
-                       (3, 8, 24, 0, 0): 387.6e6, (3, 8, 24, 0, 2): 387.7e6,
-                       (3, 8, 32, 1, 0): 313.4e6, (3, 8, 32, 1, 2): 311.3e6,
-                       (3, 8, 32, 0, 0): 382.1e6, (3, 8, 32, 0, 2): 382.1e6,
-                       (4, 8, 20, 1, 0): 329.1e6, (4, 8, 20, 1, 2): 312.4e6,
-                       (4, 8, 20, 0, 0): 412.1e6, (4, 8, 20, 0, 2): 412.0e6,
-                       (4, 8, 30, 1, 0): 324.7e6, (4, 8, 30, 1, 2): 311.6e6,
-                       (4, 8, 30, 0, 0): 395.4e6, (4, 8, 30, 0, 2): 395.4e6},
-    "adjoint": {(3, 8, 16, 1, 0): 363.6e6, (3, 8, 16, 1, 2): 370.6e6, (3, 8, 16, 0, 0): 463.8e6,
-                (3, 8, 16, 0, 2): 463.9e6, (3, 8, 20, 1, 0): 374.1e6, (3, 8, 20, 1, 2): 366.5e6,
-                (3, 8, 20, 0, 0): 451.5e6, (3, 8, 20, 0, 2): 451.7e6, (3, 8, 24, 1, 0): 375.7e6,
-                (3, 8, 24, 1, 2): 365.6e6, (3, 8, 24, 0, 0): 446.0e6, (3, 8, 24, 0, 2): 445.8e6,
-                (3, 8, 32, 1, 0): 371.9e6, (3, 8, 32, 1, 2): 364.2e6, (3, 8, 32, 0, 0): 439.5e6,
-                (3, 8, 32, 0, 2): 439.7e6, (4, 8, 20, 1, 0): 385.4e6, (4, 8, 20, 1, 2): 368.0e6,
-                (4, 8, 20, 0, 0): 464.9e6, (4, 8, 20, 0, 2): 464.7e6, (4, 8, 30, 1, 0): 378.6e6,
-                (4, 8, 30, 1, 2): 366.7e6, (4, 8, 30, 0, 0): 449.1e6, (4, 8, 30, 0, 2): 449.0e6},
+    "forward": {(4, 8, 20, 1, 2, 1): 351.8e6,
+                (3, 8, 16, 1, 0, 0): 353.8e6, (3, 8, 16, 1, 2, 0): 349.9e6, (3, 8, 16, 0, 0, 0): 443.6e6,
+                (3, 8, 16, 0, 2, 0): 443.7e6, (3, 8, 20, 1, 0, 0): 360.0e6, (3, 8, 20, 1, 2, 0): 348.2e6,
+                (3, 8, 20, 0, 0, 0): 431.6e6, (3, 8, 20, 0, 2, 0): 431.6e6, (3, 8, 24, 1, 0, 0): 357.4e6,
+                (3, 8, 24, 1, 2, 0): 347.8e6, (3, 8, 24, 0, 0, 0): 426.0e6, (3, 8, 24, 0, 2, 0): 425.9e6,
+                (3, 8, 32, 1, 0, 0): 351.0e6, (3, 8, 32, 1, 2, 0): 347.6e6, (3, 8, 32, 0, 0, 0): 420.1e6,
+                (3, 8, 32, 0, 2, 0): 420.2e6, (4, 8, 20, 1, 0, 0): 364.5e6, (4, 8, 20, 1, 2, 0): 347.8e6,
+                (4, 8, 20, 0, 0, 0): 445.0e6, (4, 8, 20, 0, 2, 0): 444.8e6, (4, 8, 30, 1, 0, 0): 359.0e6,
+                (4, 8, 30, 1, 2, 0): 346.9e6, (4, 8, 30, 0, 0, 0): 429.5e6, (4, 8, 30, 0, 2, 0): 429.5e6},
+    "reconstruction": {(4, 8, 20, 1, 2, 1): 329.9e6,
+                       (3, 8, 16, 1, 0, 0): 318.1e6, (3, 8, 16, 1, 2, 0): 311.4e6,
+                       (3, 8, 16, 0, 0, 0): 404.1e6, (3, 8, 16, 0, 2, 0): 404.2e6,
+                       (3, 8, 20, 1, 0, 0): 323.9e6, (3, 8, 20, 1, 2, 0): 310.7e6,
+                       (3, 8, 20, 0, 0, 0): 392.9e6, (3, 8, 20, 0, 2, 0): 393.0e6,
+                       (3, 8, 24, 1, 0, 0): 320.6e6, (3, 8, 24, 1, 2, 0): 310.8e6,
+                       (3, 8, 24, 0, 0, 0): 387.6e6, (3, 8, 24, 0, 2, 0): 387.7e6,
+                       (3, 8, 32, 1, 0, 0): 313.4e6, (3, 8, 32, 1, 2, 0): 311.3e6,
+                       (3, 8, 32, 0, 0, 0): 382.1e6, (3, 8, 32, 0, 2, 0): 382.1e6,
+                       (4, 8, 20, 1, 0, 0): 329.1e6, (4, 8, 20, 1, 2, 0): 312.4e6,
+                       (4, 8, 20, 0, 0, 0): 412.1e6, (4, 8, 20, 0, 2, 0): 412.0e6,
+                       (4, 8, 30, 1, 0, 0): 324.7e6, (4, 8, 30, 1, 2, 0): 311.6e6,
+                       (4, 8, 30, 0, 0, 0): 395.4e6, (4, 8, 30, 0, 2, 0): 395.4e6},
+    "adjoint": {(4, 8, 20, 1, 2, 1): 372.4e6,
+                (3, 8, 16, 1, 0, 0): 363.6e6, (3, 8, 16, 1, 2, 0): 370.6e6, (3, 8, 16, 0, 0, 0): 463.8e6,
+                (3, 8, 16, 0, 2, 0): 463.9e6, (3, 8, 20, 1, 0, 0): 374.1e6, (3, 8, 20, 1, 2, 0): 366.5e6,
+                (3, 8, 20, 0, 0, 0): 451.5e6, (3, 8, 20, 0, 2, 0): 451.7e6, (3, 8, 24, 1, 0, 0): 375.7e6,
+                (3, 8, 24, 1, 2, 0): 365.6e6, (3, 8, 24, 0, 0, 0): 446.0e6, (3, 8, 24, 0, 2, 0): 445.8e6,
+                (3, 8, 32, 1, 0, 0): 371.9e6, (3, 8, 32, 1, 2, 0): 364.2e6, (3, 8, 32, 0, 0, 0): 439.5e6,
+                (3, 8, 32, 0, 2, 0): 439.7e6, (4, 8, 20, 1, 0, 0): 385.4e6, (4, 8, 20, 1, 2, 0): 368.0e6,
+                (4, 8, 20, 0, 0, 0): 464.9e6, (4, 8, 20, 0, 2, 0): 464.7e6, (4, 8, 30, 1, 0, 0): 378.6e6,
+                (4, 8, 30, 1, 2, 0): 366.7e6, (4, 8, 30, 0, 0, 0): 449.1e6, (4, 8, 30, 0, 2, 0): 449.0e6},
 }
 
 
@@ -131,7 +137,7 @@ def measured_traffic(w, st, mode):
     if w.name != "c1" or w.nshot != 120 or kind not in (3, 4):
         return None
     key = (kind, int(st[f"tile{m}_wpb"]), int(st[f"tile{m}_zrun"]), int(st[f"tile{m}_remap"]),
-           int(st[f"tile{m}_zfast"]))
+           int(st[f"tile{m}_zfast"]), 1 if st.get("fd_inplace", 0) > 0 else 0)
     return C1_LANES_TRAFFIC.get(mode, {}).get(key)
 
 
