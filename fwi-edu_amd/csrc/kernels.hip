@@ -604,8 +604,8 @@ __device__ __forceinline__ void lput(frsrc f, long cell, long CS, unsigned voff,
 // term (a float: the reference adds its exact double image) to staging slot
 // a.acc_slot instead of the accumulator.  Returns the value of the new field
 // at the cell (after source and free surface; before the adjoint injection
-// that belongs to the next step).
-template <int MODE, bool FMA, int EPI = 0, bool K8 = false>
+// that belongs to the next step).  OCP: the cache policy of the output store.
+template <int MODE, bool FMA, int EPI = 0, bool K8 = false, int OCP = CP_STREAM>
 __device__ __forceinline__ float lanes_update(const FdArgs &a, cfloat *cf, long fsz, long colb,
                                              int ix, int iz, unsigned s, int scell, frsrc P2,
                                              frsrc U, long CS,
@@ -651,11 +651,11 @@ __device__ __forceinline__ float lanes_update(const FdArgs &a, cfloat *cf, long 
         // separate injection would add it before that step's stencil
         // (propagation.c:681); the imaging below sees the field without it,
         // as it does there
-        lput<CP_STREAM>(
+        lput<OCP>(
             P2, i, CS, s * 4u,
             stored - injv * (K8 ? k[6] : cf[CO_K * fsz + i]) * a.dt * a.dt * a.invdx * a.invdz);
     } else {
-        lput<CP_STREAM>(P2, i, CS, s * 4u, stored);
+        lput<OCP>(P2, i, CS, s * 4u, stored);
     }
     if (MODE != FD_RECON && EPI != 1 && (EPI >= 2 || a.acc_on)) {
         const int px = ix - a.pml, pz = iz - a.pml;
@@ -839,7 +839,10 @@ static void launch_lanes_t(const FdArgs &a0, FdTile tl, hipStream_t st)
 typedef float float8v __attribute__((ext_vector_type(8)));
 typedef const __attribute__((address_space(4))) float8v cfloat8v;
 
-template <int MODE, int WPB, int PD, bool FMA, int ACC>
+// PCP: the cache policy of the p0 loads and the output stores (the
+// non-temporal policy; the default policy for a launch that writes over p0,
+// whose two fields the Infinity Cache may then keep)
+template <int MODE, int WPB, int PD, bool FMA, int ACC, int PCP = CP_STREAM>
 __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
 {
     constexpr int NC = WPB + 6;                    // columns of the LDS row
@@ -933,7 +936,8 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
     auto load_row = [&](int iz, int p) {
         const int ro = iz * rs4;
         pn[p] = ldb(P1, colb4 + ro + 3 * rs4);
-        pp[p] = ldb_stream(P0, colb4 + ro);
+        pp[p] = __builtin_bit_cast(float,
+                                   __builtin_amdgcn_raw_buffer_load_b32(P0, vo, colb4 + ro, PCP));
 #pragma unroll
         for (int k = 0; k < NH; ++k)
             if (k < nh) ph[p][k] = ldb(P1, hcol4[k] + ro);
@@ -1007,7 +1011,7 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
                     kv = c8[colb + iz];
                 }
                 const float k[7] = {kv[0], kv[1], kv[2], kv[3], kv[4], kv[5], kv[6]};
-                lanes_update<MODE, FMA, ACC == 2 ? 2 : (ACC == 1 ? 3 : 1), true>(
+                lanes_update<MODE, FMA, ACC == 2 ? 2 : (ACC == 1 ? 3 : 1), true, PCP>(
                     a, cf, fsz, colb, ix, iz, s, scell, P2, U, CS, q0, q1, q2, q3, q4, q5, q6,
                     xm1, xp1, xm2, xp2, xm3, xp3, p0v, k, pa[p], pu[p],
                     MODE == FD_ADJ && iz == rr, injv, pt[p]);
@@ -1023,17 +1027,17 @@ __global__ __launch_bounds__(WPB * 64) void fd_lanes_lds_kernel(FdArgs a)
     }
 }
 
-template <int MODE, int WPB, bool FMA, int ACC>
+template <int MODE, int WPB, bool FMA, int ACC, int PCP = CP_STREAM>
 static void launch_lanes_lds_p(const FdArgs &a, const dim3 &grid, int pd, hipStream_t st)
 {
     if (pd >= 4)
-        fd_go(fd_lanes_lds_kernel<MODE, WPB, 4, FMA, ACC>, grid, dim3(WPB * 64), 0, st, a);
+        fd_go(fd_lanes_lds_kernel<MODE, WPB, 4, FMA, ACC, PCP>, grid, dim3(WPB * 64), 0, st, a);
     else if (pd == 3)
-        fd_go(fd_lanes_lds_kernel<MODE, WPB, 3, FMA, ACC>, grid, dim3(WPB * 64), 0, st, a);
+        fd_go(fd_lanes_lds_kernel<MODE, WPB, 3, FMA, ACC, PCP>, grid, dim3(WPB * 64), 0, st, a);
     else if (pd == 2)
-        fd_go(fd_lanes_lds_kernel<MODE, WPB, 2, FMA, ACC>, grid, dim3(WPB * 64), 0, st, a);
+        fd_go(fd_lanes_lds_kernel<MODE, WPB, 2, FMA, ACC, PCP>, grid, dim3(WPB * 64), 0, st, a);
     else
-        fd_go(fd_lanes_lds_kernel<MODE, WPB, 1, FMA, ACC>, grid, dim3(WPB * 64), 0, st, a);
+        fd_go(fd_lanes_lds_kernel<MODE, WPB, 1, FMA, ACC, PCP>, grid, dim3(WPB * 64), 0, st, a);
 }
 
 // The accumulation epilogue is a template choice: the launches without it
@@ -1046,6 +1050,10 @@ static void launch_lanes_lds_f(const FdArgs &a, const dim3 &grid, int pd, hipStr
         launch_lanes_lds_p<MODE, WPB, FMA, 0>(a, grid, pd, st);
     else if (a.acc_stage)
         launch_lanes_lds_p<MODE, WPB, FMA, MODE == FD_RECON ? 0 : 1>(a, grid, pd, st);
+    else if (WPB == 8 && !FMA && a.out == a.prev)
+        // the launches adding the staged terms beside the register-ring
+        // stencil's steps in place (its block shape, strict arithmetic)
+        launch_lanes_lds_p<MODE, WPB, FMA, MODE == FD_RECON ? 0 : 2, 0>(a, grid, pd, st);
     else
         launch_lanes_lds_p<MODE, WPB, FMA, MODE == FD_RECON ? 0 : 2>(a, grid, pd, st);
 }
@@ -1208,13 +1216,17 @@ __global__ __launch_bounds__(512) void fd_ring_kernel(FdArgs a)
     // halo column) row iz0 + j in slot j mod U, and with UF the forward field
     // of the imaging condition (Fu) too
     float V[U], P[U], H[U], Fu[UF ? U : 1];
-    const frsrc Uc = col_rsrc(UF ? a.u : a.prev, colb * CS, UF ? fbytes - cb : 0);
     // staging: the column's place in the model ([nx][nz] cells x lanes), or
     // none outside it.  In the reconstruction STG stores the new field's model
     // cells a second time, into a.snap (a field of the same layout): the
     // forward field that an imaging step of the adjoint reads, while the
     // launch itself updates its work field in place
     const bool mcol = STG && active && ix >= a.pml && ix < a.pml + a.nx;
+    // the forward field is read where the imaging term is formed: model
+    // columns, in runs that meet the model's rows (elsewhere a zero-range
+    // resource: no request, value 0, never used)
+    const bool urun = UF && mcol && iz0 < a.pml + a.nz && iz1 >= a.pml;
+    const frsrc Uc = col_rsrc(UF ? a.u : a.prev, colb * CS, urun ? fbytes - cb : 0);
     const frsrc Snc = col_rsrc(STG && MODE == FD_RECON ? a.snap : a.out, (colb - D) * CS,
                                STG && MODE == FD_RECON ? fbytes - cb + (long)D * rs4 : 0);
     const long abase = STG ? (long)(ix - a.pml) * a.nz * a.acc_cell + s : 0;
