@@ -15,8 +15,9 @@ namespace fwiedu {
 
 enum FdMode { FD_FWD = 0, FD_RECON = 1, FD_ADJ = 2 };
 // accumulation steps staged (as float terms) before one pass over the FP64
-// accumulators adds them in order
-constexpr int ACC_STAGES = 3;
+// accumulators adds them in order (7: a pass every 8th accumulation step; 3,
+// every 4th, made a 221 x 601 gradient 0.8 % slower)
+constexpr int ACC_STAGES = 7;
 
 // Geometry of one padded simulation window.
 struct Geom {
