@@ -83,8 +83,10 @@ def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
 
 # HBM bytes per launch of each stencil mode of the c1 gradient at 120 shots in
 # lanes (forward and adjoint: averaged over the gradient's launches, 3200 plain,
-# 600 staging and 201 adding launches of 4001; the reconstruction's over its
-# plain launches and those that store an imaging step's field a second time),
+# 700 staging and 101 adding launches of 4001 with steps in place; 600 and 201
+# with three rotating fields, measured when three terms were staged per pass;
+# the reconstruction's over its plain launches and those that store an imaging
+# step's field a second time),
 # per stencil shape (kernel kind 3: lanes with LDS row exchange; kind 4: the
 # same with the register ring; columns per block, rows per wave, XCD order,
 # block order) and time steps in place (1) or on three rotating fields (0);
@@ -97,7 +99,7 @@ def timed_gradients(L, w, obs, steps, warmup, batch, ktime_every=0):
 # the LDS kernel (that shape read 0.4-0.5 % less then than with the staging
 # in the ring stencil).
 C1_LANES_TRAFFIC = {
-    "forward": {(4, 8, 20, 1, 2, 1): 351.8e6,
+    "forward": {(4, 8, 20, 1, 2, 1): 349.0e6,
                 (3, 8, 16, 1, 0, 0): 353.8e6, (3, 8, 16, 1, 2, 0): 349.9e6, (3, 8, 16, 0, 0, 0): 443.6e6,
                 (3, 8, 16, 0, 2, 0): 443.7e6, (3, 8, 20, 1, 0, 0): 360.0e6, (3, 8, 20, 1, 2, 0): 348.2e6,
                 (3, 8, 20, 0, 0, 0): 431.6e6, (3, 8, 20, 0, 2, 0): 431.6e6, (3, 8, 24, 1, 0, 0): 357.4e6,
@@ -106,7 +108,7 @@ C1_LANES_TRAFFIC = {
                 (3, 8, 32, 0, 2, 0): 420.2e6, (4, 8, 20, 1, 0, 0): 364.5e6, (4, 8, 20, 1, 2, 0): 347.8e6,
                 (4, 8, 20, 0, 0, 0): 445.0e6, (4, 8, 20, 0, 2, 0): 444.8e6, (4, 8, 30, 1, 0, 0): 359.0e6,
                 (4, 8, 30, 1, 2, 0): 346.9e6, (4, 8, 30, 0, 0, 0): 429.5e6, (4, 8, 30, 0, 2, 0): 429.5e6},
-    "reconstruction": {(4, 8, 20, 1, 2, 1): 329.9e6,
+    "reconstruction": {(4, 8, 20, 1, 2, 1): 330.1e6,
                        (3, 8, 16, 1, 0, 0): 318.1e6, (3, 8, 16, 1, 2, 0): 311.4e6,
                        (3, 8, 16, 0, 0, 0): 404.1e6, (3, 8, 16, 0, 2, 0): 404.2e6,
                        (3, 8, 20, 1, 0, 0): 323.9e6, (3, 8, 20, 1, 2, 0): 310.7e6,
@@ -119,7 +121,7 @@ C1_LANES_TRAFFIC = {
                        (4, 8, 20, 0, 0, 0): 412.1e6, (4, 8, 20, 0, 2, 0): 412.0e6,
                        (4, 8, 30, 1, 0, 0): 324.7e6, (4, 8, 30, 1, 2, 0): 311.6e6,
                        (4, 8, 30, 0, 0, 0): 395.4e6, (4, 8, 30, 0, 2, 0): 395.4e6},
-    "adjoint": {(4, 8, 20, 1, 2, 1): 372.4e6,
+    "adjoint": {(4, 8, 20, 1, 2, 1): 366.2e6,
                 (3, 8, 16, 1, 0, 0): 363.6e6, (3, 8, 16, 1, 2, 0): 370.6e6, (3, 8, 16, 0, 0, 0): 463.8e6,
                 (3, 8, 16, 0, 2, 0): 463.9e6, (3, 8, 20, 1, 0, 0): 374.1e6, (3, 8, 20, 1, 2, 0): 366.5e6,
                 (3, 8, 20, 0, 0, 0): 451.5e6, (3, 8, 20, 0, 2, 0): 451.7e6, (3, 8, 24, 1, 0, 0): 375.7e6,
