@@ -1380,9 +1380,9 @@ static bool snap_step(const BatchState &b, int it)
 // Cache, which then holds them from one step to the next (the in-place
 // launches load p0 and store the output at the default cache policy); larger
 // batches keep three rotating fields streamed past the caches (221 x 601 with
-// 120 shots: 2 x 111.6 MB in place, 2 % off a gradient; 400 x 1200 with 256
-// shots, 2 x 671 MB, in place and at the default policy: 148 against 156-162
-// gradient shots/s).
+// 120 shots: 2 x 111.6 MB in place, with the shapes timed that way, 6 % off a
+// gradient; 400 x 1200 with 256 shots, 2 x 671 MB, in place and at the default
+// policy: 148 against 156-162 gradient shots/s).
 static constexpr size_t INFINITY_CACHE_BYTES = (size_t)256 << 20;
 static bool in_place(const BatchState &b)
 {
@@ -1758,7 +1758,8 @@ static void adjoint_loop(BatchState &b)
             launch_inject(left ? b.tb.inj_left : b.tb.inj, const_cast<float *>(cur), b.SS, b.CS,
                           K, b.rec, it, g.dt, g.invdx, g.invdz, c.st);
         ad.inj_on = (fused && it >= 1) ? 1 : 0;
-        float *out = (inpl && is_work(AW, prev)) ? const_cast<float *>(prev) : pick_free(AW, cur, prev);
+        float *out =
+            (inpl && is_work(AW, prev)) ? const_cast<float *>(prev) : pick_free(AW, cur, prev);
         ad.out = out;
         ad.cur = cur;
         ad.prev = prev;
@@ -1812,8 +1813,8 @@ static void key_push_geom(std::vector<long> &k, const Geom &g)
 
 static std::vector<long> batch_key(int pass, const BatchState &b, bool f1, bool f2)
 {
-    std::vector<long> k = {pass, (long)f1, (long)f2, (long)ctx().fma, (long)in_place(b), (long)b.nt, (long)b.S, (long)b.Sp,
-                           (long)b.lanes, (long)b.ck16, b.sc ? (long)b.sc->d : 0L,
+    std::vector<long> k = {pass, (long)f1, (long)f2, (long)ctx().fma, (long)in_place(b),
+                           (long)b.nt, (long)b.S, (long)b.Sp, (long)b.lanes, (long)b.ck16, b.sc ? (long)b.sc->d : 0L,
                            b.tb.rcv.n, b.tb.inj.n, (long)b.tb.nci, b.nslot, b.tb.rcv_left.n,
                            b.tb.inj_left.n};
     key_push_geom(k, b.g);
