@@ -1116,11 +1116,15 @@ static void launch_lanes_lds_t(const FdArgs &a0, FdTile tl, hipStream_t st)
 //    fd_lanes_lds_kernel does.  Each branch issues its own first loads, so
 //    that no value in flight is moved between registers where they join;
 //  * STG (forward and adjoint launches that stage their accumulation term,
-//    every 5th step three times in four): the cell's pseudo-Hessian or
+//    every 5th step seven times in eight): the cell's pseudo-Hessian or
 //    imaging term goes to the launch's staging slot (non-temporal), the
 //    adjoint's forward field coming through one more ring of the wave's
 //    column, loaded with the rows; the launches that add the staged terms to
-//    the FP64 accumulator run fd_lanes_lds_kernel.
+//    the FP64 accumulator run fd_lanes_lds_kernel.  In the reconstruction
+//    STG stores the new field's model cells into a.snap as well;
+//  * PCP, the cache policy of the p0 loads and the output stores:
+//    non-temporal for a launch that writes a third field, the default policy
+//    for one that writes over p0.
 // Rows past a run's end (the last group of a run) still load, publish and
 // pass the barrier, computing nothing.
 template <int D>
