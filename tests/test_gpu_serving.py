@@ -30,9 +30,11 @@ def L():
     return lib
 
 
-# (problem, nz, nx, nt, shots): uneven grids, both problems, 1-12 shots
+# (problem, nz, nx, nt, shots): uneven grids, both problems, 1-64 shots (the
+# 50- and 64-shot jobs run shots in lanes, their time steps in place)
 MIX = [(0, 60, 90, 400, 3), (1, 48, 70, 300, 4), (0, 100, 64, 500, 12), (1, 72, 120, 350, 2),
-       (0, 40, 40, 300, 1), (1, 90, 60, 420, 5), (0, 128, 150, 300, 6), (1, 64, 64, 260, 1)]
+       (0, 40, 40, 300, 1), (1, 90, 60, 420, 5), (0, 128, 150, 300, 6), (1, 64, 64, 260, 1),
+       (1, 56, 80, 300, 50), (0, 64, 96, 280, 64)]
 
 
 def alone(L, job):
@@ -67,7 +69,11 @@ def test_jobs_same_alone_and_in_the_mix(L, bits):
     L.fwiedu_set_checkpoint_bits(bits)
     try:
         for j in jobs:
+            L.fwiedu_reset_stats()
             rec, g, mis = alone(L, j)
+            if j.w.nshot >= 48:
+                # shots in lanes, the time steps in place
+                assert flib.stat("layout") == 1 and flib.stat("fd_inplace") > 0, j.w.name
             assert np.array_equal(j.traces(), rec), j.w.name
             if j.problem == 1:
                 assert np.array_equal(j.grad.a, g), j.w.name
