@@ -1582,9 +1582,9 @@ static void forward_loop(BatchState &b, bool save_ckpt, bool precond)
         a.rec_on = 1;
     }
     const RcvTable &rt = fused ? b.tb.rcv_left : b.tb.rcv;
-    // the LDS stencil stages the pseudo-Hessian terms of ACC_STAGES steps and
-    // adds them with the next one's (one pass over the FP64 accumulator
-    // instead of ACC_STAGES + 1)
+    // the LDS and register-ring stencils stage the pseudo-Hessian terms of
+    // ACC_STAGES steps and the LDS stencil adds them with the next one's (one
+    // pass over the FP64 accumulator instead of ACC_STAGES + 1)
     const bool defer = precond && b.accstage && b.tile_m[FD_FWD].kind >= 3;
     a.accstage = b.accstage;
     a.accstage_stride = b.accstage_stride;
